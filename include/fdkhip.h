@@ -1,0 +1,176 @@
+/*
+ * fdkhip.h -- C ABI of the MI355X (gfx950) FDK cone-beam backprojector.
+ *
+ * Drop-in for the backprojection that MIRT's fbp/cbct_back.m hands to its
+ * `back_call` (default fdk_mex_call, fbp/cbct_back.m:274-283):
+ *
+ *   img = fdk_mex('fdk,ts,back', int32([nx ny nz]), [dx dy dz], ...
+ *                 [offset_x offset_y offset_z], uint8(mask), ...
+ *                 dso, dsd, dfs, [ds dt], [offset_s offset_t], ...
+ *                 proj, double(betas), int32(nthread));
+ *
+ * Conventions are those of the reference mex gateway
+ * (mex/src/fdk/fdk-orig/fdk,mex.c:22-171) and kernel
+ * (mex/src/fdk-helix/mex/fdk,ts.c:55-251):
+ *   - proj  is [nt ns na], t (detector row) fastest, FP32;
+ *   - image is [nz nx ny], z fastest ("zxy trick"), FP32, overwritten;
+ *   - mask  is [nx ny], x fastest, uint8, 0 = column not reconstructed;
+ *   - beta  is [na] source angles in radians (double);
+ *   - dfs = 0 selects the arc (3rd generation) detector, dfs = inf a flat
+ *     one, dso or dsd = inf the parallel-beam case; any other dfs is treated
+ *     as flat with a warning, like the reference ("dfs not done - junk!");
+ *   - dz/dt < 0 is an error, like the reference ("todo: dz/dt < 0 not done").
+ * The vertical interpolation uses the debugged weights of
+ * mex/src/fdk-helix/mex/fdk,ts.c:203 (p0 = wd*p1 + wu*p2), which the MATLAB
+ * path also uses (fbp/cbct_back.m:251).
+ *
+ * Every function returns FDKHIP_OK (0) or a negative FDKHIP_E* code; the
+ * message of the last failure on the calling thread is fdkhip_last_error().
+ * Calls are synchronous with respect to the host unless they take a stream,
+ * in which case they are asynchronous on that stream (a hipStream_t passed as
+ * void*; NULL = the null stream).
+ */
+#ifndef FDKHIP_H
+#define FDKHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FDKHIP_OK            0
+#define FDKHIP_EARG         -1  /* bad argument (reference: Fail in fdk_mex_back, fdk,mex.c:39-90) */
+#define FDKHIP_EGEOM        -2  /* "todo: dz/dt < 0 not done" (fdk,ts.c:153-154) */
+#define FDKHIP_ETHREADS     -3  /* "too many threads" (cbct,mask2.c:738) */
+#define FDKHIP_EHIP         -4  /* HIP runtime error */
+#define FDKHIP_ENOMEM       -5  /* device or host allocation failed */
+#define FDKHIP_EUNSUPPORTED -6  /* valid request this build does not serve */
+
+/* storage format of filtered projections on the device */
+#define FDKHIP_F32  0
+#define FDKHIP_F16  1  /* IEEE binary16 with a per-view FP32 scale */
+#define FDKHIP_BF16 2  /* bfloat16 with a per-view FP32 scale */
+
+/* arithmetic mode of the FP32 path */
+#define FDKHIP_MATH_EXACT 0  /* reference operation order, no FMA: bit-for-bit */
+#define FDKHIP_MATH_FAST  1  /* FMA and folded weights: ~1e-7 relative L2 */
+
+/* Scan geometry: the operands of 'fdk,ts,back' other than proj/beta/mask.
+ * Field meaning as cbct_ig / cbct_cg (mex/src/fdk/fdk-orig/cbct,def.h:17-43),
+ * kept in double on the host; the device narrows to float exactly where the
+ * reference gateway does (fdk,mex.c:119-137). */
+typedef struct fdkhip_geom {
+	int32_t nx, ny, nz;              /* image size */
+	double dx, dy, dz;               /* voxel size; dy may be < 0 */
+	double offset_x, offset_y, offset_z; /* image center offsets [pixels] */
+	double dso, dsd, dfs;            /* distances; dfs 0 = arc, inf = flat */
+	int32_t ns, nt;                  /* detector columns, rows */
+	double ds, dt;                   /* detector spacing */
+	double offset_s, offset_t;       /* detector offsets [pixels] */
+} fdkhip_geom;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *fdkhip_last_error(void);
+
+/* Library version string. */
+const char *fdkhip_version(void);
+
+/*
+ * 'fdk,ts,back' on host memory: the compute of fdk_mex_back
+ * (mex/src/fdk/fdk-orig/fdk,mex.c:22-171) and fdk_ts_back_t
+ * (mex/src/fdk-helix/mex/fdk,ts,t.c:107-167), with the same 12 operands.
+ * proj [nt ns na] FP32 and image [nz nx ny] FP32 are host pointers; mask may
+ * be NULL (all ones).  nthread is accepted for signature compatibility and
+ * checked as the reference does (1..250); the GPU ignores it.
+ * Copies proj to the device, backprojects on the current HIP device, copies
+ * the image back.  Synchronous.
+ */
+int fdkhip_fdk_ts_back(
+	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
+	const uint8_t *mask,
+	double dso, double dsd, double dfs,
+	const double dst[2], const double ost[2],
+	const float *proj, int32_t ns, int32_t nt, int32_t na,
+	const double *beta, int32_t nthread,
+	float *image);
+
+/*
+ * 'fdk,st,back' on host memory (use_mex = 3 in fbp/cbct_back.m:121-127):
+ * fdk_st_back1 (mex/src/fdk/fdk-orig/fdk,st.c:38-141) over all views, proj
+ * [ns nt na] (s fastest), image [nx ny nz] (x fastest), no zero padding,
+ * edge samples skipped and vertical weights as in that file.  Synchronous.
+ */
+int fdkhip_fdk_st_back(
+	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
+	const uint8_t *mask,
+	double dso, double dsd, double dfs,
+	const double dst[2], const double ost[2],
+	const float *proj, int32_t ns, int32_t nt, int32_t na,
+	const double *beta, int32_t nthread,
+	float *image);
+
+/*
+ * Device-resident 'fdk,ts,back': a plan holds the per-view tables, the
+ * device mask and the launch configuration for one geometry and one view
+ * list, so repeated backprojections of new projections cost one launch
+ * sequence (replayed from a HIP graph after the first call).  No counterpart
+ * object exists in the reference; the arithmetic is fdk_ts_back_t's.
+ *
+ * fmt:  FDKHIP_F32 / F16 / BF16 storage of proj in fdkhip_plan_execute.
+ * math: FDKHIP_MATH_EXACT or FDKHIP_MATH_FAST (F16/BF16 always use FAST).
+ * mask_host may be NULL (all ones).  beta is copied.
+ */
+typedef struct fdkhip_plan fdkhip_plan;
+
+int fdkhip_plan_create(fdkhip_plan **plan, const fdkhip_geom *g,
+	int32_t na, const double *beta, const uint8_t *mask_host,
+	int32_t fmt, int32_t math);
+
+/* Backproject device projections proj_dev ([nt ns na], plan's fmt) into the
+ * device image image_dev ([nz nx ny] FP32, fully overwritten).  scale_dev is
+ * the [na] FP32 per-view scale for F16/BF16 (ignored for F32).  Asynchronous
+ * on `stream`.  Calls that share a plan must not overlap in time. */
+int fdkhip_plan_execute(fdkhip_plan *plan, const void *proj_dev,
+	const float *scale_dev, float *image_dev, void *stream);
+
+/* Same as fdkhip_plan_execute but always launches directly (no graph). */
+int fdkhip_plan_execute_direct(fdkhip_plan *plan, const void *proj_dev,
+	const float *scale_dev, float *image_dev, void *stream);
+
+/* Launch configuration chosen for the plan (for reports and the tuner):
+ * cfg[0] = waves per workgroup, cfg[1] = slices per lane, cfg[2] = views per
+ * LDS stage, cfg[3] = z origin, cfg[4] = z groups, cfg[5] = workgroups. */
+int fdkhip_plan_config(const fdkhip_plan *plan, int32_t cfg[6]);
+
+/* Override the launch configuration: waves per workgroup and slices per lane
+ * must be a pair the build supports; vstage (views per LDS stage) and
+ * slot_cap (largest staged footprint, floats) are upper bounds, 0 = chosen
+ * automatically.  Views whose footprint exceeds the slot gather from HBM.
+ * fdkhip_plan_config shows what was taken. */
+int fdkhip_plan_set_config(fdkhip_plan *plan, int32_t waves, int32_t zper,
+	int32_t vstage, int32_t slot_cap);
+
+int fdkhip_plan_destroy(fdkhip_plan *plan);
+
+/* Number of (voxel, view) updates the reference loop performs for this plan
+ * (sum over in-mask, in-FOV columns of iz_max - iz_min, fdk,ts.c:159-188),
+ * computed on the device.  Synchronous. */
+int fdkhip_plan_count_updates(fdkhip_plan *plan, int64_t *n_updates);
+
+/*
+ * Per-view quantisation of FP32 projections on the device, the storage step
+ * for the F16/BF16 paths: scale[ia] = 2^e with e the smallest integer such
+ * that max|proj[:,:,ia]| / 2^e <= 2^15 (so the scaling is exact), then
+ * out = round_to_nearest_even(proj / scale) in the chosen format.
+ * All pointers are device pointers.  Asynchronous on `stream`.
+ */
+int fdkhip_quantize_views(const float *proj_dev, int32_t ns, int32_t nt,
+	int32_t na, int32_t fmt, void *out_dev, float *scale_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDKHIP_H */

@@ -1,0 +1,365 @@
+"""Cone-beam FDK backprojection: ``cbct_back`` and the ``fdk_mex`` gateway.
+
+Host-side mirror of the reference interface for this path:
+
+* :func:`fdk_mex` -- the mex gateway ``fdk_mex('fdk,ts,back' | 'fdk,st,back',
+  nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj, beta, nthread)``
+  (``mex/src/fdk/fdk-orig/fdk,mex.c:22-227``): same operands, same checks and
+  messages, arrays in MATLAB column-major index order; computes on the GPU
+  through ``libfdkhip.so``.
+* :func:`cbct_back` -- ``fbp/cbct_back.m:1-137``: builds the view angles,
+  casts to single, replicates edge rows (``extrapolate_t``), permutes
+  ``[ns nt na] -> [nt ns na]``, calls ``back_call`` and scales by the
+  angular step.
+* :class:`FdkPlan` -- device-resident backprojection of torch tensors for
+  repeated use (one plan per geometry; projections FP32, FP16 or BF16 with a
+  per-view scale).
+
+There is no CPU fallback anywhere in this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import FdkhipError, Geom
+from .geom import CTGeom, ImageGeom, deg2rad
+
+
+class FdkMexError(ValueError):
+    """Argument error raised where the reference gateway calls Fail."""
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _is_int32(a):
+    return isinstance(a, np.ndarray) and a.dtype == np.int32
+
+
+def _is_real_double(a):
+    return isinstance(a, np.ndarray) and a.dtype == np.float64
+
+
+def _scalar_double(x, name):
+    if isinstance(x, np.ndarray):
+        if x.dtype != np.float64 or x.size != 1:
+            raise FdkMexError(f"{name} must be a double scalar")
+        return float(x.reshape(-1)[0])
+    if isinstance(x, (float, int)) and not isinstance(x, bool):
+        return float(x)
+    raise FdkMexError(f"{name} must be a double scalar")
+
+
+def fdk_mex(cmd, nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj, beta,
+            nthread, *extra):
+    """Mirror of the mex gateway ``fdk_mex`` (fdk,mex.c:175-214, 22-171).
+
+    ``cmd`` is ``'fdk,ts,back'`` (proj ``[nt ns na]``, result ``[nz nx ny]``)
+    or ``'fdk,st,back'`` (proj ``[ns nt na]``, result ``[nx ny nz]``).
+    ``nxyz`` int32[3], ``dxyz``/``cxyz`` float64[3], ``mask`` uint8 ``[nx ny]``,
+    ``dst``/``ost`` float64[2], ``proj`` float32 2-D or 3-D, ``beta`` float64
+    (scalar for 2-D proj), ``nthread`` int32 scalar.  Returns a Fortran-order
+    float32 array.
+    """
+    if not isinstance(cmd, str):
+        raise FdkMexError("usage error. see above")
+    if extra or cmd not in ("fdk,ts,back", "fdk,st,back"):
+        if cmd not in ("fdk,ts,back", "fdk,st,back"):
+            raise FdkMexError(f"command '{cmd}' unknown")
+        raise FdkMexError("usage error. see above")   # nrhs != 13
+    is_st = cmd.startswith("fdk,st")
+
+    if not _is_int32(nxyz):
+        raise FdkMexError("mxIsInt32")
+    if nxyz.size != 3:
+        raise FdkMexError("mx_nxyz must have 3 elements")
+    nx, ny, nz = (int(v) for v in nxyz.reshape(-1))
+    if not _is_real_double(dxyz):
+        raise FdkMexError("mxIsRealDouble")
+    if dxyz.size != 3:
+        raise FdkMexError("mx_dxyz must have 3 elements")
+    if not _is_real_double(cxyz):
+        raise FdkMexError("mxIsRealDouble")
+    if cxyz.size != 3:
+        raise FdkMexError("mx_cxyz must have 3 elements")
+    if not (isinstance(mask, np.ndarray) and mask.dtype == np.uint8):
+        raise FdkMexError("mxIsUint8")
+    if mask.ndim != 2:
+        raise FdkMexError("mask must be 2d")
+    if mask.shape != (nx, ny):
+        raise FdkMexError("mask size not nx by ny?")
+    dso = _scalar_double(dso, "dso")
+    dsd = _scalar_double(dsd, "dsd")
+    dfs = _scalar_double(dfs, "dfs")
+    if not _is_real_double(dst):
+        raise FdkMexError("mxIsRealDouble")
+    if dst.size != 2:
+        raise FdkMexError("mx_dst must have 2 elements")
+    if not _is_real_double(ost):
+        raise FdkMexError("mxIsRealDouble")
+    if ost.size != 2:
+        raise FdkMexError("mx_ost must have 2 elements")
+    if not (isinstance(proj, np.ndarray) and proj.dtype == np.float32):
+        raise FdkMexError("mxIsRealSingle")
+    if proj.ndim == 2:
+        na = 1
+        beta = np.asarray([_scalar_double(beta, "beta")], dtype=np.float64)
+    elif proj.ndim == 3:
+        na = proj.shape[2]
+        if not _is_real_double(beta):
+            raise FdkMexError("mxIsRealDouble")
+        if beta.size != na:
+            raise FdkMexError(f"beta length={beta.size} vs na=size(proj,3)={na}")
+    else:
+        raise FdkMexError("mx_proj must be 2d or 3d")
+    if not (isinstance(nthread, np.ndarray) and nthread.dtype == np.int32
+            and nthread.size == 1):
+        raise FdkMexError("mxIsScalarInt32")
+    nthread = int(nthread.reshape(-1)[0])
+
+    if is_st:
+        ns, nt = proj.shape[0], proj.shape[1]
+        out = np.zeros((nx, ny, nz), dtype=np.float32, order="F")
+    else:
+        nt, ns = proj.shape[0], proj.shape[1]
+        out = np.zeros((nz, nx, ny), dtype=np.float32, order="F")
+
+    lib = _lib.load()
+    projf = np.asfortranarray(proj)
+    maskf = np.asfortranarray(mask)
+    nxyz_c = np.ascontiguousarray(nxyz.reshape(-1), dtype=np.int32)
+    dxyz_c = np.ascontiguousarray(dxyz.reshape(-1))
+    cxyz_c = np.ascontiguousarray(cxyz.reshape(-1))
+    dst_c = np.ascontiguousarray(dst.reshape(-1))
+    ost_c = np.ascontiguousarray(ost.reshape(-1))
+    beta_c = np.ascontiguousarray(np.asarray(beta, np.float64).reshape(-1))
+    fn = lib.fdkhip_fdk_st_back if is_st else lib.fdkhip_fdk_ts_back
+    rc = fn(_ptr(nxyz_c), _ptr(dxyz_c), _ptr(cxyz_c), _ptr(maskf), dso, dsd, dfs,
+            _ptr(dst_c), _ptr(ost_c), _ptr(projf), ns, nt, na, _ptr(beta_c),
+            nthread, _ptr(out))
+    _lib.check(rc)
+    return out
+
+
+def _views(cg: CTGeom, na: int):
+    """betas of cbct_back.m:82: deg2rad(orbit_start + orbit * [0:na-1] / na)."""
+    k = np.arange(na, dtype=np.float64)
+    return deg2rad(np.asarray(cg.orbit_start, dtype=np.float64) + cg.orbit * k / na)
+
+
+def angle_scale(cg: CTGeom, ia_skip: int = 1) -> np.float32:
+    """cbct_back.m:133-135: (0.5 * deg2rad(|orbit|) / (na/ia_skip)), in single."""
+    return np.float32(0.5 * float(deg2rad(abs(cg.orbit))) / (cg.na / ia_skip))
+
+
+def cbct_back(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: int = 1,
+              offset_source: float = 0.0, nthread=None, back_call=None,
+              scale_dang: bool = True, extrapolate_t: int = 0):
+    """Mirror of ``cbct_back`` (fbp/cbct_back.m:1-137).
+
+    ``proj`` is ``[ns nt na]`` (numpy, MATLAB index order); returns the
+    backprojection ``[nx ny nz]`` as a float32 numpy array.  ``use_mex`` 1
+    (default) calls ``back_call`` once with all views, 2 once per view and
+    sums, 3 uses the ``'fdk,st,back'`` order; ``back_call`` defaults to
+    :func:`fdk_mex` (the GPU gateway).  ``use_mex=0`` (the reference's MATLAB
+    backprojector ``cbct_back_mat``) is not provided by this build.
+    """
+    if use_mex is None:
+        use_mex = 1
+    if nthread is None:
+        nthread = 1
+    if back_call is None:
+        back_call = fdk_mex
+    if not use_mex:
+        raise NotImplementedError(
+            "use_mex=0 (cbct_back_mat, the MATLAB backprojector) is not provided; "
+            "use_mex=1 computes the same backprojection on the GPU")
+    if cg.pitch != 0 or np.any(cg.source_zs != 0):
+        raise ValueError("helix not yet supported")
+    del offset_source   # accepted as in cbct_back.m; the mex path ignores it
+
+    mask = ig.mask_or
+    nx, ny = mask.shape
+    nz = ig.nz
+    ns, nt, na = cg.ns, cg.nt, cg.na
+    ia_list = np.arange(0, na, ia_skip)
+    betas = _views(cg, na)
+    proj = np.asarray(proj, dtype=np.float32)
+    if proj.ndim == 2:
+        proj = proj[:, :, None]
+    if proj.shape != (ns, nt, na):
+        raise ValueError(f"proj shape {proj.shape} != [ns nt na] = {(ns, nt, na)}")
+    nthread = np.array([nthread], dtype=np.int32)
+    offset_t = cg.offset_t
+    if extrapolate_t > 0:
+        e = int(extrapolate_t)
+        top = np.repeat(proj[:, :1, :], e, axis=1)
+        bot = np.repeat(proj[:, -1:, :], e, axis=1)
+        proj = np.concatenate([top, proj, bot], axis=1)
+        nt = nt + 2 * e
+        if offset_t != 0:
+            raise ValueError("offset_t not yet done for extrapolation")
+
+    args = (np.array([nx, ny, nz], dtype=np.int32),
+            np.array([ig.dx, ig.dy, ig.dz], dtype=np.float64),
+            np.array([ig.offset_x, ig.offset_y, ig.offset_z], dtype=np.float64),
+            mask.astype(np.uint8), float(cg.dso), float(cg.dsd), float(cg.dfs),
+            np.array([cg.ds, cg.dt], dtype=np.float64),
+            np.array([cg.offset_s, offset_t], dtype=np.float64))
+    if use_mex == 1:
+        ptsa = np.asfortranarray(np.transpose(proj, (1, 0, 2)))   # sta -> tsa
+        img = back_call("fdk,ts,back", *args, np.asfortranarray(ptsa[:, :, ia_list]),
+                        np.ascontiguousarray(betas[ia_list]), nthread)
+        img = np.asfortranarray(np.transpose(img, (1, 2, 0)))    # zxy -> xyz
+    elif use_mex == 2:
+        ptsa = np.asfortranarray(np.transpose(proj, (1, 0, 2)))
+        img = np.float32(0)
+        for ia in ia_list:
+            tmp = back_call("fdk,ts,back", *args, np.asfortranarray(ptsa[:, :, ia]),
+                            float(betas[ia]), nthread)
+            img = img + tmp
+        img = np.asfortranarray(np.transpose(img, (1, 2, 0)))
+    elif use_mex == 3:
+        img = back_call("fdk,st,back", *args, np.asfortranarray(proj[:, :, ia_list]),
+                        np.ascontiguousarray(betas[ia_list]), nthread)
+    else:
+        raise ValueError("bug")
+    if scale_dang:
+        img = angle_scale(cg, ia_skip) * img
+    return img
+
+
+def geom_of(cg: CTGeom, ig: ImageGeom, extrapolate_t: int = 0) -> Geom:
+    """fdkhip_geom of a (ct_geom, image_geom) pair as cbct_back passes it."""
+    return Geom(ig.nx, ig.ny, ig.nz, ig.dx, ig.dy, ig.dz,
+                ig.offset_x, ig.offset_y, ig.offset_z,
+                float(cg.dso), float(cg.dsd), float(cg.dfs),
+                cg.ns, cg.nt + 2 * int(extrapolate_t), float(cg.ds), float(cg.dt),
+                float(cg.offset_s), float(cg.offset_t))
+
+
+class FdkPlan:
+    """Device-resident 'fdk,ts,back' for one geometry (libfdkhip plan).
+
+    ``proj`` passed to :meth:`__call__` is a CUDA (HIP) torch tensor holding
+    ``[nt ns na]`` with t fastest, i.e. a contiguous tensor of shape
+    ``(na, ns, nt)``, in the plan's storage format (``float32``, ``float16``
+    or ``bfloat16`` with a ``(na,)`` float32 ``scale``).  The result is the
+    ``[nz nx ny]`` image as a contiguous ``(ny, nx, nz)`` float32 tensor;
+    ``image_xyz`` views it as ``[nx ny nz]`` without copying.
+    """
+
+    def __init__(self, geom: Geom, beta, mask=None, *, fmt="f32", math_mode="exact"):
+        self.lib = _lib.load()
+        self.geom = geom
+        self.fmt = _lib.FMT_NAMES[fmt] if isinstance(fmt, str) else int(fmt)
+        self.math = {"exact": _lib.MATH_EXACT, "fast": _lib.MATH_FAST}[math_mode] \
+            if isinstance(math_mode, str) else int(math_mode)
+        beta = np.ascontiguousarray(np.atleast_1d(beta), dtype=np.float64)
+        self.na = int(beta.size)
+        self._mask = None
+        if mask is not None:
+            m = np.asarray(mask)
+            if m.shape != (geom.nx, geom.ny):
+                raise ValueError("mask must be [nx ny]")
+            self._mask = np.asfortranarray(m != 0).astype(np.uint8, order="F")
+        h = C.c_void_p()
+        rc = self.lib.fdkhip_plan_create(C.byref(h), C.byref(geom), self.na,
+                                         _ptr(beta), _ptr(self._mask), self.fmt, self.math)
+        _lib.check(rc)
+        self.handle = h
+
+    @classmethod
+    def from_geoms(cls, cg: CTGeom, ig: ImageGeom, *, ia_skip=1, **kw):
+        beta = _views(cg, cg.na)[::ia_skip]
+        return cls(geom_of(cg, ig), beta, ig.mask_or, **kw)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.fdkhip_plan_destroy(h)
+            self.handle = None
+
+    @property
+    def config(self):
+        cfg = (C.c_int32 * 6)()
+        _lib.check(self.lib.fdkhip_plan_config(self.handle, cfg))
+        keys = ("waves", "zper", "vstage", "z0", "zgroups", "workgroups")
+        return dict(zip(keys, list(cfg)))
+
+    def set_config(self, waves, zper, vstage=0, slot_cap=0):
+        _lib.check(self.lib.fdkhip_plan_set_config(self.handle, waves, zper, vstage,
+                                                   slot_cap))
+
+    def count_updates(self) -> int:
+        n = C.c_int64()
+        _lib.check(self.lib.fdkhip_plan_count_updates(self.handle, C.byref(n)))
+        return int(n.value)
+
+    def _check_proj(self, proj, scale):
+        import torch
+        g = self.geom
+        want = {_lib.F32: torch.float32, _lib.F16: torch.float16,
+                _lib.BF16: torch.bfloat16}[self.fmt]
+        if proj.dtype != want or not proj.is_cuda or not proj.is_contiguous():
+            raise ValueError(f"proj must be a contiguous device tensor of dtype {want}")
+        if proj.numel() != self.na * g.ns * g.nt:
+            raise ValueError("proj must hold [nt ns na]")
+        if self.fmt != _lib.F32:
+            if scale is None or scale.dtype != torch.float32 or scale.numel() != self.na \
+                    or not scale.is_cuda:
+                raise ValueError("F16/BF16 projections need a (na,) float32 device scale")
+
+    def __call__(self, proj, scale=None, out=None, *, stream=None, graph=True):
+        import torch
+        self._check_proj(proj, scale)
+        g = self.geom
+        if out is None:
+            out = torch.empty((g.ny, g.nx, g.nz), dtype=torch.float32, device=proj.device)
+        elif out.dtype != torch.float32 or not out.is_contiguous() \
+                or out.numel() != g.nx * g.ny * g.nz:
+            raise ValueError("out must be a contiguous float32 [nz nx ny] tensor")
+        s = stream if stream is not None else torch.cuda.current_stream(proj.device)
+        fn = self.lib.fdkhip_plan_execute if graph else self.lib.fdkhip_plan_execute_direct
+        rc = fn(self.handle, C.c_void_p(proj.data_ptr()),
+                C.c_void_p(scale.data_ptr()) if scale is not None else None,
+                C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream))
+        _lib.check(rc)
+        return out
+
+    @staticmethod
+    def image_xyz(img):
+        """[nz nx ny] (tensor (ny, nx, nz)) viewed as [nx ny nz]."""
+        return img.permute(1, 0, 2)
+
+
+def quantize_views(proj, fmt="f16", *, stream=None):
+    """Per-view FP16/BF16 storage of ``[nt ns na]`` FP32 device projections.
+
+    Returns ``(q, scale)``: ``q`` has ``proj``'s shape in the chosen format,
+    ``scale`` is ``(na,)`` float32 with ``proj ~= q * scale`` per view
+    (scale a power of two, see fdkhip_quantize_views)."""
+    import torch
+    lib = _lib.load()
+    f = _lib.FMT_NAMES[fmt] if isinstance(fmt, str) else int(fmt)
+    if proj.dtype != torch.float32 or not proj.is_cuda or not proj.is_contiguous():
+        raise ValueError("proj must be a contiguous float32 device tensor (na, ns, nt)")
+    na, ns, nt = proj.shape
+    dt = torch.float16 if f == _lib.F16 else torch.bfloat16
+    q = torch.empty(proj.shape, dtype=dt, device=proj.device)
+    scale = torch.empty((na,), dtype=torch.float32, device=proj.device)
+    s = stream if stream is not None else torch.cuda.current_stream(proj.device)
+    rc = lib.fdkhip_quantize_views(C.c_void_p(proj.data_ptr()), ns, nt, na, f,
+                                   C.c_void_p(q.data_ptr()), C.c_void_p(scale.data_ptr()),
+                                   C.c_void_p(s.cuda_stream))
+    _lib.check(rc)
+    return q, scale
+
+
+__all__ = ["fdk_mex", "cbct_back", "FdkPlan", "FdkMexError", "FdkhipError",
+           "quantize_views", "geom_of", "angle_scale"]

@@ -1,0 +1,649 @@
+// fdk_api.hip -- host side of the C ABI declared in include/fdkhip.h.
+//
+// Argument checks and scalar narrowing mirror the reference mex gateway
+// fdk_mex_back (mex/src/fdk/fdk-orig/fdk,mex.c:22-171); the per-view sines
+// and cosines are those of fdk_ts_back1 (mex/src/fdk-helix/mex/fdk,ts.c:97-98:
+// sinf/cosf of the beta narrowed to float), computed here on the host with
+// the same libm so that the device sees the reference's exact values.
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fdkhip.h"
+#include "fdk_kernels.h"
+
+using namespace fdk;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const char *fmt, ...)
+{
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof buf, fmt, ap);
+	va_end(ap);
+	g_last_error = buf;
+	return code;
+}
+
+#define HIP_TRY(expr)                                                          \
+	do {                                                                       \
+		hipError_t e_ = (expr);                                                \
+		if (e_ != hipSuccess)                                                  \
+			return fail(FDKHIP_EHIP, "%s: %s", #expr, hipGetErrorString(e_));  \
+	} while (0)
+
+extern "C" const char *fdkhip_last_error(void)
+{
+	return g_last_error.c_str();
+}
+
+extern "C" const char *fdkhip_version(void)
+{
+	return "fdkhip 0.1 (gfx950)";
+}
+
+// ---------------------------------------------------------------------------
+// geometry
+
+static int check_geom(const fdkhip_geom *g)
+{
+	if (!g)
+		return fail(FDKHIP_EARG, "geometry pointer is NULL");
+	if (g->nx < 0 || g->ny < 0 || g->nz < 0)
+		return fail(FDKHIP_EARG, "image size must be >= 0 (nx=%d ny=%d nz=%d)",
+			g->nx, g->ny, g->nz);
+	if (g->ns < 0 || g->nt < 0)
+		return fail(FDKHIP_EARG, "detector size must be >= 0 (ns=%d nt=%d)", g->ns, g->nt);
+	if ((long long) g->ns * g->nt > (1ll << 31) || (long long) g->nx * g->ny > (1ll << 31))
+		return fail(FDKHIP_EARG, "detector or image plane too large");
+	return FDKHIP_OK;
+}
+
+// fdk,mex.c:119-137 (double -> float fields) and fdk,ts.c:90-94 (w*).
+static GeomF narrow_geom(const fdkhip_geom *g, int na)
+{
+	GeomF f;
+	f.nx = g->nx; f.ny = g->ny; f.nz = g->nz;
+	f.ns = g->ns; f.nt = g->nt; f.na = na;
+	f.dx = (float) g->dx; f.dy = (float) g->dy; f.dz = (float) g->dz;
+	f.dso = (float) g->dso; f.dsd = (float) g->dsd;
+	f.ds = (float) g->ds; f.dt = (float) g->dt;
+	const float ox = (float) g->offset_x, oy = (float) g->offset_y, oz = (float) g->offset_z;
+	const float os = (float) g->offset_s, ot = (float) g->offset_t;
+	f.wx = (float) ((g->nx - 1) / 2. + ox);
+	f.wy = (float) ((g->ny - 1) / 2. + oy);
+	f.wz = (float) ((g->nz - 1) / 2. + oz);
+	f.ws = (float) ((g->ns - 1) / 2. + os);
+	f.wt = (float) ((g->nt - 1) / 2. + ot);
+	const float dfs = (float) g->dfs;
+	if (isinf(f.dsd) || isinf(f.dso))
+		f.kind = KIND_PAR;
+	else if (dfs == 0)
+		f.kind = KIND_ARC;
+	else {
+		if (!isinf(dfs))
+			fprintf(stderr, "WARN fdkhip: dfs not done - junk!\n");
+		f.kind = KIND_FLAT;
+	}
+	return f;
+}
+
+// [na] {cosf, sinf} of the float-narrowed beta (fdk,ts,t.c:94 -> fdk,ts.c:97-98)
+static std::vector<float2> trig_table(const double *beta, int na)
+{
+	std::vector<float2> t((size_t) na);
+	for (int ia = 0; ia < na; ++ia) {
+		const float b = (float) beta[ia];
+		t[ia].x = cosf(b);
+		t[ia].y = sinf(b);
+	}
+	return t;
+}
+
+// Conservative z range [lo, hi) outside of which no column of the mask meets
+// any view's detector rows: t_inc = mag*dz/dt with mag in
+// [dsd/(dso + r), dsd/(dso - r)], r = largest distance of a masked column
+// centre from the rotation axis.  Computed in double with a 2-slice margin.
+static void support_z(const fdkhip_geom *g, const uint8_t *mask, int &lo, int &hi)
+{
+	lo = 0;
+	hi = g->nz;
+	const double wx = (g->nx - 1) / 2. + g->offset_x, wy = (g->ny - 1) / 2. + g->offset_y;
+	const double wz = (g->nz - 1) / 2. + g->offset_z, wt = (g->nt - 1) / 2. + g->offset_t;
+	double r2 = 0;
+	for (int iy = 0; iy < g->ny; ++iy)
+		for (int ix = 0; ix < g->nx; ++ix) {
+			if (mask && !mask[(long) iy * g->nx + ix])
+				continue;
+			const double x = g->dx * (ix - wx), y = g->dy * (iy - wy);
+			r2 = std::max(r2, x * x + y * y);
+		}
+	const double r = sqrt(r2) + fabs(g->dx) + fabs(g->dy);
+	double mag_lo, mag_hi;
+	if (isinf(g->dsd) || isinf(g->dso)) {
+		mag_lo = mag_hi = 1;
+	} else {
+		if (!(g->dso - r > 0))
+			return;   // volume reaches the source: no useful bound
+		mag_lo = g->dsd / (g->dso + r);
+		mag_hi = g->dsd / (g->dso - r);
+	}
+	const double rate = g->dz / g->dt;
+	if (!(rate > 0) || !isfinite(rate))
+		return;
+	double zmin = 1e300, zmax = -1e300;
+	for (double mag : {mag_lo, mag_hi}) {
+		const double tinc = mag * rate;
+		for (double tb : {-1.0 - 1.0, (double) g->nt + 1.0}) {
+			const double z = wz + (tb - wt) / tinc;
+			zmin = std::min(zmin, z);
+			zmax = std::max(zmax, z);
+		}
+	}
+	if (!isfinite(zmin) || !isfinite(zmax))
+		return;
+	lo = (int) std::max(0.0, floor(zmin) - 2);
+	hi = (int) std::min((double) g->nz, ceil(zmax) + 3);
+	if (lo >= hi) {
+		lo = 0;
+		hi = 0;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// plan
+
+struct fdkhip_plan {
+	fdkhip_geom geom;
+	GeomF gf;
+	int na = 0, fmt = FMT_F32, math = FDKHIP_MATH_EXACT;
+	int device = 0;
+	float2 *d_trig = nullptr;
+	uint8_t *d_mask = nullptr;
+	unsigned long long *d_count = nullptr;
+	int *d_maxarea = nullptr;
+	int *d_any = nullptr;
+	// launch configuration
+	int nw = 8, zl = 16, vstage = 8, slot = 0;
+	int z0 = 0, ngroups = 0, tiles_x = 0, tiles_y = 0;
+	int zs_lo = 0, zs_hi = 0;
+	long long n_updates = -1;
+	int maxarea = 0, fov_any = 0;
+	size_t lds_limit = 65536;
+	// graph cache
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t gexec = nullptr;
+	const void *g_proj = nullptr;
+	const float *g_scale = nullptr;
+	float *g_image = nullptr;
+	hipStream_t cap_stream = nullptr;
+};
+
+static BackArgs make_args(const fdkhip_plan *p, const void *proj, const float *scale,
+	float *image)
+{
+	BackArgs a;
+	memset(&a, 0, sizeof a);
+	a.g = p->gf;
+	a.proj = proj;
+	a.scale = scale;
+	a.trig = p->d_trig;
+	a.mask = p->d_mask;
+	a.image = image;
+	a.tiles_x = p->tiles_x;
+	a.tiles_y = p->tiles_y;
+	a.ntiles = p->tiles_x * p->tiles_y;
+	a.z0 = p->z0;
+	a.ngroups = p->ngroups;
+	a.vstage = p->vstage;
+	a.slot = p->slot;
+	a.count = p->d_count;
+	a.maxarea = p->d_maxarea;
+	a.fov_any = p->d_any;
+	return a;
+}
+
+static void drop_graph(fdkhip_plan *p)
+{
+	if (p->gexec) hipGraphExecDestroy(p->gexec);
+	if (p->graph) hipGraphDestroy(p->graph);
+	p->gexec = nullptr;
+	p->graph = nullptr;
+	p->g_proj = nullptr;
+	p->g_scale = nullptr;
+	p->g_image = nullptr;
+}
+
+// Tiling for (nw, zl), a survey pass over it, then the LDS slot size.
+// vstage_req > 0 fixes the views per stage; slot_cap > 0 caps the slot (floats)
+// -- views whose footprint does not fit gather from HBM instead of LDS.
+static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_cap)
+{
+	if (!back_config_supported(nw, zl))
+		return fail(FDKHIP_EUNSUPPORTED, "configuration waves=%d zper=%d not built", nw, zl);
+	p->nw = nw;
+	p->zl = zl;
+	p->tiles_x = (p->geom.nx + 7) / 8;
+	p->tiles_y = (p->geom.ny + 7) / 8;
+	const int gz = nw * zl;
+	p->z0 = p->zs_lo;
+	p->ngroups = (p->zs_hi > p->zs_lo) ? (p->zs_hi - p->zs_lo + gz - 1) / gz : 0;
+	drop_graph(p);
+	p->maxarea = 0;
+	p->fov_any = 0;
+	p->n_updates = 0;
+	if (p->ngroups > 0 && p->tiles_x > 0 && p->tiles_y > 0 && p->na > 0) {
+		HIP_TRY(launch_zero_u64(p->d_count, 1, nullptr));
+		HIP_TRY(launch_zero_i32(p->d_maxarea, 1, nullptr));
+		HIP_TRY(launch_zero_i32(p->d_any, 1, nullptr));
+		BackArgs a = make_args(p, nullptr, nullptr, nullptr);
+		HIP_TRY(launch_survey(a, nw, zl, nullptr));
+		unsigned long long cnt = 0;
+		HIP_TRY(hipMemcpy(&cnt, p->d_count, sizeof cnt, hipMemcpyDeviceToHost));
+		HIP_TRY(hipMemcpy(&p->maxarea, p->d_maxarea, sizeof(int), hipMemcpyDeviceToHost));
+		HIP_TRY(hipMemcpy(&p->fov_any, p->d_any, sizeof(int), hipMemcpyDeviceToHost));
+		p->n_updates = (long long) cnt;
+	}
+	// views per stage and slot size within the LDS budget
+	int slot = std::max(p->maxarea, 64);
+	slot = (slot + 63) & ~63;
+	if (slot_cap > 0)
+		slot = std::max(64, std::min(slot, slot_cap & ~63));
+	int vstage = vstage_req > 0 ? vstage_req : nw;
+	if (vstage_req > 0) {
+		// keep the requested stage depth, give up slot size first
+		while (slot > 64 && back_lds_bytes(nw, vstage, slot) > p->lds_limit)
+			slot -= 64;
+	}
+	while (vstage > 1 && back_lds_bytes(nw, vstage, slot) > p->lds_limit)
+		--vstage;
+	while (slot > 64 && back_lds_bytes(nw, vstage, slot) > p->lds_limit)
+		slot -= 64;
+	if (back_lds_bytes(nw, vstage, slot) > p->lds_limit)
+		return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
+	p->vstage = vstage;
+	p->slot = slot;
+	return FDKHIP_OK;
+}
+
+static int plan_free(fdkhip_plan *p)
+{
+	if (!p)
+		return FDKHIP_OK;
+	drop_graph(p);
+	if (p->cap_stream) hipStreamDestroy(p->cap_stream);
+	hipFree(p->d_trig);
+	hipFree(p->d_mask);
+	hipFree(p->d_count);
+	hipFree(p->d_maxarea);
+	hipFree(p->d_any);
+	delete p;
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
+	int32_t na, const double *beta, const uint8_t *mask_host, int32_t fmt, int32_t math)
+{
+	if (!out)
+		return fail(FDKHIP_EARG, "plan output pointer is NULL");
+	*out = nullptr;
+	int rc = check_geom(g);
+	if (rc)
+		return rc;
+	if (na < 0)
+		return fail(FDKHIP_EARG, "na must be >= 0 (na=%d)", na);
+	if (na > 0 && !beta)
+		return fail(FDKHIP_EARG, "beta is NULL");
+	if (fmt != FDKHIP_F32 && fmt != FDKHIP_F16 && fmt != FDKHIP_BF16)
+		return fail(FDKHIP_EARG, "unknown projection format %d", fmt);
+	if (math != FDKHIP_MATH_EXACT && math != FDKHIP_MATH_FAST)
+		return fail(FDKHIP_EARG, "unknown math mode %d", math);
+
+	fdkhip_plan *p = new fdkhip_plan;
+	p->geom = *g;
+	p->na = na;
+	p->fmt = fmt;
+	p->math = (fmt == FDKHIP_F32) ? math : FDKHIP_MATH_FAST;
+	p->gf = narrow_geom(g, na);
+	hipGetDevice(&p->device);
+	int lds = 0;
+	if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device)
+		== hipSuccess && lds > 0)
+		p->lds_limit = std::min((size_t) lds, (size_t) 65536);
+
+#define PLAN_TRY(expr)                                                          \
+	do {                                                                        \
+		hipError_t e_ = (expr);                                                 \
+		if (e_ != hipSuccess) {                                                 \
+			int c_ = fail(FDKHIP_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+			plan_free(p);                                                       \
+			return c_;                                                          \
+		}                                                                       \
+	} while (0)
+
+	if (na > 0) {
+		std::vector<float2> trig = trig_table(beta, na);
+		PLAN_TRY(hipMalloc(&p->d_trig, sizeof(float2) * na));
+		PLAN_TRY(hipMemcpy(p->d_trig, trig.data(), sizeof(float2) * na, hipMemcpyHostToDevice));
+	}
+	const long nxy = (long) g->nx * g->ny;
+	if (mask_host && nxy > 0) {
+		PLAN_TRY(hipMalloc(&p->d_mask, nxy));
+		PLAN_TRY(hipMemcpy(p->d_mask, mask_host, nxy, hipMemcpyHostToDevice));
+	}
+	PLAN_TRY(hipMalloc(&p->d_count, sizeof(unsigned long long)));
+	PLAN_TRY(hipMalloc(&p->d_maxarea, sizeof(int)));
+	PLAN_TRY(hipMalloc(&p->d_any, sizeof(int)));
+#undef PLAN_TRY
+
+	support_z(g, mask_host, p->zs_lo, p->zs_hi);
+	// default: 8 waves x 16 slices = 128-slice z-groups (4 x 16 when the
+	// support is short)
+	int nw = 8, zl = 16;
+	if (p->zs_hi - p->zs_lo <= 64)
+		nw = 4;
+	rc = configure(p, nw, zl, 0, 0);
+	if (rc) {
+		plan_free(p);
+		return rc;
+	}
+	// fdk,ts.c:153-154: the reference fails on the first in-FOV column
+	if (p->gf.dz / p->gf.dt < 0 && p->fov_any) {
+		plan_free(p);
+		return fail(FDKHIP_EGEOM, "todo: dz/dt < 0 not done");
+	}
+	*out = p;
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_plan_destroy(fdkhip_plan *p)
+{
+	return plan_free(p);
+}
+
+extern "C" int fdkhip_plan_config(const fdkhip_plan *p, int32_t cfg[6])
+{
+	if (!p || !cfg)
+		return fail(FDKHIP_EARG, "NULL argument");
+	cfg[0] = p->nw;
+	cfg[1] = p->zl;
+	cfg[2] = p->vstage;
+	cfg[3] = p->z0;
+	cfg[4] = p->ngroups;
+	cfg[5] = p->tiles_x * p->tiles_y * p->ngroups;
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_plan_set_config(fdkhip_plan *p, int32_t waves, int32_t zper,
+	int32_t vstage, int32_t slot_cap)
+{
+	if (!p)
+		return fail(FDKHIP_EARG, "plan is NULL");
+	if (vstage < 0 || vstage > 64)
+		return fail(FDKHIP_EARG, "views per stage out of range (%d)", vstage);
+	if (slot_cap < 0)
+		return fail(FDKHIP_EARG, "slot cap must be >= 0 (%d)", slot_cap);
+	return configure(p, waves, zper, vstage, slot_cap);
+}
+
+extern "C" int fdkhip_plan_count_updates(fdkhip_plan *p, int64_t *n)
+{
+	if (!p || !n)
+		return fail(FDKHIP_EARG, "NULL argument");
+	*n = p->n_updates;
+	return FDKHIP_OK;
+}
+
+static int plan_launch(fdkhip_plan *p, const void *proj, const float *scale, float *image,
+	hipStream_t s)
+{
+	const long ncol = (long) p->geom.nx * p->geom.ny;
+	const int zend = std::min(p->z0 + p->ngroups * p->nw * p->zl, p->geom.nz);
+	const int zbeg = p->ngroups > 0 ? p->z0 : 0;
+	if (ncol > 0 && p->geom.nz > 0)
+		HIP_TRY(launch_fill(image, ncol, p->geom.nz, zbeg, p->ngroups > 0 ? zend : 0, s));
+	if (p->ngroups > 0 && ncol > 0 && p->na > 0) {
+		BackArgs a = make_args(p, proj, scale, image);
+		HIP_TRY(launch_back(a, p->nw, p->zl, p->fmt, p->math == FDKHIP_MATH_FAST, s));
+	} else if (p->ngroups > 0 && ncol > 0) {
+		// no views: zero the z-groups too
+		HIP_TRY(launch_fill(image, ncol, p->geom.nz, 0, 0, s));
+	}
+	return FDKHIP_OK;
+}
+
+static int check_exec(fdkhip_plan *p, const void *proj, const float *scale, float *image)
+{
+	if (!p)
+		return fail(FDKHIP_EARG, "plan is NULL");
+	const long nvox = (long) p->geom.nx * p->geom.ny * p->geom.nz;
+	if (nvox > 0 && !image)
+		return fail(FDKHIP_EARG, "image is NULL");
+	if (p->na > 0 && (long) p->geom.ns * p->geom.nt > 0 && !proj)
+		return fail(FDKHIP_EARG, "proj is NULL");
+	if (p->fmt != FDKHIP_F32 && p->na > 0 && !scale)
+		return fail(FDKHIP_EARG, "per-view scale is required for F16/BF16 projections");
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_plan_execute_direct(fdkhip_plan *p, const void *proj,
+	const float *scale, float *image, void *stream)
+{
+	int rc = check_exec(p, proj, scale, image);
+	if (rc)
+		return rc;
+	return plan_launch(p, proj, scale, image, (hipStream_t) stream);
+}
+
+extern "C" int fdkhip_plan_execute(fdkhip_plan *p, const void *proj, const float *scale,
+	float *image, void *stream)
+{
+	int rc = check_exec(p, proj, scale, image);
+	if (rc)
+		return rc;
+	if (!p->gexec || p->g_proj != proj || p->g_scale != scale || p->g_image != image) {
+		drop_graph(p);
+		if (!p->cap_stream)
+			HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+		HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+		rc = plan_launch(p, proj, scale, image, p->cap_stream);
+		hipGraph_t graph = nullptr;
+		hipError_t e = hipStreamEndCapture(p->cap_stream, &graph);
+		if (rc)
+			return rc;
+		if (e != hipSuccess)
+			return fail(FDKHIP_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+		p->graph = graph;
+		HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+		p->g_proj = proj;
+		p->g_scale = scale;
+		p->g_image = image;
+	}
+	HIP_TRY(hipGraphLaunch(p->gexec, (hipStream_t) stream));
+	return FDKHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// quantisation
+
+extern "C" int fdkhip_quantize_views(const float *proj, int32_t ns, int32_t nt, int32_t na,
+	int32_t fmt, void *out, float *scale, void *stream)
+{
+	if (fmt != FDKHIP_F16 && fmt != FDKHIP_BF16)
+		return fail(FDKHIP_EARG, "quantisation format must be F16 or BF16 (got %d)", fmt);
+	if (ns < 0 || nt < 0 || na < 0)
+		return fail(FDKHIP_EARG, "negative size");
+	if (na == 0 || (long) ns * nt == 0)
+		return FDKHIP_OK;
+	if (!proj || !out || !scale)
+		return fail(FDKHIP_EARG, "NULL device pointer");
+	hipStream_t s = (hipStream_t) stream;
+	int *maxbits = nullptr;
+	HIP_TRY(hipMallocAsync((void **) &maxbits, sizeof(int) * na, s));
+	hipError_t e = launch_quantize(proj, ns, nt, na, fmt, out, scale, maxbits, s);
+	hipFreeAsync(maxbits, s);
+	if (e != hipSuccess)
+		return fail(FDKHIP_EHIP, "quantize: %s", hipGetErrorString(e));
+	return FDKHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-memory drop-ins for 'fdk,ts,back' and 'fdk,st,back'
+
+// fdk,mex.c:39-91 + cbct,mask2.c:738 checks shared by both commands
+static int check_mex_args(const int32_t *nxyz, const double *dxyz, const double *cxyz,
+	const double *dst, const double *ost, int32_t ns, int32_t nt, int32_t na,
+	const double *beta, int32_t nthread, const float *proj, const float *image)
+{
+	if (!nxyz) return fail(FDKHIP_EARG, "mx_nxyz must have 3 elements");
+	if (!dxyz) return fail(FDKHIP_EARG, "mx_dxyz must have 3 elements");
+	if (!cxyz) return fail(FDKHIP_EARG, "mx_cxyz must have 3 elements");
+	if (!dst) return fail(FDKHIP_EARG, "mx_dst must have 2 elements");
+	if (!ost) return fail(FDKHIP_EARG, "mx_ost must have 2 elements");
+	if (nthread > 250) return fail(FDKHIP_ETHREADS, "too many threads");
+	if (nthread < 1) return fail(FDKHIP_EARG, "nthread must be >= 1 (got %d)", nthread);
+	if (na < 0 || ns < 0 || nt < 0) return fail(FDKHIP_EARG, "negative projection size");
+	if (na > 0 && !beta) return fail(FDKHIP_EARG, "beta length=0 vs na=size(proj,3)=%d", na);
+	if (na > 0 && (long) ns * nt > 0 && !proj) return fail(FDKHIP_EARG, "proj is NULL");
+	if ((long) nxyz[0] * nxyz[1] * nxyz[2] > 0 && !image)
+		return fail(FDKHIP_EARG, "image is NULL");
+	return FDKHIP_OK;
+}
+
+static fdkhip_geom mex_geom(const int32_t *nxyz, const double *dxyz, const double *cxyz,
+	double dso, double dsd, double dfs, const double *dst, const double *ost,
+	int32_t ns, int32_t nt)
+{
+	fdkhip_geom g;
+	g.nx = nxyz[0]; g.ny = nxyz[1]; g.nz = nxyz[2];
+	g.dx = dxyz[0]; g.dy = dxyz[1]; g.dz = dxyz[2];
+	g.offset_x = cxyz[0]; g.offset_y = cxyz[1]; g.offset_z = cxyz[2];
+	g.dso = dso; g.dsd = dsd; g.dfs = dfs;
+	g.ns = ns; g.nt = nt;
+	g.ds = dst[0]; g.dt = dst[1];
+	g.offset_s = ost[0]; g.offset_t = ost[1];
+	return g;
+}
+
+// RAII device buffer for the synchronous entry points
+struct DevBuf {
+	void *p = nullptr;
+	~DevBuf() { if (p) hipFree(p); }
+};
+
+extern "C" int fdkhip_fdk_ts_back(
+	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
+	const uint8_t *mask, double dso, double dsd, double dfs,
+	const double dst[2], const double ost[2],
+	const float *proj, int32_t ns, int32_t nt, int32_t na,
+	const double *beta, int32_t nthread, float *image)
+{
+	int rc = check_mex_args(nxyz, dxyz, cxyz, dst, ost, ns, nt, na, beta, nthread, proj, image);
+	if (rc)
+		return rc;
+	const fdkhip_geom g = mex_geom(nxyz, dxyz, cxyz, dso, dsd, dfs, dst, ost, ns, nt);
+	rc = check_geom(&g);
+	if (rc)
+		return rc;
+	const size_t nvox = (size_t) g.nx * g.ny * g.nz;
+	if (nvox == 0)
+		return FDKHIP_OK;
+	fdkhip_plan *p = nullptr;
+	rc = fdkhip_plan_create(&p, &g, na, beta, mask, FDKHIP_F32, FDKHIP_MATH_EXACT);
+	if (rc)
+		return rc;
+	DevBuf dproj, dimg;
+	const size_t nproj = (size_t) ns * nt * na;
+	hipError_t e = hipSuccess;
+	if (nproj > 0)
+		e = hipMalloc(&dproj.p, nproj * sizeof(float));
+	if (e == hipSuccess)
+		e = hipMalloc(&dimg.p, nvox * sizeof(float));
+	if (e != hipSuccess) {
+		plan_free(p);
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	}
+	if (nproj > 0)
+		e = hipMemcpy(dproj.p, proj, nproj * sizeof(float), hipMemcpyHostToDevice);
+	if (e != hipSuccess) {
+		plan_free(p);
+		return fail(FDKHIP_EHIP, "hipMemcpy proj: %s", hipGetErrorString(e));
+	}
+	rc = fdkhip_plan_execute_direct(p, dproj.p, nullptr, (float *) dimg.p, nullptr);
+	if (rc == FDKHIP_OK) {
+		e = hipMemcpy(image, dimg.p, nvox * sizeof(float), hipMemcpyDeviceToHost);
+		if (e != hipSuccess)
+			rc = fail(FDKHIP_EHIP, "hipMemcpy image: %s", hipGetErrorString(e));
+	}
+	plan_free(p);
+	return rc;
+}
+
+extern "C" int fdkhip_fdk_st_back(
+	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
+	const uint8_t *mask, double dso, double dsd, double dfs,
+	const double dst[2], const double ost[2],
+	const float *proj, int32_t ns, int32_t nt, int32_t na,
+	const double *beta, int32_t nthread, float *image)
+{
+	int rc = check_mex_args(nxyz, dxyz, cxyz, dst, ost, ns, nt, na, beta, nthread, proj, image);
+	if (rc)
+		return rc;
+	const fdkhip_geom g = mex_geom(nxyz, dxyz, cxyz, dso, dsd, dfs, dst, ost, ns, nt);
+	rc = check_geom(&g);
+	if (rc)
+		return rc;
+	if (nthread != 1)
+		fprintf(stderr, "WARN fdkhip: only nthread=1 supported for fdk_st_, ignoring\n");
+	const size_t nvox = (size_t) g.nx * g.ny * g.nz;
+	if (nvox == 0)
+		return FDKHIP_OK;
+	const size_t nxy = (size_t) g.nx * g.ny;
+	const size_t nproj = (size_t) ns * nt * na;
+	std::vector<float2> trig = trig_table(beta, na);
+	DevBuf dproj, dimg, dtrig, dmask;
+	hipError_t e = hipMalloc(&dimg.p, nvox * sizeof(float));
+	if (e == hipSuccess && nproj > 0)
+		e = hipMalloc(&dproj.p, nproj * sizeof(float));
+	if (e == hipSuccess && na > 0)
+		e = hipMalloc(&dtrig.p, na * sizeof(float2));
+	if (e == hipSuccess && mask)
+		e = hipMalloc(&dmask.p, nxy);
+	if (e != hipSuccess)
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	if (nproj > 0)
+		HIP_TRY(hipMemcpy(dproj.p, proj, nproj * sizeof(float), hipMemcpyHostToDevice));
+	if (na > 0)
+		HIP_TRY(hipMemcpy(dtrig.p, trig.data(), na * sizeof(float2), hipMemcpyHostToDevice));
+	if (mask)
+		HIP_TRY(hipMemcpy(dmask.p, mask, nxy, hipMemcpyHostToDevice));
+	StArgs a;
+	memset(&a, 0, sizeof a);
+	a.g = narrow_geom(&g, na);
+	a.proj = (const float *) dproj.p;
+	a.trig = (const float2 *) dtrig.p;
+	a.mask = (const uint8_t *) dmask.p;
+	a.image = (float *) dimg.p;
+	if (na == 0 || (long) ns * nt == 0) {
+		HIP_TRY(launch_fill(a.image, (long) nxy, g.nz, 0, 0, nullptr));
+	} else {
+		HIP_TRY(launch_st(a, nullptr));
+	}
+	HIP_TRY(hipMemcpy(image, dimg.p, nvox * sizeof(float), hipMemcpyDeviceToHost));
+	return FDKHIP_OK;
+}

@@ -1,0 +1,56 @@
+// fdk_kernels.h -- device-side argument blocks and launchers of the FDK
+// backprojector kernels (fdk_kernels.hip).  Internal to the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fdk {
+
+enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2 };
+enum { KIND_FLAT = 0, KIND_ARC = 1, KIND_PAR = 2 };
+
+// Scan geometry as the reference kernel sees it: every scalar narrowed to
+// float as in fdk,mex.c:119-137, w* as in fdk,ts.c:90-94.
+struct GeomF {
+	int nx, ny, nz, ns, nt, na;
+	float dx, dy, dz, dso, dsd, ds, dt;
+	float wx, wy, wz, ws, wt;
+	int kind;
+};
+
+struct BackArgs {
+	GeomF g;
+	const void *proj;        // [na][ns][nt] (t fastest), storage format fmt
+	const float *scale;      // [na] per-view scale (F16/BF16), else unused
+	const float2 *trig;      // [na] {cosf(beta), sinf(beta)} of the float beta
+	const uint8_t *mask;     // [ny][nx] or null
+	float *image;            // [ny][nx][nz]
+	int tiles_x, tiles_y, ntiles;
+	int z0, ngroups;         // z-groups cover [z0, z0 + ngroups*NW*ZL)
+	int vstage, slot;        // views per LDS stage, floats per view slot
+	unsigned long long *count;   // survey: reference loop iterations
+	int *maxarea;            // survey: largest staged footprint (floats)
+	int *fov_any;            // survey: some column meets some view
+};
+
+struct StArgs {
+	GeomF g;
+	const float *proj;       // [na][nt][ns] (s fastest)
+	const float2 *trig;
+	const uint8_t *mask;
+	float *image;            // [nz][ny][nx]
+};
+
+size_t back_lds_bytes(int nw, int vstage, int slot);
+bool back_config_supported(int nw, int zl);
+hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s);
+hipError_t launch_survey(const BackArgs &a, int nw, int zl, hipStream_t s);
+hipError_t launch_fill(float *image, long ncol, int nz, int zlo, int zhi, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
+hipError_t launch_zero_i32(int *p, int n, hipStream_t s);
+hipError_t launch_st(const StArgs &a, hipStream_t s);
+hipError_t launch_quantize(const float *proj, int ns, int nt, int na, int fmt,
+	void *out, float *scale, int *maxbits, hipStream_t s);
+
+} // namespace fdk

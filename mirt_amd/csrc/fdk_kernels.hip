@@ -1,0 +1,792 @@
+// fdk_kernels.hip -- gfx950 kernels of the FDK cone-beam backprojector.
+//
+// The arithmetic follows fdk_ts_back1 of the reference
+// (mex/src/fdk-helix/mex/fdk,ts.c:55-251, axial branch) and fdk_st_back1
+// (mex/src/fdk/fdk-orig/fdk,st.c:38-141).  This file is compiled with
+// -ffp-contract=off: every multiply-add below is a separate multiply and add,
+// as in the reference C built without fast-math, unless it is written as an
+// explicit fmaf() (only in the FAST paths).
+//
+// Work decomposition of the main kernel (back_kernel):
+//   * a workgroup owns an 8x8 tile of (x,y) columns and a z-group of
+//     NW*ZL slices; wave w of the workgroup owns slices
+//     [zg0 + w*ZL, zg0 + (w+1)*ZL) of all 64 columns, lane = column, and keeps
+//     those ZL voxels in FP32 registers for the whole view loop;
+//   * views are processed in stages of `vstage` views:
+//       setup   -- one lane per (view, column) computes the reference's
+//                  per-column quantities (s bin, weights, z range, t_bin) in
+//                  the reference's float operation order, walks t_bin by
+//                  repeated addition exactly as the reference loop does and
+//                  records the value at each wave's first slice; a wave
+//                  reduction gives the view's detector footprint for the tile;
+//       stage   -- the footprint (zero-padded view, as fdk_ts_put_view does,
+//                  mex/src/fdk/fdk-orig/fdk,ts,t.c:13-21) is copied from HBM
+//                  into LDS, converted to FP32 (F16/BF16 times the view scale);
+//       update  -- every lane runs its ZL slices: bilinear interpolation from
+//                  LDS in registers, FP32 accumulation in registers;
+//   * one write per voxel at the end, into the [nz nx ny] image.
+// The reference's t_bin sequence (t_bin += t_inc per slice) is reproduced
+// exactly, which together with the reference operation order makes the EXACT
+// mode bit-for-bit equal to the reference C on IEEE FP32.
+//
+// No MFMA: the operation is a gather plus a few multiply-adds per voxel-view.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "fdk_kernels.h"
+
+namespace fdk {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+// (int) conversion with the x86-64 cvttss2si result for NaN and out-of-range
+// values (INT_MIN), which is what the reference C does on its host.
+__device__ __forceinline__ int to_int_x86(float f)
+{
+	return (f >= -2147483648.0f && f < 2147483648.0f) ? (int) f : (int) 0x80000000u;
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h)
+{
+	return __uint_as_float(((uint32_t) h) << 16);
+}
+
+template <int FMT>
+__device__ __forceinline__ float load_sample(const void *proj, long idx)
+{
+	if constexpr (FMT == FMT_F32)
+		return static_cast<const float *>(proj)[idx];
+	else if constexpr (FMT == FMT_F16)
+		return __half2float(static_cast<const __half *>(proj)[idx]);
+	else
+		return bf16_to_f32(static_cast<const uint16_t *>(proj)[idx]);
+}
+
+// Per-column, per-view quantities of fdk,ts.c:120-177 (axial branch,
+// cone_par = 0, source_zs = 0), same float operation order.
+struct ColSetup {
+	int is;          // left s bin, in [-1, ns-1]
+	float wl, wr;    // horizontal weights
+	float w2;        // fan-beam weight
+	float t_inc;     // t' step per slice
+	float t_bin;     // t' at iz_min
+	int iz_min, iz_max;
+};
+
+// Returns SETUP_OUT when the column misses the view's detector (fdk,ts.c:134),
+// SETUP_IDLE when it meets it but reads no sample, SETUP_ACTIVE otherwise.
+enum { SETUP_OUT = 0, SETUP_ACTIVE = 1, SETUP_IDLE = 2 };
+
+__device__ __forceinline__ int column_setup(const GeomF &g, float xx, float yy,
+	float cosb, float sinb, ColSetup &c)
+{
+	const float xbeta = xx * cosb + yy * sinb;
+	const float ybetas = g.dso - (-xx * sinb + yy * cosb);
+	const float mag = (g.kind == KIND_PAR) ? 1.f : g.dsd / ybetas;
+	float ss;
+	if (g.kind == KIND_ARC)
+		ss = (float) ((double) g.dsd * atan2((double) xbeta, (double) ybetas));
+	else
+		ss = mag * xbeta;
+	const float ss_bin = ss / g.ds + g.ws;
+	const int is = to_int_x86(floorf(ss_bin));
+	if (is < -1 || is >= g.ns)
+		return SETUP_OUT;
+	c.is = is;
+	c.wr = ss_bin - (float) is;
+	c.wl = 1.f - c.wr;
+	c.t_inc = mag * g.dz / g.dt;
+	// t_inc <= 0 means a voxel behind the source (or dz = 0, where the
+	// reference's slice range is empty): no samples are read for it.  The
+	// footprint bounds below rely on t_bin increasing along z.
+	if (!(c.t_inc > 0.f) || c.t_inc == INFINITY)
+		return SETUP_IDLE;
+	int iz_min = to_int_x86(floorf(g.wz - (g.wt + 1.f) / c.t_inc) + 1.f);
+	int iz_max = to_int_x86(ceilf(g.wz + ((float) g.nt - g.wt) / c.t_inc) - 1.f + 1.f);
+	iz_min = max(iz_min, 0);
+	iz_max = min(iz_max, g.nz);
+	float t_bin = c.t_inc * ((float) iz_min - g.wz) + g.wt;
+	if (t_bin == -1.f) {
+		++iz_min;
+		t_bin += c.t_inc;
+	}
+	c.t_bin = t_bin;
+	c.iz_min = iz_min;
+	c.iz_max = iz_max;
+	c.w2 = (g.kind == KIND_ARC)
+		? (g.dsd * g.dsd) / (ybetas * ybetas + xbeta * xbeta)
+		: mag * mag;
+	return SETUP_ACTIVE;
+}
+
+// Walk of the reference's t_bin sequence over one z-group [zg0, zg1):
+// t_bin starts at iz_min and grows by repeated addition of t_inc
+// (fdk,ts.c:169,244), so the value at slice z is reproduced only by the same
+// sequence of additions.  Records, for each of the NW slice chunks of ZL
+// slices, t_bin at the chunk's first active slice, and returns the effective
+// range [zlo, zhi) and t_bin at its ends.  Slices whose t_bin is <= -1 or
+// >= nt are dropped: every sample they would read is a zero pad, so the
+// reference adds exactly 0 there.
+template <int NW, int ZL>
+__device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
+	int zg0, int zg1, float (&Tk)[NW], int &zlo, int &zhi,
+	float &t_first, float &t_last)
+{
+	const int lo = max(c.iz_min, zg0), hi = min(c.iz_max, zg1);
+	float t = c.t_bin;
+	int z = c.iz_min;
+	for (; z < lo; ++z)
+		t += c.t_inc;
+	while (z < hi && t <= -1.f) {
+		t += c.t_inc;
+		++z;
+	}
+	zlo = z;
+	zhi = hi;
+	t_first = t;
+	t_last = t;
+	float tt = t;
+#pragma unroll
+	for (int k = 0; k < NW; ++k) {
+		const int zs = max(zg0 + k * ZL, zlo);
+		const int ze = min(zg0 + (k + 1) * ZL, zhi);
+		Tk[k] = 0.f;
+		if (zs < ze) {
+			Tk[k] = tt;
+			for (int zz = zs; zz < ze - 1; ++zz)
+				tt += c.t_inc;
+			t_last = tt;
+			tt += c.t_inc;
+		}
+	}
+	if (zlo < zhi && t_last >= (float) g.nt) {
+		// rare (rounding at the last slice): trim the zero-contribution tail
+		float t2 = t_first, tl = t_first;
+		int z2 = zlo;
+		while (z2 < zhi && t2 < (float) g.nt) {
+			tl = t2;
+			t2 += c.t_inc;
+			++z2;
+		}
+		zhi = z2;
+		t_last = tl;
+	}
+	return zlo < zhi;
+}
+
+// Wave-wide reductions (wave64).
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		v = min(v, __shfl_xor(v, o, 64));
+	return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		v = max(v, __shfl_xor(v, o, 64));
+	return v;
+}
+
+// Logical block index with consecutive indices on the same XCD: the hardware
+// hands workgroup i to XCD i % 8, so neighbouring tiles (which share detector
+// footprints) land in the same L2.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks)
+{
+	const int xcd = bid & 7, local = bid >> 3;
+	const int per = nblocks >> 3, rem = nblocks & 7;
+	return xcd * per + min(xcd, rem) + local;
+}
+
+// ---------------------------------------------------------------------------
+// main kernel
+
+// LDS layout (floats / ints), all per stage:
+//   slots   [vstage][slot]      staged footprints (FP32)
+//   P_cb    [vstage][64]  int   LDS index of sample (s = is, t = 0) in its slot
+//   P_wl, P_wr, P_w2, P_tinc [vstage][64] float
+//   P_zlo, P_zhi [vstage][64] int   effective slice range [zlo, zhi)
+//   P_is    [vstage][64]  int
+//   P_T     [vstage][NW][64] float  t_bin at each wave's first active slice
+//   V_info  [vstage][4]   int   s_lo, t_lo, tw, mode
+template <int NW>
+struct Lds {
+	int slot_off, cb, wl, wr, w2, tinc, zlo, zhi, is, T, vinfo, total;
+	__device__ __forceinline__ Lds(int vstage, int slot)
+	{
+		slot_off = 0;
+		cb = vstage * slot;
+		wl = cb + vstage * 64;
+		wr = wl + vstage * 64;
+		w2 = wr + vstage * 64;
+		tinc = w2 + vstage * 64;
+		zlo = tinc + vstage * 64;
+		zhi = zlo + vstage * 64;
+		is = zhi + vstage * 64;
+		T = is + vstage * 64;
+		vinfo = T + vstage * NW * 64;
+		total = vinfo + vstage * 4;
+	}
+};
+
+enum { MODE_LDS = 0, MODE_GLOBAL = 1, MODE_NONE = 2 };
+
+template <int NW, int ZL, int FMT, bool FAST>
+__global__ void __launch_bounds__(NW * 64)
+back_kernel(const BackArgs a)
+{
+	extern __shared__ float smem[];
+	int *smem_i = reinterpret_cast<int *>(smem);
+	const GeomF &g = a.g;
+	const int tid = threadIdx.x;
+	const int wave = tid >> 6;
+	const int lane = tid & 63;
+
+	const int nblocks = a.ntiles * a.ngroups;
+	const int lb = xcd_block(blockIdx.x, nblocks);
+	const int tile = lb / a.ngroups;
+	const int group = lb - tile * a.ngroups;
+	const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+	const int zg0 = a.z0 + group * (NW * ZL);
+	const int zg1 = min(zg0 + NW * ZL, g.nz);
+
+	const int ix = tx * 8 + (lane & 7);
+	const int iy = ty * 8 + (lane >> 3);
+	const bool colvalid = ix < g.nx && iy < g.ny &&
+		(a.mask == nullptr || a.mask[(long) iy * g.nx + ix] != 0);
+	// fdk,ts.c:114,120
+	const float yy = g.dy * ((float) iy - g.wy);
+	const float xx = g.dx * ((float) ix - g.wx);
+
+	const Lds<NW> L(a.vstage, a.slot);
+	const int zc = zg0 + wave * ZL;   // this wave's first slice
+
+	float acc[ZL];
+#pragma unroll
+	for (int j = 0; j < ZL; ++j)
+		acc[j] = 0.f;
+
+	const long view_elems = (long) g.ns * g.nt;
+
+	for (int v0 = 0; v0 < g.na; v0 += a.vstage) {
+		const int nv = min(a.vstage, g.na - v0);
+
+		// ---------------- setup: one lane per (view, column)
+		for (int vv = wave; vv < nv; vv += NW) {
+			const int v = v0 + vv;
+			const float2 cs = a.trig[v];
+			ColSetup c;
+			bool act = colvalid && column_setup(g, xx, yy, cs.x, cs.y, c) == SETUP_ACTIVE;
+			int zlo = 0, zhi = 0;
+			float t_first = 0.f, t_last = 0.f;
+			float Tk[NW];
+#pragma unroll
+			for (int k = 0; k < NW; ++k)
+				Tk[k] = 0.f;
+			if (act)
+				act = setup_walk<NW, ZL>(g, c, zg0, zg1, Tk, zlo, zhi, t_first, t_last);
+			// footprint of this view over the tile and z-group (padded coords)
+			const int s_lo = wave_min(act ? c.is + 1 : 0x7fffffff);
+			const int s_hi = wave_max(act ? c.is + 2 : (int) 0x80000000);
+			const int t_lo = wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff);
+			const int t_hi = wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000);
+			int mode = MODE_NONE, tw = 0;
+			if (s_lo <= s_hi) {
+				tw = t_hi - t_lo + 1;
+				const long area = (long) (s_hi - s_lo + 1) * tw;
+				mode = (area <= a.slot) ? MODE_LDS : MODE_GLOBAL;
+			}
+			const int idx = vv * 64 + lane;
+			smem_i[L.cb + idx] = act ? vv * a.slot + (c.is + 1 - s_lo) * tw + (1 - t_lo) : 0;
+			float wl = act ? c.wl : 0.f, wr = act ? c.wr : 0.f;
+			const float w2 = act ? c.w2 : 0.f;
+			if (FAST) {
+				float sc = 1.f;
+				if (FMT != FMT_F32)
+					sc = a.scale[v];
+				wl = w2 * wl * sc;
+				wr = w2 * wr * sc;
+			}
+			smem[L.wl + idx] = wl;
+			smem[L.wr + idx] = wr;
+			smem[L.w2 + idx] = w2;
+			smem[L.tinc + idx] = act ? c.t_inc : 0.f;
+			smem_i[L.zlo + idx] = zlo;
+			smem_i[L.zhi + idx] = act ? zhi : zlo;
+			smem_i[L.is + idx] = act ? c.is : 0;
+#pragma unroll
+			for (int k = 0; k < NW; ++k)
+				smem[L.T + (vv * NW + k) * 64 + lane] = Tk[k];
+			if (lane == 0) {
+				smem_i[L.vinfo + vv * 4 + 0] = s_lo;
+				smem_i[L.vinfo + vv * 4 + 1] = t_lo;
+				smem_i[L.vinfo + vv * 4 + 2] = tw;
+				smem_i[L.vinfo + vv * 4 + 3] = (mode << 16) | (s_hi - s_lo + 1);
+			}
+		}
+		__syncthreads();
+
+		// ---------------- stage footprints into LDS (zero-padded view)
+		for (int vv = 0; vv < nv; ++vv) {
+			const int ms = smem_i[L.vinfo + vv * 4 + 3];
+			if ((ms >> 16) != MODE_LDS)
+				continue;
+			const int sw = ms & 0xffff;
+			const int s_lo = smem_i[L.vinfo + vv * 4 + 0];
+			const int t_lo = smem_i[L.vinfo + vv * 4 + 1];
+			const int tw = smem_i[L.vinfo + vv * 4 + 2];
+			float *slot = smem + vv * a.slot;
+			const long vbase = (long) (v0 + vv) * view_elems;
+			// waves take padded s columns, lanes run along t (coalesced in HBM)
+			for (int r = wave; r < sw; r += NW) {
+				const int s = s_lo + r - 1;    // unpadded column
+				for (int tc = lane; tc < tw; tc += 64) {
+					const int t = t_lo + tc - 1;   // unpadded row
+					float val = 0.f;
+					if (s >= 0 && s < g.ns && t >= 0 && t < g.nt)
+						val = load_sample<FMT>(a.proj, vbase + (long) s * g.nt + t);
+					slot[r * tw + tc] = val;
+				}
+			}
+		}
+		__syncthreads();
+
+		// ---------------- update: this wave's ZL slices of its column
+		for (int vv = 0; vv < nv; ++vv) {
+			const int mode = smem_i[L.vinfo + vv * 4 + 3] >> 16;
+			if (mode == MODE_NONE)
+				continue;
+			const int idx = vv * 64 + lane;
+			const int zlo = smem_i[L.zlo + idx];
+			const int zhi = smem_i[L.zhi + idx];
+			const int jlo = max(zlo - zc, 0);
+			const int jhi = min(zhi - zc, ZL);
+			if (jlo >= jhi)
+				continue;
+			float t = smem[L.T + (vv * NW + wave) * 64 + lane];
+			const float t_inc = smem[L.tinc + idx];
+			const float wl = smem[L.wl + idx];
+			const float wr = smem[L.wr + idx];
+			const float w2 = smem[L.w2 + idx];
+			if (mode == MODE_LDS) {
+				const int tw = smem_i[L.vinfo + vv * 4 + 2];
+				const float *p1 = smem + smem_i[L.cb + idx];
+				const float *p2 = p1 + tw;
+#pragma unroll
+				for (int j = 0; j < ZL; ++j) {
+					if (j >= jlo && j < jhi) {
+						const int it = (int) floorf(t);
+						const float wu = t - (float) it;
+						const float wd = 1.f - wu;
+						const float a0 = p1[it], a1 = p1[it + 1];
+						const float b0 = p2[it], b1 = p2[it + 1];
+						if (FAST) {
+							const float q1 = fmaf(wr, b0, wl * a0);
+							const float q2 = fmaf(wr, b1, wl * a1);
+							acc[j] = fmaf(wd, q1, acc[j]);
+							acc[j] = fmaf(wu, q2, acc[j]);
+						} else {
+							const float q1 = wl * a0 + wr * b0;
+							const float q2 = wl * a1 + wr * b1;
+							const float p0 = wd * q1 + wu * q2;
+							acc[j] += w2 * p0;
+						}
+						t += t_inc;
+					}
+				}
+			} else {
+				// footprint larger than the LDS slot: gather from HBM
+				const int v = v0 + vv;
+				const int is = smem_i[L.is + idx];
+				const long vbase = (long) v * view_elems;
+#pragma unroll
+				for (int j = 0; j < ZL; ++j) {
+					if (j >= jlo && j < jhi) {
+						const int it = (int) floorf(t);
+						const float wu = t - (float) it;
+						const float wd = 1.f - wu;
+						float s4[4];
+#pragma unroll
+						for (int q = 0; q < 4; ++q) {
+							const int s = is + (q >> 1), tt = it + (q & 1);
+							s4[q] = (s >= 0 && s < g.ns && tt >= 0 && tt < g.nt)
+								? load_sample<FMT>(a.proj, vbase + (long) s * g.nt + tt)
+								: 0.f;
+						}
+						if (FAST) {
+							const float q1 = fmaf(wr, s4[2], wl * s4[0]);
+							const float q2 = fmaf(wr, s4[3], wl * s4[1]);
+							acc[j] = fmaf(wd, q1, acc[j]);
+							acc[j] = fmaf(wu, q2, acc[j]);
+						} else {
+							const float q1 = wl * s4[0] + wr * s4[2];
+							const float q2 = wl * s4[1] + wr * s4[3];
+							const float p0 = wd * q1 + wu * q2;
+							acc[j] += w2 * p0;
+						}
+						t += t_inc;
+					}
+				}
+			}
+		}
+		__syncthreads();
+	}
+
+	// ---------------- one write per voxel, [nz nx ny]
+	if (ix < g.nx && iy < g.ny) {
+		float *col = a.image + ((long) iy * g.nx + ix) * g.nz;
+#pragma unroll
+		for (int j = 0; j < ZL; ++j) {
+			const int z = zc + j;
+			if (z < zg1)
+				col[z] = acc[j];
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// survey kernel: per plan, over the same tiling as back_kernel -- counts the
+// reference loop's (voxel, view) iterations, the largest staged footprint and
+// whether any column meets any view (for the dz/dt < 0 error).
+
+template <int NW, int ZL>
+__global__ void __launch_bounds__(64)
+survey_kernel(const BackArgs a)
+{
+	const GeomF &g = a.g;
+	const int lane = threadIdx.x;
+	const int nblocks = a.ntiles * a.ngroups;
+	const int lb = blockIdx.x;
+	if (lb >= nblocks)
+		return;
+	const int tile = lb / a.ngroups;
+	const int group = lb - tile * a.ngroups;
+	const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+	const int zg0 = a.z0 + group * (NW * ZL);
+	const int zg1 = min(zg0 + NW * ZL, g.nz);
+	const int ix = tx * 8 + (lane & 7);
+	const int iy = ty * 8 + (lane >> 3);
+	const bool colvalid = ix < g.nx && iy < g.ny &&
+		(a.mask == nullptr || a.mask[(long) iy * g.nx + ix] != 0);
+	const float yy = g.dy * ((float) iy - g.wy);
+	const float xx = g.dx * ((float) ix - g.wx);
+	unsigned long long cnt = 0;
+	int maxarea = 0, any = 0;
+	for (int v = 0; v < g.na; ++v) {
+		const float2 cs = a.trig[v];
+		ColSetup c;
+		const int code = colvalid ? column_setup(g, xx, yy, cs.x, cs.y, c) : SETUP_OUT;
+		any |= (code != SETUP_OUT) ? 1 : 0;
+		bool act = code == SETUP_ACTIVE;
+		// reference loop iterations of this column-view (whole range, once)
+		if (act && group == 0)
+			cnt += (unsigned long long) max(c.iz_max - c.iz_min, 0);
+		int zlo = 0, zhi = 0;
+		float t_first = 0.f, t_last = 0.f;
+		float Tk[NW];
+		if (act)
+			act = setup_walk<NW, ZL>(g, c, zg0, zg1, Tk, zlo, zhi, t_first, t_last);
+		const int s_lo = wave_min(act ? c.is + 1 : 0x7fffffff);
+		const int s_hi = wave_max(act ? c.is + 2 : (int) 0x80000000);
+		const int t_lo = wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff);
+		const int t_hi = wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000);
+		if (s_lo <= s_hi) {
+			const long area = (long) (s_hi - s_lo + 1) * (t_hi - t_lo + 1);
+			maxarea = max(maxarea, (int) min(area, (long) 0x7fffffff));
+		}
+	}
+	// wave reduce the count
+	for (int o = 32; o > 0; o >>= 1)
+		cnt += __shfl_xor(cnt, o, 64);
+	any = wave_max(any);
+	if (lane == 0) {
+		atomicAdd(a.count, cnt);
+		atomicMax(a.maxarea, maxarea);
+		if (any)
+			atomicMax(a.fov_any, 1);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// zero fill of the slices outside every z-group, [nz nx ny]
+
+__global__ void fill_kernel(float *image, long ncol, int nz, int zlo, int zhi)
+{
+	// zero z in [0, zlo) and [zhi, nz) of every column
+	const int nout = zlo + (nz - zhi);
+	const long total = ncol * nout;
+	for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < total;
+		i += (long) gridDim.x * blockDim.x) {
+		const long col = i / nout;
+		const int r = (int) (i - col * nout);
+		const int z = (r < zlo) ? r : zhi + (r - zlo);
+		image[col * nz + z] = 0.f;
+	}
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = 0ull;
+}
+
+__global__ void zero_i32_kernel(int *p, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// 'fdk,st,back' (fdk,st.c:38-141): proj [ns nt na], image [nx ny nz].
+// Thread per (x,y) column and a chunk of ST_Z slices; views inner.
+
+constexpr int ST_Z = 16;
+
+__global__ void __launch_bounds__(256)
+st_kernel(const StArgs a)
+{
+	const GeomF &g = a.g;
+	const long nxy = (long) g.nx * g.ny;
+	const long j = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	const int zc = blockIdx.y * ST_Z;
+	if (j >= nxy)
+		return;
+	const int ix = (int) (j % g.nx), iy = (int) (j / g.nx);
+	float acc[ST_Z];
+#pragma unroll
+	for (int k = 0; k < ST_Z; ++k)
+		acc[k] = 0.f;
+	const bool valid = a.mask == nullptr || a.mask[j] != 0;
+	if (valid) {
+		const float yy = g.dy * ((float) iy - g.wy);
+		const float xx = g.dx * ((float) ix - g.wx);
+		const long view_elems = (long) g.ns * g.nt;
+		for (int v = 0; v < g.na; ++v) {
+			const float2 cs = a.trig[v];
+			const float xbeta = xx * cs.x + yy * cs.y;
+			const float ybetas = g.dso - (-xx * cs.y + yy * cs.x);
+			const float mag = (g.kind == KIND_PAR) ? 1.f : g.dsd / ybetas;
+			float ss;
+			if (g.kind == KIND_ARC)
+				ss = (float) ((double) g.dsd * atan2((double) xbeta, (double) ybetas));
+			else
+				ss = mag * xbeta;
+			const float ss_bin = ss / g.ds + g.ws;
+			const int is = to_int_x86(floorf(ss_bin));
+			if (is < 0 || is >= g.ns - 1)
+				continue;
+			const float wr = ss_bin - (float) is, wl = 1.f - wr;
+			const float w2 = (g.kind == KIND_ARC)
+				? (g.dsd * g.dsd) / (ybetas * ybetas + xbeta * xbeta)
+				: mag * mag;
+			const float *view = a.proj + (long) v * view_elems;
+#pragma unroll
+			for (int k = 0; k < ST_Z; ++k) {
+				const int iz = zc + k;
+				if (iz >= g.nz)
+					break;
+				const float zz = g.dz * ((float) iz - g.wz);
+				const float tt = mag * zz;
+				const float tt_bin = tt / g.dt + g.wt;
+				const int it = to_int_x86(floorf(tt_bin));
+				if (it < 0 || it >= g.nt - 1)
+					continue;
+				const float wu = tt_bin - (float) it, wd = 1.f - wu;
+				const float p1 = wl * view[is + it * g.ns] + wr * view[is + 1 + it * g.ns];
+				const float p2 = wl * view[is + (it + 1) * g.ns]
+					+ wr * view[is + 1 + (it + 1) * g.ns];
+				acc[k] += w2 * (wu * p1 + wd * p2);
+			}
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < ST_Z; ++k) {
+		const int iz = zc + k;
+		if (iz < g.nz)
+			a.image[j + (long) iz * nxy] = acc[k];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// per-view quantisation (storage step of the F16 / BF16 paths)
+
+__global__ void __launch_bounds__(256)
+view_maxabs_kernel(const float *proj, long view_elems, int *maxbits)
+{
+	const int v = blockIdx.y;
+	const float *p = proj + (long) v * view_elems;
+	float m = 0.f;
+	for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < view_elems;
+		i += (long) gridDim.x * blockDim.x) {
+		const float x = fabsf(p[i]);
+		m = (x > m || x != x) ? x : m;   // NaN wins
+	}
+	for (int o = 32; o > 0; o >>= 1) {
+		const float y = __shfl_xor(m, o, 64);
+		m = (y > m || y != y) ? y : m;
+	}
+	__shared__ float red[4];
+	if ((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = m;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int w = 1; w < (int) (blockDim.x >> 6); ++w)
+			m = (red[w] > m || red[w] != red[w]) ? red[w] : m;
+		// non-negative floats (and NaN, 0x7fc..) order like their bit patterns
+		atomicMax(maxbits + v, (int) __float_as_uint(m));
+	}
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
+{
+	uint32_t u = __float_as_uint(f);
+	if ((u & 0x7fffffffu) > 0x7f800000u)  // NaN: keep it quiet
+		return (uint16_t) ((u >> 16) | 0x40u);
+	u += 0x7fffu + ((u >> 16) & 1u);
+	return (uint16_t) (u >> 16);
+}
+
+__global__ void __launch_bounds__(256)
+view_quantize_kernel(const float *proj, long view_elems, int fmt,
+	const int *maxbits, void *out, float *scale)
+{
+	const int v = blockIdx.y;
+	const float m = __uint_as_float((uint32_t) maxbits[v]);
+	int e = 0;
+	if (m > 0.f && m <= 3.4e38f) {
+		int ex;
+		frexpf(m, &ex);   // m = f * 2^ex, f in [0.5, 1)
+		e = ex - 15;      // m / 2^e in [2^14, 2^15)
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		scale[v] = ldexpf(1.f, e);
+	const float *p = proj + (long) v * view_elems;
+	for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < view_elems;
+		i += (long) gridDim.x * blockDim.x) {
+		const float q = ldexpf(p[i], -e);
+		if (fmt == FMT_F16)
+			static_cast<__half *>(out)[(long) v * view_elems + i] = __float2half_rn(q);
+		else
+			static_cast<uint16_t *>(out)[(long) v * view_elems + i] = f32_to_bf16_rne(q);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// launchers (host side)
+
+template <int NW, int ZL, int FMT, bool FAST>
+static hipError_t launch_back_t(const BackArgs &a, size_t lds_bytes, hipStream_t s)
+{
+	const int nblocks = a.ntiles * a.ngroups;
+	hipLaunchKernelGGL((back_kernel<NW, ZL, FMT, FAST>), dim3(nblocks), dim3(NW * 64),
+		lds_bytes, s, a);
+	return hipGetLastError();
+}
+
+size_t back_lds_bytes(int nw, int vstage, int slot)
+{
+	// slots + 9 per-(view,column) arrays + T + vinfo, in 4-byte words
+	const size_t words = (size_t) vstage * slot + (size_t) vstage * 64 * 8
+		+ (size_t) vstage * nw * 64 + (size_t) vstage * 4;
+	return words * 4;
+}
+
+#define FDK_DISPATCH_FMT(NW, ZL)                                                  \
+	do {                                                                          \
+		if (fmt == FMT_F32 && !fast) return launch_back_t<NW, ZL, FMT_F32, false>(a, lds, s); \
+		if (fmt == FMT_F32 && fast) return launch_back_t<NW, ZL, FMT_F32, true>(a, lds, s);   \
+		if (fmt == FMT_F16) return launch_back_t<NW, ZL, FMT_F16, true>(a, lds, s);          \
+		if (fmt == FMT_BF16) return launch_back_t<NW, ZL, FMT_BF16, true>(a, lds, s);        \
+	} while (0)
+
+hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s)
+{
+	const size_t lds = back_lds_bytes(nw, a.vstage, a.slot);
+	if (zl == 16) {
+		if (nw == 4) FDK_DISPATCH_FMT(4, 16);
+		if (nw == 8) FDK_DISPATCH_FMT(8, 16);
+	} else if (zl == 8) {
+		if (nw == 4) FDK_DISPATCH_FMT(4, 8);
+		if (nw == 8) FDK_DISPATCH_FMT(8, 8);
+	} else if (zl == 32) {
+		if (nw == 4) FDK_DISPATCH_FMT(4, 32);
+	}
+	return hipErrorInvalidValue;
+}
+
+bool back_config_supported(int nw, int zl)
+{
+	return (zl == 16 && (nw == 4 || nw == 8)) || (zl == 8 && (nw == 4 || nw == 8))
+		|| (zl == 32 && nw == 4);
+}
+
+hipError_t launch_survey(const BackArgs &a, int nw, int zl, hipStream_t s)
+{
+	const int nblocks = a.ntiles * a.ngroups;
+#define FDK_SURVEY(NW, ZL) \
+	if (nw == NW && zl == ZL) { \
+		hipLaunchKernelGGL((survey_kernel<NW, ZL>), dim3(nblocks), dim3(64), 0, s, a); \
+		return hipGetLastError(); }
+	FDK_SURVEY(4, 16)
+	FDK_SURVEY(8, 16)
+	FDK_SURVEY(4, 8)
+	FDK_SURVEY(8, 8)
+	FDK_SURVEY(4, 32)
+#undef FDK_SURVEY
+	return hipErrorInvalidValue;
+}
+
+hipError_t launch_fill(float *image, long ncol, int nz, int zlo, int zhi, hipStream_t s)
+{
+	if (zlo <= 0 && zhi >= nz)
+		return hipSuccess;
+	const long total = ncol * (long) (zlo + (nz - zhi));
+	const int blocks = (int) min((total + 255) / 256, (long) 65536);
+	hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, image, ncol, nz, zlo, zhi);
+	return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s)
+{
+	hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+	return hipGetLastError();
+}
+
+hipError_t launch_zero_i32(int *p, int n, hipStream_t s)
+{
+	hipLaunchKernelGGL(zero_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+	return hipGetLastError();
+}
+
+hipError_t launch_st(const StArgs &a, hipStream_t s)
+{
+	const long nxy = (long) a.g.nx * a.g.ny;
+	dim3 grid((unsigned) ((nxy + 255) / 256), (unsigned) ((a.g.nz + ST_Z - 1) / ST_Z));
+	hipLaunchKernelGGL(st_kernel, grid, dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_quantize(const float *proj, int ns, int nt, int na, int fmt,
+	void *out, float *scale, int *maxbits, hipStream_t s)
+{
+	const long view_elems = (long) ns * nt;
+	hipError_t e = launch_zero_i32(maxbits, na, s);
+	if (e != hipSuccess)
+		return e;
+	const int bx = (int) min((view_elems + 255) / 256, (long) 64);
+	hipLaunchKernelGGL(view_maxabs_kernel, dim3(bx, na), dim3(256), 0, s, proj, view_elems, maxbits);
+	if ((e = hipGetLastError()) != hipSuccess)
+		return e;
+	hipLaunchKernelGGL(view_quantize_kernel, dim3(bx, na), dim3(256), 0, s, proj, view_elems,
+		fmt, (const int *) maxbits, out, scale);
+	return hipGetLastError();
+}
+
+} // namespace fdk
