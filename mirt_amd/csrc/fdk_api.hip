@@ -178,13 +178,15 @@ struct fdkhip_plan {
 	unsigned long long *d_count = nullptr;
 	int *d_maxarea = nullptr;
 	int *d_any = nullptr;
+	void *d_zeros = nullptr;      // zero words: source of staged pad samples
 	// launch configuration
 	int nw = 8, zl = 16, vstage = 8, slot = 0;
 	int z0 = 0, ngroups = 0, tiles_x = 0, tiles_y = 0;
 	int zs_lo = 0, zs_hi = 0;
 	long long n_updates = -1;
 	int maxarea = 0, fov_any = 0;
-	size_t lds_limit = 65536;
+	size_t lds_limit = 65536;   // per workgroup, for the current configuration
+	size_t lds_max = 65536;     // device limit per workgroup
 	// graph cache
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t gexec = nullptr;
@@ -201,6 +203,7 @@ static BackArgs make_args(const fdkhip_plan *p, const void *proj, const float *s
 	memset(&a, 0, sizeof a);
 	a.g = p->gf;
 	a.proj = proj;
+	a.zeros = p->d_zeros;
 	a.scale = scale;
 	a.trig = p->d_trig;
 	a.mask = p->d_mask;
@@ -259,12 +262,14 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 		HIP_TRY(hipMemcpy(&p->fov_any, p->d_any, sizeof(int), hipMemcpyDeviceToHost));
 		p->n_updates = (long long) cnt;
 	}
-	// views per stage and slot size within the LDS budget
+	// views per stage and slot size within the LDS budget: 8 consumer waves
+	// (+4 producers) fill a CU with one workgroup, 4 (+2) fit two
+	p->lds_limit = (nw >= 8) ? p->lds_max : p->lds_max / 2;
 	int slot = std::max(p->maxarea, 64);
 	slot = (slot + 63) & ~63;
 	if (slot_cap > 0)
 		slot = std::max(64, std::min(slot, slot_cap & ~63));
-	int vstage = vstage_req > 0 ? vstage_req : nw;
+	int vstage = vstage_req > 0 ? vstage_req : 16;
 	if (vstage_req > 0) {
 		// keep the requested stage depth, give up slot size first
 		while (slot > 64 && back_lds_bytes(nw, vstage, slot) > p->lds_limit)
@@ -292,6 +297,7 @@ static int plan_free(fdkhip_plan *p)
 	hipFree(p->d_count);
 	hipFree(p->d_maxarea);
 	hipFree(p->d_any);
+	hipFree(p->d_zeros);
 	delete p;
 	return FDKHIP_OK;
 }
@@ -324,7 +330,7 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	int lds = 0;
 	if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device)
 		== hipSuccess && lds > 0)
-		p->lds_limit = std::min((size_t) lds, (size_t) 65536);
+		p->lds_max = (size_t) lds;
 
 #define PLAN_TRY(expr)                                                          \
 	do {                                                                        \
@@ -349,6 +355,8 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	PLAN_TRY(hipMalloc(&p->d_count, sizeof(unsigned long long)));
 	PLAN_TRY(hipMalloc(&p->d_maxarea, sizeof(int)));
 	PLAN_TRY(hipMalloc(&p->d_any, sizeof(int)));
+	PLAN_TRY(hipMalloc(&p->d_zeros, 256));
+	PLAN_TRY(hipMemset(p->d_zeros, 0, 256));
 #undef PLAN_TRY
 
 	support_z(g, mask_host, p->zs_lo, p->zs_hi);

@@ -22,6 +22,7 @@ struct GeomF {
 struct BackArgs {
 	GeomF g;
 	const void *proj;        // [na][ns][nt] (t fastest), storage format fmt
+	const void *zeros;       // >= 4 zero bytes: source of the staged pad samples
 	const float *scale;      // [na] per-view scale (F16/BF16), else unused
 	const float2 *trig;      // [na] {cosf(beta), sinf(beta)} of the float beta
 	const uint8_t *mask;     // [ny][nx] or null

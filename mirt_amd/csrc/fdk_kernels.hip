@@ -55,6 +55,15 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
 }
 
 template <int FMT>
+__device__ __forceinline__ const void *view_ptr(const void *proj, long elem)
+{
+	if constexpr (FMT == FMT_F32)
+		return static_cast<const float *>(proj) + elem;
+	else
+		return static_cast<const uint16_t *>(proj) + elem;
+}
+
+template <int FMT>
 __device__ __forceinline__ float load_sample(const void *proj, long idx)
 {
 	if constexpr (FMT == FMT_F32)
@@ -125,50 +134,70 @@ __device__ __forceinline__ int column_setup(const GeomF &g, float xx, float yy,
 // Walk of the reference's t_bin sequence over one z-group [zg0, zg1):
 // t_bin starts at iz_min and grows by repeated addition of t_inc
 // (fdk,ts.c:169,244), so the value at slice z is reproduced only by the same
-// sequence of additions.  Records, for each of the NW slice chunks of ZL
-// slices, t_bin at the chunk's first active slice, and returns the effective
-// range [zlo, zhi) and t_bin at its ends.  Slices whose t_bin is <= -1 or
-// >= nt are dropped: every sample they would read is a zero pad, so the
-// reference adds exactly 0 there.
+// sequence of additions.  Records, for each of the NW chunks of ZL slices,
+// t_bin at the chunk's first active slice, and returns the effective range
+// [zlo, zhi) with t_bin at zlo (exact) and an upper bound of t_bin at zhi-1.
+// Slices whose t_bin is <= -1 or >= nt are dropped: every sample they would
+// read is a zero pad, so the reference adds exactly 0 there.
 template <int NW, int ZL>
 __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 	int zg0, int zg1, float (&Tk)[NW], int &zlo, int &zhi,
 	float &t_first, float &t_last)
 {
 	const int lo = max(c.iz_min, zg0), hi = min(c.iz_max, zg1);
+	const float ti = c.t_inc;
 	float t = c.t_bin;
 	int z = c.iz_min;
-	for (; z < lo; ++z)
-		t += c.t_inc;
-	while (z < hi && t <= -1.f) {
-		t += c.t_inc;
+	for (; z < lo; ++z)           // z-groups above the first: walk up to zg0
+		t += ti;
+	while (z < hi && t <= -1.f) { // rare: leading zero-contribution slices
+		t += ti;
 		++z;
 	}
 	zlo = z;
 	zhi = hi;
 	t_first = t;
-	t_last = t;
+#pragma unroll
+	for (int k = 0; k < NW; ++k)
+		Tk[k] = 0.f;
+	if (zlo >= zhi) {
+		t_last = t;
+		return false;
+	}
+	// chunk holding zlo, and the additions up to the next chunk boundary
+	const int k0 = (zlo - zg0) / ZL;
+	const int n0 = zg0 + (k0 + 1) * ZL - zlo;     // 1..ZL
 	float tt = t;
 #pragma unroll
-	for (int k = 0; k < NW; ++k) {
-		const int zs = max(zg0 + k * ZL, zlo);
-		const int ze = min(zg0 + (k + 1) * ZL, zhi);
-		Tk[k] = 0.f;
-		if (zs < ze) {
+	for (int k = 0; k < NW; ++k)
+		if (k == k0)
+			Tk[k] = t;
+#pragma unroll
+	for (int j = 0; j < ZL; ++j)
+		tt = (j < n0) ? tt + ti : tt;              // adding nothing leaves tt exact
+	// later chunks: ZL additions each (lanes past their range add harmlessly)
+#pragma unroll
+	for (int k = 1; k < NW; ++k) {
+		if (k > k0) {
 			Tk[k] = tt;
-			for (int zz = zs; zz < ze - 1; ++zz)
-				tt += c.t_inc;
-			t_last = tt;
-			tt += c.t_inc;
+#pragma unroll
+			for (int j = 0; j < ZL; ++j)
+				tt += ti;
 		}
 	}
-	if (zlo < zhi && t_last >= (float) g.nt) {
-		// rare (rounding at the last slice): trim the zero-contribution tail
+	// upper bound of t_bin at zhi - 1: m additions of ti each round to within
+	// half an ulp of the running value (|t| < 2^24 here)
+	const float m = (float) (zhi - 1 - zlo);
+	const float est = fmaf(m, ti, t_first);
+	const float mag = fmaxf(fmaxf(fabsf(t_first), fabsf(est)), 1.f);
+	t_last = est + (m + 1.f) * mag * 1.2e-7f + 1e-6f;
+	if (t_last >= (float) g.nt) {
+		// maybe a trailing zero-contribution slice: exact walk to trim it
 		float t2 = t_first, tl = t_first;
 		int z2 = zlo;
 		while (z2 < zhi && t2 < (float) g.nt) {
 			tl = t2;
-			t2 += c.t_inc;
+			t2 += ti;
 			++z2;
 		}
 		zhi = z2;
@@ -206,20 +235,23 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks)
 // ---------------------------------------------------------------------------
 // main kernel
 
-// LDS layout (floats / ints), all per stage:
-//   slots   [vstage][slot]      staged footprints (FP32)
-//   P_cb    [vstage][64]  int   LDS index of sample (s = is, t = 0) in its slot
+// dynamic LDS of the main kernel; every access below indexes this array
+// with a word offset so that the compiler emits LDS (ds_*) instructions
+extern __shared__ float fdk_lds[];
+
+// LDS layout of one stage buffer (4-byte words); the kernel holds two.
+//   slots   [vstage][slot]       staged footprints (FP32)
+//   P_cb    [vstage][64]  int    LDS word index of sample (s = is, t = 0)
 //   P_wl, P_wr, P_w2, P_tinc [vstage][64] float
-//   P_zlo, P_zhi [vstage][64] int   effective slice range [zlo, zhi)
+//   P_zlo, P_zhi [vstage][64] int    effective slice range [zlo, zhi)
 //   P_is    [vstage][64]  int
-//   P_T     [vstage][NW][64] float  t_bin at each wave's first active slice
-//   V_info  [vstage][4]   int   s_lo, t_lo, tw, mode
+//   P_T     [vstage][NW][64] float   t_bin at each consumer wave's first slice
+//   V_info  [vstage][4]   int    s_lo, t_lo, tw, (mode << 16) | s-width
 template <int NW>
 struct Lds {
-	int slot_off, cb, wl, wr, w2, tinc, zlo, zhi, is, T, vinfo, total;
+	int cb, wl, wr, w2, tinc, zlo, zhi, is, T, vinfo, total;
 	__device__ __forceinline__ Lds(int vstage, int slot)
 	{
-		slot_off = 0;
 		cb = vstage * slot;
 		wl = cb + vstage * 64;
 		wr = wl + vstage * 64;
@@ -236,15 +268,239 @@ struct Lds {
 
 enum { MODE_LDS = 0, MODE_GLOBAL = 1, MODE_NONE = 2 };
 
+
+// Producer side: per-(column, view) setup of `nv` views into the stage
+// buffer at word offset `bo`, and staging of their footprints, by producer
+// wave `pw` of NP.  The footprint goes to LDS with asynchronous
+// global-to-LDS loads (one dword per lane: the FP32 sample, or the 16-bit
+// sample zero-extended); pad samples are read from `zeros`.  The caller
+// waits for the loads (vmcnt) before the stage barrier.
+template <int NW, int NP, int ZL, int FMT, bool FAST>
+__device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW> &L,
+	int v0, int nv, int pw, int lane, bool colvalid, float xx, float yy, int zg0, int zg1)
+{
+	const GeomF &g = a.g;
+	float *buf = fdk_lds + bo;
+	int *bufi = reinterpret_cast<int *>(fdk_lds) + bo;
+	const long view_elems = (long) g.ns * g.nt;
+	for (int vv = pw; vv < nv; vv += NP) {
+		const int v = v0 + vv;
+		const float2 cs = a.trig[v];
+		ColSetup c;
+		bool act = colvalid && column_setup(g, xx, yy, cs.x, cs.y, c) == SETUP_ACTIVE;
+		int zlo = 0, zhi = 0;
+		float t_first = 0.f, t_last = 0.f;
+		float Tk[NW];
+		if (act)
+			act = setup_walk<NW, ZL>(g, c, zg0, zg1, Tk, zlo, zhi, t_first, t_last);
+		// footprint of this view over the tile and z-group (padded coords)
+		const int s_lo = wave_min(act ? c.is + 1 : 0x7fffffff);
+		const int s_hi = wave_max(act ? c.is + 2 : (int) 0x80000000);
+		const int t_lo = wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff);
+		const int t_hi = min(wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000),
+			g.nt + 1);
+		int mode = MODE_NONE, tw = 0, sw = 0;
+		if (s_lo <= s_hi) {
+			tw = t_hi - t_lo + 1;
+			sw = s_hi - s_lo + 1;
+			mode = ((long) sw * tw <= a.slot) ? MODE_LDS : MODE_GLOBAL;
+		}
+		const int idx = vv * 64 + lane;
+		bufi[L.cb + idx] = act ? vv * a.slot + (c.is + 1 - s_lo) * tw + (1 - t_lo) : 0;
+		float wl = act ? c.wl : 0.f, wr = act ? c.wr : 0.f;
+		const float w2 = act ? c.w2 : 0.f;
+		if (FAST) {
+			const float sc = (FMT != FMT_F32) ? a.scale[v] : 1.f;
+			wl = w2 * wl * sc;
+			wr = w2 * wr * sc;
+		}
+		buf[L.wl + idx] = wl;
+		buf[L.wr + idx] = wr;
+		buf[L.w2 + idx] = w2;
+		buf[L.tinc + idx] = act ? c.t_inc : 0.f;
+		bufi[L.zlo + idx] = zlo;
+		bufi[L.zhi + idx] = act ? zhi : zlo;
+		bufi[L.is + idx] = act ? c.is : 0;
+#pragma unroll
+		for (int k = 0; k < NW; ++k)
+			buf[L.T + (vv * NW + k) * 64 + lane] = act ? Tk[k] : 0.f;
+		if (lane == 0) {
+			bufi[L.vinfo + vv * 4 + 0] = s_lo;
+			bufi[L.vinfo + vv * 4 + 1] = t_lo;
+			bufi[L.vinfo + vv * 4 + 2] = tw;
+			bufi[L.vinfo + vv * 4 + 3] = (mode << 16) | (sw & 0xffff);
+		}
+		if (mode != MODE_LDS)
+			continue;
+		// stage the zero-padded footprint (fdk_ts_put_view, fdk,ts,t.c:13-21):
+		// LDS word e of the slot <- padded sample (s_lo + e / tw, t_lo + e % tw)
+		const int area = sw * tw;
+		const unsigned magic = 0xffffffffu / (unsigned) tw + 1u;   // e/tw, exact (tw >= 2)
+		const int slot = bo + vv * a.slot;
+		const char *vp = static_cast<const char *>(view_ptr<FMT>(a.proj, (long) v * view_elems));
+		constexpr int esz = (FMT == FMT_F32) ? 4 : 2;
+		for (int e0 = 0; e0 < area; e0 += 64) {
+			const int e = e0 + lane;
+			int r = (int) __umulhi((unsigned) e, magic);
+			r -= (r * tw > e) ? 1 : 0;
+			const int tc = e - r * tw;
+			const int sc = s_lo + r - 1;      // unpadded column
+			const int tr = t_lo + tc - 1;     // unpadded row
+			const bool inb = sc >= 0 && sc < g.ns && tr >= 0 && tr < g.nt;
+			const char *src = inb ? vp + (size_t) (unsigned) (sc * g.nt + tr) * esz
+				: static_cast<const char *>(a.zeros);
+			if (e < area) {
+				const auto gsrc = (const void __attribute__((address_space(1))) *) src;
+				const auto ldst = (void __attribute__((address_space(3))) *) (fdk_lds + slot + e0);
+				if constexpr (FMT == FMT_F32)
+					__builtin_amdgcn_global_load_lds(gsrc, ldst, 4, 0, 0);
+				else
+					__builtin_amdgcn_global_load_lds(gsrc, ldst, 2, 0, 0);
+			}
+		}
+	}
+}
+
+// FP32 value of a staged LDS word
+template <int FMT>
+__device__ __forceinline__ float lds_sample(int w)
+{
+	if constexpr (FMT == FMT_F32)
+		return fdk_lds[w];
+	else if constexpr (FMT == FMT_F16)
+		return __half2float(__ushort_as_half((unsigned short) __float_as_uint(fdk_lds[w])));
+	else
+		return __uint_as_float(__float_as_uint(fdk_lds[w]) << 16);
+}
+
+// One slice update: bilinear interpolation at (is, t) from the two staged
+// columns c1 (s = is) and c2 (s = is+1), added to acc.  FAST folds w2 (and
+// the view scale) into wl/wr and uses fused multiply-adds; EXACT follows
+// fdk,ts.c:199-241 operation by operation.
+template <int FMT, bool FAST>
+__device__ __forceinline__ float slice_update(float acc, float t, int c1, int c2,
+	float wl, float wr, float w2)
+{
+	const float fl = floorf(t);
+	const int it = (int) fl;
+	const float wu = t - fl;
+	const float wd = 1.f - wu;
+	const float a0 = lds_sample<FMT>(c1 + it), a1 = lds_sample<FMT>(c1 + it + 1);
+	const float b0 = lds_sample<FMT>(c2 + it), b1 = lds_sample<FMT>(c2 + it + 1);
+	if (FAST) {
+		const float q1 = fmaf(wr, b0, fmaf(wl, a0, 0.f));
+		const float q2 = fmaf(wr, b1, fmaf(wl, a1, 0.f));
+		acc = fmaf(wd, q1, acc);
+		return fmaf(wu, q2, acc);
+	} else {
+		const float q1 = wl * a0 + wr * b0;
+		const float q2 = wl * a1 + wr * b1;
+		const float p0 = wd * q1 + wu * q2;
+		return acc + w2 * p0;
+	}
+}
+
+// Consumer side: consumer wave `wave` adds the views of the stage buffer at
+// `bo` to its ZL register accumulators (slices [zc, zc + ZL) of its column).
 template <int NW, int ZL, int FMT, bool FAST>
-__global__ void __launch_bounds__(NW * 64)
+__device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW> &L,
+	int v0, int nv, int wave, int lane, int zc, float (&acc)[ZL])
+{
+	const GeomF &g = a.g;
+	const float *buf = fdk_lds + bo;
+	const int *bufi = reinterpret_cast<const int *>(fdk_lds) + bo;
+	const long view_elems = (long) g.ns * g.nt;
+	for (int vv = 0; vv < nv; ++vv) {
+		const int mode = bufi[L.vinfo + vv * 4 + 3] >> 16;
+		if (mode == MODE_NONE)
+			continue;
+		const int idx = vv * 64 + lane;
+		const int zlo = bufi[L.zlo + idx];
+		const int zhi = bufi[L.zhi + idx];
+		const int jlo = max(zlo - zc, 0);
+		const int jhi = min(zhi - zc, ZL);
+		const bool any = jlo < jhi;
+		if (!__any(any))
+			continue;                  // no lane of this wave meets the view
+		const float t0 = buf[L.T + (vv * NW + wave) * 64 + lane];
+		const float ti = buf[L.tinc + idx];
+		const float wl = buf[L.wl + idx];
+		const float wr = buf[L.wr + idx];
+		const float w2 = buf[L.w2 + idx];
+		if (mode == MODE_LDS) {
+			const int tw = bufi[L.vinfo + vv * 4 + 2];
+			const int c1 = bo + bufi[L.cb + idx];
+			const int c2 = c1 + tw;
+			float t = t0;
+			if (__all(jlo == 0 && jhi == ZL)) {
+				// every lane covers the whole chunk: straight-line slices
+#pragma unroll
+				for (int j = 0; j < ZL; ++j) {
+					acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
+					t += ti;
+				}
+			} else {
+				// partial chunk: slices outside [jlo, jhi) read at t0 (inside
+				// the staged footprint) and are discarded by a select
+				const unsigned nj = (unsigned) (jhi - jlo);
+#pragma unroll
+				for (int j = 0; j < ZL; ++j) {
+					const bool r = (unsigned) (j - jlo) < nj;
+					const float tv = r ? t : t0;
+					const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
+					acc[j] = r ? up : acc[j];
+					t = r ? t + ti : t;
+				}
+			}
+		} else if (any) {
+			// footprint larger than the LDS slot: gather from HBM
+			const int v = v0 + vv;
+			const int is = bufi[L.is + idx];
+			const float sc = (FMT != FMT_F32 && !FAST) ? a.scale[v] : 1.f;
+			const void *vp = view_ptr<FMT>(a.proj, (long) v * view_elems);
+			float t = t0;
+#pragma unroll
+			for (int j = 0; j < ZL; ++j) {
+				if (j >= jlo && j < jhi) {
+					const int it = (int) floorf(t);
+					const float wu = t - (float) it;
+					const float wd = 1.f - wu;
+					float s4[4];
+#pragma unroll
+					for (int q = 0; q < 4; ++q) {
+						const int s = is + (q >> 1), tt = it + (q & 1);
+						s4[q] = (s >= 0 && s < g.ns && tt >= 0 && tt < g.nt)
+							? load_sample<FMT>(vp, s * g.nt + tt) * sc : 0.f;
+					}
+					if (FAST) {
+						const float q1 = fmaf(wr, s4[2], wl * s4[0]);
+						const float q2 = fmaf(wr, s4[3], wl * s4[1]);
+						acc[j] = fmaf(wd, q1, acc[j]);
+						acc[j] = fmaf(wu, q2, acc[j]);
+					} else {
+						const float q1 = wl * s4[0] + wr * s4[2];
+						const float q2 = wl * s4[1] + wr * s4[3];
+						const float p0 = wd * q1 + wu * q2;
+						acc[j] += w2 * p0;
+					}
+					t += ti;
+				}
+			}
+		}
+	}
+}
+
+// Workgroup = NW consumer waves (z chunks of ZL slices, lane = column) and
+// NP producer waves.  Views go in stages of `vstage` through two LDS
+// buffers: while the consumers add stage s from one buffer, the producers
+// set up and stage s+1 into the other; one barrier per stage.
+template <int NW, int NP, int ZL, int FMT, bool FAST>
+__global__ void __launch_bounds__((NW + NP) * 64)
 back_kernel(const BackArgs a)
 {
-	extern __shared__ float smem[];
-	int *smem_i = reinterpret_cast<int *>(smem);
 	const GeomF &g = a.g;
 	const int tid = threadIdx.x;
-	const int wave = tid >> 6;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int lane = tid & 63;
 
 	const int nblocks = a.ntiles * a.ngroups;
@@ -254,197 +510,55 @@ back_kernel(const BackArgs a)
 	const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
 	const int zg0 = a.z0 + group * (NW * ZL);
 	const int zg1 = min(zg0 + NW * ZL, g.nz);
-
 	const int ix = tx * 8 + (lane & 7);
 	const int iy = ty * 8 + (lane >> 3);
-	const bool colvalid = ix < g.nx && iy < g.ny &&
-		(a.mask == nullptr || a.mask[(long) iy * g.nx + ix] != 0);
-	// fdk,ts.c:114,120
-	const float yy = g.dy * ((float) iy - g.wy);
-	const float xx = g.dx * ((float) ix - g.wx);
 
 	const Lds<NW> L(a.vstage, a.slot);
-	const int zc = zg0 + wave * ZL;   // this wave's first slice
+	const int nstage = (g.na + a.vstage - 1) / a.vstage;
 
-	float acc[ZL];
-#pragma unroll
-	for (int j = 0; j < ZL; ++j)
-		acc[j] = 0.f;
-
-	const long view_elems = (long) g.ns * g.nt;
-
-	for (int v0 = 0; v0 < g.na; v0 += a.vstage) {
-		const int nv = min(a.vstage, g.na - v0);
-
-		// ---------------- setup: one lane per (view, column)
-		for (int vv = wave; vv < nv; vv += NW) {
-			const int v = v0 + vv;
-			const float2 cs = a.trig[v];
-			ColSetup c;
-			bool act = colvalid && column_setup(g, xx, yy, cs.x, cs.y, c) == SETUP_ACTIVE;
-			int zlo = 0, zhi = 0;
-			float t_first = 0.f, t_last = 0.f;
-			float Tk[NW];
-#pragma unroll
-			for (int k = 0; k < NW; ++k)
-				Tk[k] = 0.f;
-			if (act)
-				act = setup_walk<NW, ZL>(g, c, zg0, zg1, Tk, zlo, zhi, t_first, t_last);
-			// footprint of this view over the tile and z-group (padded coords)
-			const int s_lo = wave_min(act ? c.is + 1 : 0x7fffffff);
-			const int s_hi = wave_max(act ? c.is + 2 : (int) 0x80000000);
-			const int t_lo = wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff);
-			const int t_hi = wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000);
-			int mode = MODE_NONE, tw = 0;
-			if (s_lo <= s_hi) {
-				tw = t_hi - t_lo + 1;
-				const long area = (long) (s_hi - s_lo + 1) * tw;
-				mode = (area <= a.slot) ? MODE_LDS : MODE_GLOBAL;
-			}
-			const int idx = vv * 64 + lane;
-			smem_i[L.cb + idx] = act ? vv * a.slot + (c.is + 1 - s_lo) * tw + (1 - t_lo) : 0;
-			float wl = act ? c.wl : 0.f, wr = act ? c.wr : 0.f;
-			const float w2 = act ? c.w2 : 0.f;
-			if (FAST) {
-				float sc = 1.f;
-				if (FMT != FMT_F32)
-					sc = a.scale[v];
-				wl = w2 * wl * sc;
-				wr = w2 * wr * sc;
-			}
-			smem[L.wl + idx] = wl;
-			smem[L.wr + idx] = wr;
-			smem[L.w2 + idx] = w2;
-			smem[L.tinc + idx] = act ? c.t_inc : 0.f;
-			smem_i[L.zlo + idx] = zlo;
-			smem_i[L.zhi + idx] = act ? zhi : zlo;
-			smem_i[L.is + idx] = act ? c.is : 0;
-#pragma unroll
-			for (int k = 0; k < NW; ++k)
-				smem[L.T + (vv * NW + k) * 64 + lane] = Tk[k];
-			if (lane == 0) {
-				smem_i[L.vinfo + vv * 4 + 0] = s_lo;
-				smem_i[L.vinfo + vv * 4 + 1] = t_lo;
-				smem_i[L.vinfo + vv * 4 + 2] = tw;
-				smem_i[L.vinfo + vv * 4 + 3] = (mode << 16) | (s_hi - s_lo + 1);
-			}
-		}
+	if (wave >= NW) {
+		// ------------------------------------------------ producer waves
+		const int pw = wave - NW;
+		const bool colvalid = ix < g.nx && iy < g.ny &&
+			(a.mask == nullptr || a.mask[(long) iy * g.nx + ix] != 0);
+		const float yy = g.dy * ((float) iy - g.wy);   // fdk,ts.c:114,120
+		const float xx = g.dx * ((float) ix - g.wx);
+		produce<NW, NP, ZL, FMT, FAST>(a, 0, L, 0, min(a.vstage, g.na), pw, lane,
+			colvalid, xx, yy, zg0, zg1);
+		__builtin_amdgcn_s_waitcnt(0);   // staged footprints have landed in LDS
 		__syncthreads();
-
-		// ---------------- stage footprints into LDS (zero-padded view)
-		for (int vv = 0; vv < nv; ++vv) {
-			const int ms = smem_i[L.vinfo + vv * 4 + 3];
-			if ((ms >> 16) != MODE_LDS)
-				continue;
-			const int sw = ms & 0xffff;
-			const int s_lo = smem_i[L.vinfo + vv * 4 + 0];
-			const int t_lo = smem_i[L.vinfo + vv * 4 + 1];
-			const int tw = smem_i[L.vinfo + vv * 4 + 2];
-			float *slot = smem + vv * a.slot;
-			const long vbase = (long) (v0 + vv) * view_elems;
-			// waves take padded s columns, lanes run along t (coalesced in HBM)
-			for (int r = wave; r < sw; r += NW) {
-				const int s = s_lo + r - 1;    // unpadded column
-				for (int tc = lane; tc < tw; tc += 64) {
-					const int t = t_lo + tc - 1;   // unpadded row
-					float val = 0.f;
-					if (s >= 0 && s < g.ns && t >= 0 && t < g.nt)
-						val = load_sample<FMT>(a.proj, vbase + (long) s * g.nt + t);
-					slot[r * tw + tc] = val;
-				}
+		for (int s = 0; s < nstage; ++s) {
+			if (s + 1 < nstage) {
+				const int v0 = (s + 1) * a.vstage;
+				produce<NW, NP, ZL, FMT, FAST>(a, ((s + 1) & 1) * L.total, L, v0,
+					min(a.vstage, g.na - v0), pw, lane, colvalid, xx, yy, zg0, zg1);
 			}
+			__builtin_amdgcn_s_waitcnt(0);
+			__syncthreads();
 		}
+	} else {
+		// ------------------------------------------------ consumer waves
+		const int zc = zg0 + wave * ZL;
+		float acc[ZL];
+#pragma unroll
+		for (int j = 0; j < ZL; ++j)
+			acc[j] = 0.f;
 		__syncthreads();
-
-		// ---------------- update: this wave's ZL slices of its column
-		for (int vv = 0; vv < nv; ++vv) {
-			const int mode = smem_i[L.vinfo + vv * 4 + 3] >> 16;
-			if (mode == MODE_NONE)
-				continue;
-			const int idx = vv * 64 + lane;
-			const int zlo = smem_i[L.zlo + idx];
-			const int zhi = smem_i[L.zhi + idx];
-			const int jlo = max(zlo - zc, 0);
-			const int jhi = min(zhi - zc, ZL);
-			if (jlo >= jhi)
-				continue;
-			float t = smem[L.T + (vv * NW + wave) * 64 + lane];
-			const float t_inc = smem[L.tinc + idx];
-			const float wl = smem[L.wl + idx];
-			const float wr = smem[L.wr + idx];
-			const float w2 = smem[L.w2 + idx];
-			if (mode == MODE_LDS) {
-				const int tw = smem_i[L.vinfo + vv * 4 + 2];
-				const float *p1 = smem + smem_i[L.cb + idx];
-				const float *p2 = p1 + tw;
+		for (int s = 0; s < nstage; ++s) {
+			const int v0 = s * a.vstage;
+			consume<NW, ZL, FMT, FAST>(a, (s & 1) * L.total, L, v0, min(a.vstage, g.na - v0),
+				wave, lane, zc, acc);
+			__syncthreads();
+		}
+		// one write per voxel, [nz nx ny]
+		if (ix < g.nx && iy < g.ny) {
+			float *col = a.image + ((long) iy * g.nx + ix) * g.nz;
 #pragma unroll
-				for (int j = 0; j < ZL; ++j) {
-					if (j >= jlo && j < jhi) {
-						const int it = (int) floorf(t);
-						const float wu = t - (float) it;
-						const float wd = 1.f - wu;
-						const float a0 = p1[it], a1 = p1[it + 1];
-						const float b0 = p2[it], b1 = p2[it + 1];
-						if (FAST) {
-							const float q1 = fmaf(wr, b0, wl * a0);
-							const float q2 = fmaf(wr, b1, wl * a1);
-							acc[j] = fmaf(wd, q1, acc[j]);
-							acc[j] = fmaf(wu, q2, acc[j]);
-						} else {
-							const float q1 = wl * a0 + wr * b0;
-							const float q2 = wl * a1 + wr * b1;
-							const float p0 = wd * q1 + wu * q2;
-							acc[j] += w2 * p0;
-						}
-						t += t_inc;
-					}
-				}
-			} else {
-				// footprint larger than the LDS slot: gather from HBM
-				const int v = v0 + vv;
-				const int is = smem_i[L.is + idx];
-				const long vbase = (long) v * view_elems;
-#pragma unroll
-				for (int j = 0; j < ZL; ++j) {
-					if (j >= jlo && j < jhi) {
-						const int it = (int) floorf(t);
-						const float wu = t - (float) it;
-						const float wd = 1.f - wu;
-						float s4[4];
-#pragma unroll
-						for (int q = 0; q < 4; ++q) {
-							const int s = is + (q >> 1), tt = it + (q & 1);
-							s4[q] = (s >= 0 && s < g.ns && tt >= 0 && tt < g.nt)
-								? load_sample<FMT>(a.proj, vbase + (long) s * g.nt + tt)
-								: 0.f;
-						}
-						if (FAST) {
-							const float q1 = fmaf(wr, s4[2], wl * s4[0]);
-							const float q2 = fmaf(wr, s4[3], wl * s4[1]);
-							acc[j] = fmaf(wd, q1, acc[j]);
-							acc[j] = fmaf(wu, q2, acc[j]);
-						} else {
-							const float q1 = wl * s4[0] + wr * s4[2];
-							const float q2 = wl * s4[1] + wr * s4[3];
-							const float p0 = wd * q1 + wu * q2;
-							acc[j] += w2 * p0;
-						}
-						t += t_inc;
-					}
-				}
+			for (int j = 0; j < ZL; ++j) {
+				const int z = zc + j;
+				if (z < zg1)
+					col[z] = acc[j];
 			}
-		}
-		__syncthreads();
-	}
-
-	// ---------------- one write per voxel, [nz nx ny]
-	if (ix < g.nx && iy < g.ny) {
-		float *col = a.image + ((long) iy * g.nx + ix) * g.nz;
-#pragma unroll
-		for (int j = 0; j < ZL; ++j) {
-			const int z = zc + j;
-			if (z < zg1)
-				col[z] = acc[j];
 		}
 	}
 }
@@ -494,7 +608,8 @@ survey_kernel(const BackArgs a)
 		const int s_lo = wave_min(act ? c.is + 1 : 0x7fffffff);
 		const int s_hi = wave_max(act ? c.is + 2 : (int) 0x80000000);
 		const int t_lo = wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff);
-		const int t_hi = wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000);
+		const int t_hi = min(wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000),
+			g.nt + 1);
 		if (s_lo <= s_hi) {
 			const long area = (long) (s_hi - s_lo + 1) * (t_hi - t_lo + 1);
 			maxarea = max(maxarea, (int) min(area, (long) 0x7fffffff));
@@ -681,33 +796,41 @@ view_quantize_kernel(const float *proj, long view_elems, int fmt,
 // ---------------------------------------------------------------------------
 // launchers (host side)
 
-template <int NW, int ZL, int FMT, bool FAST>
+template <int NW, int NP, int ZL, int FMT, bool FAST>
 static hipError_t launch_back_t(const BackArgs &a, size_t lds_bytes, hipStream_t s)
 {
 	const int nblocks = a.ntiles * a.ngroups;
-	hipLaunchKernelGGL((back_kernel<NW, ZL, FMT, FAST>), dim3(nblocks), dim3(NW * 64),
-		lds_bytes, s, a);
+	hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST>), dim3(nblocks),
+		dim3((NW + NP) * 64), lds_bytes, s, a);
 	return hipGetLastError();
 }
 
+// two stage buffers of: slots + 8 per-(view,column) arrays + T + vinfo (words)
 size_t back_lds_bytes(int nw, int vstage, int slot)
 {
-	// slots + 9 per-(view,column) arrays + T + vinfo, in 4-byte words
 	const size_t words = (size_t) vstage * slot + (size_t) vstage * 64 * 8
 		+ (size_t) vstage * nw * 64 + (size_t) vstage * 4;
-	return words * 4;
+	return 2 * words * 4;
 }
 
-#define FDK_DISPATCH_FMT(NW, ZL)                                                  \
-	do {                                                                          \
-		if (fmt == FMT_F32 && !fast) return launch_back_t<NW, ZL, FMT_F32, false>(a, lds, s); \
-		if (fmt == FMT_F32 && fast) return launch_back_t<NW, ZL, FMT_F32, true>(a, lds, s);   \
-		if (fmt == FMT_F16) return launch_back_t<NW, ZL, FMT_F16, true>(a, lds, s);          \
-		if (fmt == FMT_BF16) return launch_back_t<NW, ZL, FMT_BF16, true>(a, lds, s);        \
+// producer waves per workgroup for a consumer count
+static int producers_for(int nw)
+{
+	return nw >= 8 ? 4 : 2;
+}
+
+#define FDK_DISPATCH_FMT(NW, ZL)                                                               \
+	do {                                                                                       \
+		constexpr int NP_ = (NW) >= 8 ? 4 : 2;                                                 \
+		if (fmt == FMT_F32 && !fast) return launch_back_t<NW, NP_, ZL, FMT_F32, false>(a, lds, s); \
+		if (fmt == FMT_F32 && fast) return launch_back_t<NW, NP_, ZL, FMT_F32, true>(a, lds, s);   \
+		if (fmt == FMT_F16) return launch_back_t<NW, NP_, ZL, FMT_F16, true>(a, lds, s);          \
+		if (fmt == FMT_BF16) return launch_back_t<NW, NP_, ZL, FMT_BF16, true>(a, lds, s);        \
 	} while (0)
 
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s)
 {
+	(void) producers_for;
 	const size_t lds = back_lds_bytes(nw, a.vstage, a.slot);
 	if (zl == 16) {
 		if (nw == 4) FDK_DISPATCH_FMT(4, 16);

@@ -78,33 +78,43 @@ def ellipsoid_proj(cg: CTGeom, params, *, oversample: int = 1, device="cpu",
         tt = (off[:, None] * dt + tt[None, :]).T.reshape(-1)
     sss, ttt = torch.meshgrid(ss, tt, indexing="ij")
     uu, vv, azim0, polar = _coords(cg, sss, ttt)
-    cpol, spol = torch.cos(polar), torch.sin(polar)
+    cpol, spol = torch.cos(polar)[..., None], torch.sin(polar)[..., None]
+    uu, vv, azim0 = uu[..., None], vv[..., None], azim0[..., None]
     na = len(beta)
     out = torch.zeros((sss.shape[0], sss.shape[1], na), dtype=dtype, device=device)
-    for p in params:
-        cx, cy, cz, rx, ry, rz = (float(v) for v in p[:6])
-        xang = math.radians(p[6])
-        if p[7] != 0:
-            raise ValueError("z rotation not done")
-        val = float(p[8])
-        for ib in range(na):
-            az = float(beta[ib]) + azim0
-            cz_eff = cz - float(szs[ib])
+    # views in chunks: every elementwise step is over [ns, nt, views]
+    chunk = max(1, int(2 ** 24 // max(1, sss.numel())))
+    for v0 in range(0, na, chunk):
+        v1 = min(na, v0 + chunk)
+        bt = torch.as_tensor(np.asarray(beta[v0:v1], dtype=np.float64), dtype=dtype,
+                             device=device)
+        zs = torch.as_tensor(np.asarray(szs[v0:v1], dtype=np.float64), dtype=dtype,
+                             device=device)
+        acc = out[:, :, v0:v1]
+        for p in params:
+            cx, cy, cz, rx, ry, rz = (float(v) for v in p[:6])
+            xang = math.radians(p[6])
+            if p[7] != 0:
+                raise ValueError("z rotation not done")
+            val = float(p[8])
+            az = bt + azim0                      # [ns, nt, nv]
+            cz_eff = cz - zs
             ushift = cx * torch.cos(az) + cy * torch.sin(az)
             vshift = (cx * torch.sin(az) - cy * torch.cos(az)) * spol + cz_eff * cpol
             az = az - xang
+            caz, saz = torch.cos(az), torch.sin(az)
             du, dv = uu - ushift, vv - vshift
-            p1 = du * torch.cos(az) + dv * torch.sin(az) * spol
-            p2 = du * torch.sin(az) - dv * torch.cos(az) * spol
+            p1 = du * caz + dv * saz * spol
+            p2 = du * saz - dv * caz * spol
             p3 = dv * cpol
-            e1 = -torch.sin(az) * cpol
-            e2 = torch.cos(az) * cpol
+            e1 = -saz * cpol
+            e2 = caz * cpol
             e3 = spol
             A = e1 * e1 / rx ** 2 + e2 * e2 / ry ** 2 + e3 * e3 / rz ** 2
             B = p1 * e1 / rx ** 2 + p2 * e2 / ry ** 2 + p3 * e3 / rz ** 2
             Cc = p1 * p1 / rx ** 2 + p2 * p2 / ry ** 2 + p3 * p3 / rz ** 2 - 1
             # real part of 2*val*sqrt(B^2 - A C)/A: zero where the ray misses
-            out[:, :, ib] += 2 * val * torch.sqrt(torch.clamp(B * B - A * Cc, min=0)) / A
+            acc += 2 * val * torch.sqrt(torch.clamp(B * B - A * Cc, min=0)) / A
     if oversample > 1:
         No = oversample
         ns, nt = cg.ns, cg.nt
