@@ -74,7 +74,8 @@ _lib = None
 
 
 def lib_path() -> str:
-    return _build.LIB
+    """In-tree libfdkhip.so, or the library named by $MIRT_AMD_LIB."""
+    return os.environ.get("MIRT_AMD_LIB") or _build.LIB
 
 
 def load() -> C.CDLL:

@@ -1,0 +1,359 @@
+"""Host-side logic, runnable without a GPU.
+
+* libfdkhip.so loads and exports every function include/fdkhip.h declares;
+  argument errors are reported before any device call (codes + messages).
+* fdk_mex mirrors the gateway's checks and messages (fdk,mex.c:39-90, 175-214).
+* cbct_back's host steps (fbp/cbct_back.m:74-137) with an injected back_call
+  -- the reference's own fake-backend strategy (cbct_back.m:19,38): angles,
+  permutes, ia_skip, extrapolate_t, the angular scale, use_mex 1/2/3.
+* ct_geom / image_geom conventions; fbp_ramp closed form; fdk_filter against
+  direct convolution with the ramp taps; ellipsoid_proj sphere chords and the
+  parallel-beam closed form.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+from oracle import oracle as O
+
+
+# --------------------------------------------------------------------------
+# C ABI
+
+def test_library_exports_header_symbols():
+    from mirt_amd import _lib
+    lib = _lib.load()
+    hdr = open(os.path.join(ROOT, "include", "fdkhip.h")).read()
+    declared = set(re.findall(r"\b(fdkhip_[a-z0-9_]+)\s*\(", hdr))
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared <= set(_lib.exported_symbols()) | {"fdkhip_version"}
+    assert "gfx950" in _lib.version()
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _call_ts(**over):
+    from mirt_amd import _lib
+    lib = _lib.load()
+    args = dict(nxyz=np.array([4, 4, 4], np.int32), dxyz=np.ones(3), cxyz=np.zeros(3),
+                mask=np.ones(16, np.uint8), dso=541.0, dsd=949.0, dfs=0.0,
+                dst=np.ones(2), ost=np.zeros(2), proj=np.zeros(8 * 6 * 2, np.float32),
+                ns=8, nt=6, na=2, beta=np.zeros(2), nthread=1,
+                image=np.zeros(64, np.float32))
+    args.update(over)
+    rc = lib.fdkhip_fdk_ts_back(_ptr(args["nxyz"]), _ptr(args["dxyz"]), _ptr(args["cxyz"]),
+                                _ptr(args["mask"]), args["dso"], args["dsd"], args["dfs"],
+                                _ptr(args["dst"]), _ptr(args["ost"]), _ptr(args["proj"]),
+                                args["ns"], args["nt"], args["na"], _ptr(args["beta"]),
+                                args["nthread"], _ptr(args["image"]))
+    return rc, lib.fdkhip_last_error().decode()
+
+
+def test_cabi_argument_errors_before_device():
+    from mirt_amd import _lib
+    rc, msg = _call_ts(nthread=251)
+    assert rc == _lib.FDKHIP_ETHREADS and msg == "too many threads"
+    rc, msg = _call_ts(nthread=0)
+    assert rc == _lib.FDKHIP_EARG and "nthread" in msg
+    rc, msg = _call_ts(nxyz=None)
+    assert rc == _lib.FDKHIP_EARG and "mx_nxyz" in msg
+    rc, msg = _call_ts(beta=None)
+    assert rc == _lib.FDKHIP_EARG and "beta length" in msg
+    rc, msg = _call_ts(image=None)
+    assert rc == _lib.FDKHIP_EARG and "image" in msg
+    rc, msg = _call_ts(na=-1)
+    assert rc == _lib.FDKHIP_EARG
+
+
+def test_quantize_argument_errors():
+    from mirt_amd import _lib
+    lib = _lib.load()
+    rc = lib.fdkhip_quantize_views(None, 4, 4, 2, 0, None, None, None)
+    assert rc == _lib.FDKHIP_EARG
+    assert "F16 or BF16" in lib.fdkhip_last_error().decode()
+    # empty input is a no-op
+    assert lib.fdkhip_quantize_views(None, 0, 4, 2, 1, None, None, None) == 0
+
+
+# --------------------------------------------------------------------------
+# fdk_mex gateway mirror (argument checks only: no device call happens)
+
+def _mex_args():
+    return ["fdk,ts,back", np.array([4, 4, 4], np.int32), np.ones(3), np.zeros(3),
+            np.ones((4, 4), np.uint8), 541.0, 949.0, 0.0, np.ones(2), np.zeros(2),
+            np.zeros((6, 8, 2), np.float32), np.zeros(2), np.array([1], np.int32)]
+
+
+@pytest.mark.parametrize("pos,val,msg", [
+    (0, "fdk,xx,back", "command 'fdk,xx,back' unknown"),
+    (1, np.array([4, 4, 4], np.int64), "mxIsInt32"),
+    (1, np.array([4, 4], np.int32), "mx_nxyz must have 3 elements"),
+    (2, np.ones(3, np.float32), "mxIsRealDouble"),
+    (3, np.zeros(2), "mx_cxyz must have 3 elements"),
+    (4, np.ones((4, 4), np.int8), "mxIsUint8"),
+    (4, np.ones((4, 4, 1), np.uint8), "mask must be 2d"),
+    (4, np.ones((4, 5), np.uint8), "mask size not nx by ny?"),
+    (8, np.ones(3), "mx_dst must have 2 elements"),
+    (9, np.zeros(1), "mx_ost must have 2 elements"),
+    (10, np.zeros((6, 8, 2), np.float64), "mxIsRealSingle"),
+    (10, np.zeros((6, 8, 2, 1), np.float32), "mx_proj must be 2d or 3d"),
+    (11, np.zeros(3), "beta length=3 vs na=size(proj,3)=2"),
+    (12, np.array([1], np.int64), "mxIsScalarInt32"),
+])
+def test_fdk_mex_checks(pos, val, msg):
+    from mirt_amd import FdkMexError, fdk_mex
+    args = _mex_args()
+    args[pos] = val
+    with pytest.raises(FdkMexError) as e:
+        fdk_mex(*args)
+    assert str(e.value) == msg
+
+
+def test_fdk_mex_usage():
+    from mirt_amd import FdkMexError, fdk_mex
+    args = _mex_args()
+    with pytest.raises(FdkMexError, match="usage error"):
+        fdk_mex(*args, np.zeros(1))                 # nrhs != 13
+    args[0] = 3
+    with pytest.raises(FdkMexError, match="usage error"):
+        fdk_mex(*args)
+
+
+# --------------------------------------------------------------------------
+# cbct_back host steps with an injected back_call (the oracle, on the CPU)
+
+def oracle_back_call(cmd, nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj, beta,
+                     nthread):
+    """A back_call with fdk_mex's contract, computed by the oracle (tests only)."""
+    proj = np.asarray(proj, np.float32)
+    if cmd == "fdk,ts,back":
+        nt, ns = proj.shape[0], proj.shape[1]
+    else:
+        ns, nt = proj.shape[0], proj.shape[1]
+    g = O.geom(nxyz, dxyz, cxyz, dso, dsd, dfs, ns, nt, dst, ost)
+    beta = np.atleast_1d(np.asarray(beta, np.float64))
+    pc = np.ascontiguousarray(np.asfortranarray(proj).reshape(-1, order="F"))
+    m = np.asfortranarray(mask).reshape(-1, order="F")
+    if cmd == "fdk,ts,back":
+        out = O.back_ts_f32(g, pc, beta, m)                       # C (ny, nx, nz)
+        return np.asfortranarray(out.transpose(2, 1, 0))          # MATLAB [nz nx ny]
+    out = O.back_st_f32(g, pc, beta, m)                           # C (nz, ny, nx)
+    return np.asfortranarray(out.transpose(2, 1, 0))              # MATLAB [nx ny nz]
+
+
+@pytest.fixture(scope="module")
+def small_scan():
+    from mirt_amd.geom import ct_geom, image_geom
+    cg = ct_geom("fan", ns=40, nt=16, na=12, ds=1.5, dt=1.6, offset_s=0.25, dsd=500.0,
+                 dso=300.0, dfs=0.0, orbit=360)
+    ig = image_geom(nx=14, ny=12, nz=10, dx=2.0, dz=1.8, mask="all-but-edge")
+    rng = np.random.default_rng(3)
+    proj = np.asfortranarray(rng.standard_normal(cg.dim).astype(np.float32))
+    return cg, ig, proj
+
+
+def direct_backprojection(cg, ig, proj, ia_skip=1):
+    """[nx ny nz] by the oracle directly (no cbct_back logic)."""
+    g = O.geom((ig.nx, ig.ny, ig.nz), (ig.dx, ig.dy, ig.dz),
+               (ig.offset_x, ig.offset_y, ig.offset_z), cg.dso, cg.dsd, cg.dfs,
+               cg.ns, cg.nt, (cg.ds, cg.dt), (cg.offset_s, cg.offset_t))
+    beta = np.deg2rad(cg.orbit_start + cg.orbit * np.arange(cg.na) / cg.na)[::ia_skip]
+    tsa = np.ascontiguousarray(proj.transpose(2, 0, 1)[::ia_skip])   # (na, ns, nt)
+    mask = np.ascontiguousarray(ig.mask_or.T.astype(np.uint8))
+    out = O.back_ts_f32(g, tsa, beta, mask)                              # (ny, nx, nz)
+    return out.transpose(1, 0, 2)
+
+
+def test_cbct_back_use_mex_1_2_3(small_scan):
+    from mirt_amd import cbct_back
+    from mirt_amd.cbct_back import angle_scale
+    cg, ig, proj = small_scan
+    want = direct_backprojection(cg, ig, proj)
+    img1 = cbct_back(proj, cg, ig, back_call=oracle_back_call, scale_dang=False)
+    assert img1.shape == (ig.nx, ig.ny, ig.nz)
+    assert np.array_equal(img1, want)
+    img2 = cbct_back(proj, cg, ig, use_mex=2, back_call=oracle_back_call, scale_dang=False)
+    assert np.array_equal(img2, want)   # per-view sums in view order: same roundings
+    img3 = cbct_back(proj, cg, ig, use_mex=3, back_call=oracle_back_call, scale_dang=False)
+    assert img3.shape == want.shape and np.isfinite(img3).all()
+    scaled = cbct_back(proj, cg, ig, back_call=oracle_back_call)
+    s = np.float32(0.5 * np.deg2rad(360.0) / cg.na)
+    assert angle_scale(cg) == s
+    assert np.array_equal(scaled, s * want)
+
+
+def test_cbct_back_ia_skip_and_extrapolate(small_scan):
+    from mirt_amd import cbct_back
+    cg, ig, proj = small_scan
+    want = direct_backprojection(cg, ig, proj, ia_skip=3)
+    img = cbct_back(proj, cg, ig, ia_skip=3, back_call=oracle_back_call, scale_dang=False)
+    assert np.array_equal(img, want)
+    # extrapolate_t: rows replicated outward (cbct_back.m:86-94)
+    seen = {}
+
+    def spy(cmd, *a):
+        seen["proj"] = a[9]
+        return oracle_back_call(cmd, *a)
+    cbct_back(proj, cg, ig, extrapolate_t=3, back_call=spy, scale_dang=False)
+    p = seen["proj"]                      # [nt+6 ns na]
+    assert p.shape == (cg.nt + 6, cg.ns, cg.na)
+    assert np.array_equal(p[0], proj[:, 0, :]) and np.array_equal(p[2], proj[:, 0, :])
+    assert np.array_equal(p[-1], proj[:, -1, :])
+    assert np.array_equal(p[3:-3], proj.transpose(1, 0, 2))
+
+
+def test_cbct_back_rejects_helix_and_mat(small_scan):
+    from mirt_amd import cbct_back
+    from mirt_amd.geom import ct_geom
+    cg, ig, proj = small_scan
+    helix = ct_geom("fan", ns=40, nt=16, na=12, ds=1.5, dt=1.6, dsd=500.0, dso=300.0,
+                    pitch=1.0)
+    with pytest.raises(ValueError, match="helix not yet supported"):
+        cbct_back(proj, helix, ig, back_call=oracle_back_call)
+    with pytest.raises(NotImplementedError):
+        cbct_back(proj, cg, ig, use_mex=0)
+    from mirt_amd.geom import ct_geom as cgf
+    cg2 = cgf("fan", ns=40, nt=16, na=12, ds=1.5, dt=1.6, dsd=500.0, dso=300.0,
+              offset_t=0.5)
+    with pytest.raises(ValueError, match="offset_t not yet done"):
+        cbct_back(proj, cg2, ig, extrapolate_t=2, back_call=oracle_back_call)
+
+
+# --------------------------------------------------------------------------
+# geometry conventions
+
+def test_ct_geom_conventions():
+    from mirt_amd.geom import ct_geom
+    cg = ct_geom("ge1", nt=800, na=400, down=8)         # cbct_back.m:293
+    assert (cg.ns, cg.nt, cg.na) == (112, 100, 50)       # 4*ceil(888/8/4), 2*ceil(.../2)
+    assert cg.ds == pytest.approx(1.0239 * 8) and cg.dt == pytest.approx(1.0964 * 8)
+    assert cg.ws == (112 - 1) / 2 + 1.25 and cg.dfs == 0 and cg.dsd == 949.075
+    cg = ct_geom("fan", ns=888, nt=64, na=984, offset_s=1.25, dsd=949, dod=408)
+    assert cg.dso == 541 and cg.dt == -1.0               # dt defaults to -ds
+    assert np.allclose(cg.s[:2], (np.arange(2) - cg.ws) * 1.0)
+    assert np.allclose(cg.ar, np.deg2rad(np.arange(984) / 984 * 360))
+    cg = ct_geom("fan", ns=100, nt=8, dsd=np.inf)         # parallel handled gracefully
+    assert math.isinf(cg.dso) and cg.na == 2 * math.floor(100 * math.pi / 2 / 2)
+    cg = ct_geom("fan", ns=64, nt=8, na=10, dsd=1000, dso=500, dt=("dz", 0.5))
+    assert cg.dt == pytest.approx(0.5 * 1000 / 500)
+    with pytest.raises(ValueError):
+        ct_geom("fan", ns=64, nt=8, dsd=1000)
+
+
+def test_image_geom_conventions():
+    from mirt_amd.geom import image_geom
+    ig = image_geom(nx=512, ny=496, nz=480, fov=500, mask="all-but-edge", down=8)
+    assert (ig.nx, ig.ny, ig.nz) == (64, 62, 60)
+    assert ig.dx == pytest.approx(500 / 512 * 8) and ig.dy == -ig.dx and ig.dz == ig.dx
+    mo = ig.mask_or
+    assert mo.shape == (64, 62) and not mo[0].any() and not mo[:, -1].any()
+    assert mo[1:-1, 1:-1].all()
+    ig = image_geom(nx=10, ny=8, nz=6, dx=1.0, offsets="dsp")
+    assert (ig.offset_x, ig.offset_y, ig.offset_z) == (0.5, 0.5, 0.5)
+    assert np.allclose(ig.x, (np.arange(10) - ig.wx) * 1.0)
+    m = np.zeros((8, 6, 4), bool)
+    m[2, 3, 1] = True
+    ig = image_geom(nx=8, ny=6, nz=4, dx=1.0, mask=m, down=2)
+    assert ig.mask.shape == (4, 3, 2) and ig.mask[1, 1, 0] and ig.mask.sum() == 1
+
+
+# --------------------------------------------------------------------------
+# FDK weight / filter, phantom
+
+def test_fbp_ramp_closed_form():
+    from mirt_amd.fdk import fbp_ramp
+    h = fbp_ramp("flat", 16, 2.0)
+    nn = np.arange(-8, 8)
+    assert h[8] == 1 / 4 / 4
+    odd = nn % 2 == 1
+    assert np.allclose(h[odd], -1 / (np.pi * nn[odd]) ** 2 / 4)
+    assert np.all(h[(nn % 2 == 0) & (nn != 0)] == 0)
+    ha = fbp_ramp("arc", 16, 2.0, 1e7)                     # arc -> flat as dsd -> inf
+    assert np.allclose(ha, h, rtol=1e-6)
+
+
+def test_fdk_filter_matches_direct_convolution():
+    from mirt_amd.fdk import fbp_ramp, fdk_filter
+    from mirt_amd.geom import ct_geom
+    for dfs in (0.0, np.inf):
+        cg = ct_geom("fan", ns=37, nt=3, na=2, ds=1.3, dt=1.0, dsd=800.0, dso=500.0,
+                     dfs=dfs)
+        rng = np.random.default_rng(1)
+        p = rng.standard_normal((37, 3, 2))
+        got = fdk_filter(torch.as_tensor(p), cg).numpy()
+        # direct: zero-padded circular convolution of length npad with ds*h
+        npad = 2 ** math.ceil(math.log2(2 * 37 - 1))
+        h = cg.ds * fbp_ramp("flat" if math.isinf(dfs) else "arc", npad, cg.ds,
+                             None if math.isinf(dfs) else cg.dsd)
+        hs = np.fft.ifftshift(h)          # tap for lag k at index k mod npad
+        want = np.zeros_like(p)
+        for i in range(37):
+            for j in range(37):
+                want[i] += hs[(i - j) % npad] * p[j]
+        assert np.allclose(got, want, atol=1e-10)
+
+
+def test_feldkamp_weight1_center_value():
+    from mirt_amd.fdk import feldkamp_weight1
+    from mirt_amd.geom import ct_geom
+    cg = ct_geom("fan", ns=5, nt=3, na=1, ds=1.0, dt=1.0, dsd=1000.0, dso=600.0, dfs=np.inf)
+    w = feldkamp_weight1(torch.ones((5, 3, 1), dtype=torch.float64), cg).numpy()
+    assert w[2, 1, 0] == pytest.approx(600.0 / 1000.0)      # dso/sqrt(dsd^2) at s=t=0
+    assert w[0, 0, 0] == pytest.approx(600.0 / math.sqrt(1000.0 ** 2 + 4 + 1))
+
+
+def test_ellipsoid_proj_parallel_closed_form():
+    from mirt_amd.geom import ct_geom
+    from mirt_amd.phantom import ellipsoid_proj
+    cg = ct_geom("fan", ns=41, nt=5, na=3, ds=1.0, dt=1.0, dsd=np.inf)
+    ell = np.array([[0, 0, 0, 10, 10, 10, 0, 0, 0.5]])
+    p = ellipsoid_proj(cg, ell).numpy()
+    u = cg.s[:, None]
+    v = cg.t[None, :]
+    want = 2 * 0.5 * np.sqrt(np.clip(100 - u ** 2 - v ** 2, 0, None))
+    for ia in range(3):
+        assert np.allclose(p[:, :, ia], want, atol=1e-12)
+
+
+def test_ellipsoid_proj_cone_chords():
+    """Arc and flat cone beams: sphere chord from the ray's distance to the centre."""
+    from mirt_amd.geom import ct_geom
+    from mirt_amd.phantom import ellipsoid_proj
+    for dfs in (0.0, np.inf):
+        cg = ct_geom("fan", ns=33, nt=7, na=2, ds=2.0, dt=2.0, dsd=949.0, dso=541.0,
+                     dfs=dfs, offset_s=0.25)
+        c, r, val = np.array([10.0, -5.0, 3.0]), 40.0, 0.7
+        ell = np.array([[*c, r, r, r, 0, 0, val]])
+        p = ellipsoid_proj(cg, ell).numpy()
+        for ia, b in enumerate(cg.ar):
+            src = np.array([-541.0 * math.sin(b), 541.0 * math.cos(b), 0.0])
+            for i in range(cg.ns):
+                for j in range(cg.nt):
+                    s, t = cg.s[i], cg.t[j]
+                    if dfs == 0:
+                        gam = s / 949.0
+                        d_loc = np.array([949.0 * math.sin(gam), -949.0 * math.cos(gam), t])
+                    else:
+                        d_loc = np.array([s, -949.0, t])
+                    # rotate the beta = 0 frame (source on +y) by beta
+                    rot = np.array([[math.cos(b), -math.sin(b), 0],
+                                    [math.sin(b), math.cos(b), 0], [0, 0, 1]])
+                    det = src + rot @ d_loc
+                    d = det - src
+                    d /= np.linalg.norm(d)
+                    w = c - src
+                    dist2 = w @ w - (w @ d) ** 2
+                    chord = 2 * val * math.sqrt(max(r * r - dist2, 0))
+                    assert p[i, j, ia] == pytest.approx(chord, abs=1e-9)
