@@ -1,17 +1,24 @@
 #!/usr/bin/env python
 """Benchmark: FDK cone-beam backprojection (cbct_back 'fdk,ts,back') on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): arc detector
-888 x 64 at ds = 1.0239 mm, dt = 1.0964 mm, quarter-detector offset 1.25,
-dso = 541 mm, dsd = 949 mm, 984 views over 360 deg; image 512 x 512 x 256 at
-0.7 mm in-plane (dy = -dx) and 0.625 mm in z; 30 seeded ellipsoids, analytic
-line integrals, Poisson noise at I0 = 2e5, then the FDK cosine weight and ramp
-filter.  The filtered views are stored on the device in FP16 with a per-view
-scale (also FP32 and BF16 for comparison).
+Default workload (the one the headline metric is quoted on), ``clinical_arc``:
+arc detector 888 x 64 at ds = 1.0239 mm, dt = 1.0964 mm, quarter-detector
+offset 1.25, dso = 541 mm, dsd = 949 mm, 984 views over 360 deg; image
+512 x 512 x 256 at 0.7 mm in-plane (dy = -dx) and 0.625 mm in z; 30 seeded
+ellipsoids, analytic line integrals, Poisson noise at I0 = 2e5, then the FDK
+cosine weight and ramp filter.  The filtered views are stored on the device
+in FP16 with a per-view scale (FP32 and BF16 are timed beside it).
 
-One step = one backprojection of all 984 views into the whole volume (the
-plan's graph: a zero fill of the slices no view reaches plus the tiled
-kernel), projections and image resident in HBM.  value = nx*ny*nz*na / t.
+One step = one backprojection of all views into the whole volume (the plan's
+graph: a zero fill of the slices no view reaches, the zero-padded copy of the
+views and the tiled kernel), projections and image resident in HBM.
+value = nx*ny*nz*na / t.  The roofline entry is for the backprojection kernel
+alone, timed with HIP events on its own stream (fdkhip_plan_profile).
+
+``--workload uneven_scans`` times 32 seeded scans of uneven size in one
+persistent launch (FdkBatch, autotuned, replayed from a HIP graph) against
+the same scans run one plan after another; other workloads are listed in
+mirt_amd/workloads.py.
 
 Prints ONE JSON line.  The CPU baseline is the reference C (oracle/_ref,
 fdk-helix axial path, pthreads) timed on this host's cores on a stated sample
@@ -22,8 +29,8 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
+import re
 import sys
 import time
 
@@ -32,66 +39,85 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (packed), AMD spec
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak, AMD spec
 FLOP_PER_UPDATE = 14         # wu, wd, 2x(2 mul + add), vertical lerp (3), x w2, +=, t_bin +=
+HEADLINE_METRIC = ("voxel-view updates/s (GUPS), 512²×256 ×984 views; "
+                   "fp16-stored vs fp32 at rel-L2")
+
+# storage/math path timed as the headline of each workload
+HEADLINE = {"clinical_arc": "f16", "large_flat": "f16", "plain_flat": "f32_exact",
+            "cpu_small": "f32_exact", "uneven_scans": "f16"}
+PATHS = {"f32_exact": ("f32", "exact"), "f32_fast": ("f32", "fast"),
+         "f16": ("f16", "fast"), "bf16": ("bf16", "fast")}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def workload(device):
-    import torch
-    from mirt_amd.geom import ct_geom, image_geom
-    from mirt_amd.phantom import ellipsoid_proj, random_ellipsoids, transmission_noise
-    from mirt_amd.fdk import feldkamp_weight1, fdk_filter
-    cg = ct_geom("fan", ns=888, nt=64, na=984, ds=1.0239, dt=1.0964, offset_s=1.25,
-                 dsd=949.0, dso=541.0, dfs=0.0, orbit=360)
-    ig = image_geom(nx=512, ny=512, nz=256, dx=0.7, dz=0.625)
-    ell = random_ellipsoids(2000, 30, ig.nx * ig.dx, ig.nz * ig.dz)
-    t0 = time.perf_counter()
-    p = ellipsoid_proj(cg, ell, device=device)                 # [ns nt na] float64
-    p = transmission_noise(p, 2e5, seed=2000)
-    p = fdk_filter(feldkamp_weight1(p, cg), cg)
-    tsa = p.permute(2, 0, 1).contiguous().to(torch.float32)    # [nt ns na]
-    torch.cuda.synchronize()
-    log(f"inputs ready in {time.perf_counter() - t0:.1f} s")
-    return cg, ig, tsa
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
-def time_plan(plan, args_, out, steps, warmup):
+def oracle_geom(cg, ig):
+    from oracle import oracle as O
+    return O.geom((ig.nx, ig.ny, ig.nz), (ig.dx, ig.dy, ig.dz),
+                  (ig.offset_x, ig.offset_y, ig.offset_z), cg.dso, cg.dsd, cg.dfs,
+                  cg.ns, cg.nt, (cg.ds, cg.dt), (cg.offset_s, cg.offset_t))
+
+
+def profiled_traffic(workload, fmt, cfg):
+    """HBM bytes per launch of the backprojection kernel from the committed
+    rocprofv3 summary for this workload and path (FETCH_SIZE and WRITE_SIZE
+    passes, scaled by the copy-kernel calibration), if the summary's launch
+    shape matches the one timed here; else None."""
+    path = os.path.join(ROOT, "profiles", f"back_kernel_{workload}_{fmt}.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    shape = re.search(r"^launch:\s*waves=(\d+) zper=(\d+) vstage=(\d+)", text, re.M)
+    if not m or not shape:
+        return None
+    if tuple(int(x) for x in shape.groups()) != (cfg["waves"], cfg["zper"], cfg["vstage"]):
+        return None
+    return float(m.group(1))
+
+
+def prepare(tsa, fmt):
+    from mirt_amd import quantize_views
+    if fmt == "f32":
+        return (tsa,)
+    return quantize_views(tsa, fmt)
+
+
+def time_steps(fn, steps, warmup):
     import torch
     s = torch.cuda.current_stream()
     for _ in range(warmup):
-        plan(*args_, out=out)
+        fn()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(s)
     for _ in range(steps):
-        plan(*args_, out=out)
+        fn()
     e1.record(s)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps      # ms per step
 
 
-def cpu_baseline(cg, ig, tsa_host, beta, images, target_s=15.0):
-    """Reference C on this host (bounded sample) + errors vs the FP64 oracle."""
+def cpu_reference_rate(g, tsa_host, beta, mask, nx_ny_nz, target_s=15.0):
+    """Reference C on this host on an evenly spaced sample of views."""
     from oracle import oracle as O
     if not O.have_reference("fixed"):
-        return None, None
-    g = O.geom((ig.nx, ig.ny, ig.nz), (ig.dx, ig.dy, ig.dz),
-               (ig.offset_x, ig.offset_y, ig.offset_z), cg.dso, cg.dsd, cg.dfs,
-               cg.ns, cg.nt, (cg.ds, cg.dt), (cg.offset_s, cg.offset_t))
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    nthread = max(1, min(16, cores))
-    mask = np.ones((ig.ny, ig.nx), np.uint8)
+        return None
+    nthread = max(1, min(16, host_cores()))
     na = tsa_host.shape[0]
-    # rate estimate on 4 views, then a sample of evenly spaced views
-    probe = np.arange(0, na, na // 4)[:4]
+    probe = np.arange(0, na, max(1, na // 4))[:4]
     O.ref_back_ts("fixed", g, tsa_host[probe], beta[probe], mask, nthread=nthread)  # warm-up
     _, tp = O.ref_back_ts("fixed", g, tsa_host[probe], beta[probe], mask,
                           nthread=nthread, timed=True)
@@ -107,29 +133,223 @@ def cpu_baseline(cg, ig, tsa_host, beta, images, target_s=15.0):
         if sum(times) > 2.5 * target_s:
             break
     t = float(np.median(times))
-    U = ig.nx * ig.ny * ig.nz * len(views)
-    base = {"value": U / t, "unit": "voxel-view updates/s", "cores": nthread,
-            "kind": "reference",
+    nx, ny, nz = nx_ny_nz
+    return {"value": nx * ny * nz * len(views) / t, "unit": "voxel-view updates/s",
+            "cores": nthread, "kind": "reference",
             "sample": f"reference C (fdk-helix fdk_ts_back_t, gcc -O3, {nthread} pthreads) "
                       f"on {len(views)} of {na} views (every {step}th), full "
-                      f"{ig.nx}x{ig.ny}x{ig.nz} volume; median of {len(times)} runs "
-                      f"of {t:.2f} s",
+                      f"{nx}x{ny}x{nz} volume; median of {len(times)} runs of {t:.2f} s",
             "seconds": t}
-    # errors on a seeded sample of columns, every slice and view
-    rng = np.random.default_rng(7)
-    cols = np.sort(rng.choice(ig.nx * ig.ny, 1024, replace=False)).astype(np.int32)
-    t0 = time.perf_counter()
+
+
+def column_errors(g, tsa_host, beta, mask, images, ncol=1024, seed=7):
+    """Relative L2 of each GPU image against the FP64 oracle and the FP32
+    restatement of the reference kernel, on seeded columns."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    ncol = min(ncol, g.nx * g.ny)
+    cols = np.sort(rng.choice(g.nx * g.ny, ncol, replace=False)).astype(np.int32)
     r64 = O.back_ts_f64(g, tsa_host, beta, mask, cols=cols)
     r32 = O.back_ts_f32(g, tsa_host, beta, mask, cols=cols)
     errs = {"columns_sampled": int(cols.size)}
     for name, img in images.items():
-        sub = img.reshape(-1, ig.nz)[cols]
+        sub = img.reshape(-1, g.nz)[cols]
         errs[f"relL2_{name}_vs_fp64"] = O.rel_l2(sub, r64)
         errs[f"relL2_{name}_vs_reference_c"] = O.rel_l2(sub, r32)
         errs[f"maxabs_{name}_vs_reference_c"] = float(np.abs(sub - r32).max())
     errs["relL2_reference_c_vs_fp64"] = O.rel_l2(r32, r64)
-    log(f"oracle errors in {time.perf_counter() - t0:.1f} s")
-    return base, errs
+    return errs
+
+
+def run_single(args, dev):
+    import torch
+    from mirt_amd import FdkPlan, geom_of
+    from mirt_amd.cbct_back import _views
+    from mirt_amd.workloads import WORKLOADS, filtered_views
+
+    scan = WORKLOADS[args.workload]()
+    cg, ig = scan.cg, scan.ig
+    t0 = time.perf_counter()
+    tsa = filtered_views(scan, dev)
+    torch.cuda.synchronize()
+    log(f"inputs ready in {time.perf_counter() - t0:.1f} s")
+    beta = _views(cg, cg.na)
+    geom = geom_of(cg, ig)
+    mask = ig.mask_or
+    head = args.path or HEADLINE[args.workload]
+    order = [head] + ([] if args.fmt_only else [p for p in PATHS if p != head])
+
+    U = ig.nx * ig.ny * ig.nz * cg.na
+    paths, images, kernel = {}, {}, {}
+    U_eff = cfg = None
+    for name in order:
+        fmt, mm = PATHS[name]
+        plan = FdkPlan(geom, beta, mask, fmt=fmt, math_mode=mm)
+        a = prepare(tsa, fmt)
+        out = torch.empty((ig.ny, ig.nx, ig.nz), dtype=torch.float32, device=dev)
+        paths[name] = time_steps(lambda: plan(*a, out=out), args.steps, args.warmup)
+        kernel[name] = plan.profile(*a, out=out, reps=max(2, args.steps))
+        images[name] = out
+        if name == head:
+            U_eff = plan.count_updates()
+            cfg = plan.config
+        log(f"{name}: {paths[name]:.3f} ms/step, kernel {kernel[name]['back']:.3f} ms, "
+            f"{U / paths[name] * 1e3:.3e} updates/s, config {plan.config}")
+        del plan, a
+
+    ms = paths[head]
+    kms = kernel[head]["back"]
+    achieved = FLOP_PER_UPDATE * U_eff / (kms * 1e-3) / 1e12
+    traffic = profiled_traffic(args.workload, head, cfg)
+    det = "arc" if cg.dfs == 0 else "flat"
+    result = {
+        "metric": HEADLINE_METRIC if args.workload == "clinical_arc"
+        else f"voxel-view updates/s (GUPS), {args.workload}",
+        "value": U / (ms * 1e-3),
+        "unit": "voxel-view updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: {len(scan.ellipsoids)} seeded ellipsoids, analytic {det} "
+                f"cone-beam line integrals"
+                + (f", Poisson I0={scan.i0:g}" if scan.i0 else ", noise-free")
+                + f", FDK weight + ramp filter, {head} storage"
+                + (" with per-view scale" if head in ("f16", "bf16") else ""),
+        "config": {"workload": f"{args.workload}: cbct_back fdk,ts,back, {det} "
+                               f"{cg.ns}x{cg.nt} det, {cg.na} views/360 deg, dso "
+                               f"{cg.dso:g} dsd {cg.dsd:g} -> {ig.nx}x{ig.ny}x{ig.nz} "
+                               f"({ig.dx:g}/{ig.dz:g} mm)",
+                   "nx": ig.nx, "ny": ig.ny, "nz": ig.nz, "na": cg.na, "ns": cg.ns,
+                   "nt": cg.nt, "storage": head, "accumulate": "f32", "launch": cfg},
+        "gups_nominal": U / (ms * 1e-3) / 1e9,
+        "gups_effective": U_eff / (ms * 1e-3) / 1e9,
+        "updates_nominal": U,
+        "updates_effective": U_eff,
+        "paths_ms_per_step": paths,
+        "kernel_ms": kernel,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic,
+                     "note": f"{FLOP_PER_UPDATE} FP32 flop x {U_eff} in-support updates per "
+                             f"launch / {kms:.3f} ms back_kernel (HIP events on its "
+                             "stream); traffic = HBM bytes per launch from the "
+                             "rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/"},
+    }
+    ref = images.get("f32_exact")
+    if ref is not None:
+        for name in images:
+            if name != "f32_exact":
+                d = (images[name] - ref).double()
+                result[f"relL2_{name}_vs_f32_exact_gpu"] = float(
+                    d.norm() / ref.double().norm())
+    result["cpu_baseline"] = None
+    if not args.no_cpu_baseline:
+        g = oracle_geom(cg, ig)
+        tsa_host = tsa.cpu().numpy()
+        m = np.ascontiguousarray(np.asarray(mask).T).astype(np.uint8)
+        t0 = time.perf_counter()
+        result["cpu_baseline"] = cpu_reference_rate(g, tsa_host, beta, m,
+                                                    (ig.nx, ig.ny, ig.nz))
+        imgs = {k: v.cpu().numpy() for k, v in images.items()}
+        result["errors"] = column_errors(g, tsa_host, beta, m, imgs)
+        log(f"cpu baseline + oracle errors in {time.perf_counter() - t0:.1f} s")
+    return result
+
+
+def run_batch(args, dev):
+    import torch
+    from mirt_amd import FdkBatch, FdkPlan, geom_of, quantize_views
+    from mirt_amd.cbct_back import _views
+    from mirt_amd.workloads import filtered_views, uneven_scans
+
+    head = args.path or HEADLINE["uneven_scans"]
+    fmt, mm = PATHS[head]
+    scans = uneven_scans(args.nscan)
+    t0 = time.perf_counter()
+    projs, scales, hosts = [], [], []
+    for sc in scans:
+        tsa = filtered_views(sc, dev)
+        if args.errors_scans > len(hosts):
+            hosts.append(tsa.cpu().numpy())
+        a = prepare(tsa, fmt)
+        projs.append(a[0])
+        scales.append(a[1] if len(a) > 1 else None)
+        del tsa
+    torch.cuda.synchronize()
+    log(f"{len(scans)} scans ready in {time.perf_counter() - t0:.1f} s")
+    scales = None if fmt == "f32" else scales
+    betas = [_views(sc.cg, sc.cg.na) for sc in scans]
+    geoms = [geom_of(sc.cg, sc.ig) for sc in scans]
+    masks = [sc.ig.mask_or for sc in scans]
+    batch = FdkBatch(list(zip(geoms, betas, masks)), fmt=fmt, math_mode=mm)
+    outs = [torch.empty((sc.ig.ny, sc.ig.nx, sc.ig.nz), dtype=torch.float32, device=dev)
+            for sc in scans]
+    t0 = time.perf_counter()
+    tuned = batch.autotune(projs, scales, outs, reps=2)
+    log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned:.3f} ms, config {batch.config}")
+    ms = time_steps(lambda: batch(projs, scales, outs), args.steps, args.warmup)
+    prof = batch.profile(projs, scales, outs, reps=max(2, args.steps))
+    U = sum(sc.ig.nx * sc.ig.ny * sc.ig.nz * sc.cg.na for sc in scans)
+    U_eff = batch.count_updates()
+    log(f"batch: {ms:.3f} ms/step, kernel {prof['back']:.3f} ms, {U / ms * 1e3:.3e} upd/s")
+
+    # the same scans one plan after another (each replayed from its graph)
+    plans = []
+    for i in range(len(scans)):
+        plans.append(FdkPlan(geoms[i], betas[i], masks[i], fmt=fmt, math_mode=mm))
+
+    def sequential():
+        for i, p in enumerate(plans):
+            p(projs[i], scales[i] if scales else None, out=outs[i])
+    ms_seq = time_steps(sequential, args.steps, args.warmup)
+    log(f"sequential plans: {ms_seq:.3f} ms/step")
+    del plans
+    batch(projs, scales, outs)
+    torch.cuda.synchronize()
+
+    kms = prof["back"]
+    achieved = FLOP_PER_UPDATE * U_eff / (kms * 1e-3) / 1e12
+    result = {
+        "metric": "voxel-view updates/s (GUPS), uneven_scans batch",
+        "value": U / (ms * 1e-3),
+        "unit": "voxel-view updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: {len(scans)} seeded uneven scans, 10-30 ellipsoids each, "
+                f"analytic flat cone-beam line integrals, FDK weight + ramp filter, {head} "
+                "storage",
+        "config": {"workload": f"uneven_scans: {len(scans)} scans in one persistent launch "
+                               "(edge {128,192,256,384}, nz=edge/2, 180-720 views, flat "
+                               "256-768 x 128-512 det)",
+                   "storage": head, "accumulate": "f32", "launch": batch.config},
+        "updates_nominal": U, "updates_effective": U_eff,
+        "sequential_plans_ms_per_step": ms_seq,
+        "kernel_ms": prof,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": None,
+                     "note": f"{FLOP_PER_UPDATE} FP32 flop x {U_eff} in-support updates / "
+                             f"{kms:.3f} ms batch_kernel (HIP events)"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline and hosts:
+        errs = {}
+        for i, h in enumerate(hosts):
+            sc = scans[i]
+            g = oracle_geom(sc.cg, sc.ig)
+            m = np.ascontiguousarray(np.asarray(masks[i]).T).astype(np.uint8)
+            e = column_errors(g, h, betas[i], m, {head: outs[i].cpu().numpy()}, ncol=256)
+            errs[sc.name] = {k: v for k, v in e.items() if "fp64" in k}
+            if i == 0:
+                result["cpu_baseline"] = cpu_reference_rate(
+                    g, h, betas[i], m, (sc.ig.nx, sc.ig.ny, sc.ig.nz), target_s=10.0)
+        result["errors"] = errs
+    return result
 
 
 def main():
@@ -137,9 +357,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="clinical_arc",
+                    choices=["clinical_arc", "plain_flat", "large_flat", "cpu_small",
+                             "uneven_scans"])
+    ap.add_argument("--path", choices=list(PATHS), default=None,
+                    help="storage/math path of the headline (default per workload)")
+    ap.add_argument("--nscan", type=int, default=32, help="uneven_scans: number of scans")
+    ap.add_argument("--errors-scans", type=int, default=4,
+                    help="uneven_scans: scans checked against the FP64 oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fmt-only", action="store_true",
-                    help="time only the headline FP16 path (profiling runs)")
+                    help="time only the headline path (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -148,91 +376,11 @@ def main():
         sys.exit(2)
 
     import torch
-    from mirt_amd import FdkPlan, geom_of, quantize_views
-    from mirt_amd.cbct_back import _views
-
     dev = torch.device("cuda:0")
-    cg, ig, tsa = workload(dev)
-    beta = _views(cg, cg.na)
-    geom = geom_of(cg, ig)
-    mask = ig.mask_or
-    q16, s16 = quantize_views(tsa, "f16")
-    torch.cuda.synchronize()
-
-    t0 = time.perf_counter()
-    plan16 = FdkPlan(geom, beta, mask, fmt="f16")
-    torch.cuda.synchronize()
-    log(f"plan (survey) {time.perf_counter() - t0:.2f} s, config {plan16.config}")
-    U = ig.nx * ig.ny * ig.nz * cg.na
-    U_eff = plan16.count_updates()
-    out16 = torch.empty((ig.ny, ig.nx, ig.nz), dtype=torch.float32, device=dev)
-    ms16 = time_plan(plan16, (q16, s16), out16, args.steps, args.warmup)
-    log(f"f16: {ms16:.3f} ms/step  {U / ms16 * 1e3:.3e} updates/s")
-
-    paths = {"f16": ms16}
-    images = {"f16": out16}
-    if not args.fmt_only:
-        for name, fmt, mm in (("f32_exact", "f32", "exact"), ("f32_fast", "f32", "fast"),
-                              ("bf16", "bf16", "fast")):
-            plan = FdkPlan(geom, beta, mask, fmt=fmt, math_mode=mm)
-            out = torch.empty_like(out16)
-            if fmt == "f32":
-                a = (tsa,)
-            else:
-                qb, sb = quantize_views(tsa, fmt)
-                a = (qb, sb)
-            paths[name] = time_plan(plan, a, out, args.steps, args.warmup)
-            images[name] = out
-            log(f"{name}: {paths[name]:.3f} ms/step")
-            del plan
-
-    achieved = FLOP_PER_UPDATE * U_eff / (ms16 * 1e-3) / 1e12
-    result = {
-        "metric": "voxel-view updates/s (GUPS), 512²×256 ×984 views; fp16-stored vs fp32 at rel-L2",
-        "value": U / (ms16 * 1e-3),
-        "unit": "voxel-view updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms16,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: 30 seeded ellipsoids, analytic arc cone-beam line integrals, "
-                "Poisson I0=2e5, FDK weight + ramp filter, FP16 per-view-scaled storage",
-        "config": {"workload": "cbct_back fdk,ts,back: arc 888x64 det, 984 views/360 deg, "
-                               "dso 541 dsd 949 -> 512x512x256 (0.7/0.625 mm)",
-                   "nx": ig.nx, "ny": ig.ny, "nz": ig.nz, "na": cg.na, "ns": cg.ns,
-                   "nt": cg.nt, "storage": "f16+per-view scale", "accumulate": "f32",
-                   "launch": plan16.config},
-        "gups_nominal": U / (ms16 * 1e-3) / 1e9,
-        "gups_effective": U_eff / (ms16 * 1e-3) / 1e9,
-        "updates_nominal": U,
-        "updates_effective": U_eff,
-        "paths_ms_per_step": paths,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None,
-                     "note": f"{FLOP_PER_UPDATE} FP32 flop x {U_eff} in-support updates per "
-                             "launch / event-timed step"},
-    }
-    if images.get("f32_exact") is not None:
-        ref = images["f32_exact"]
-        for name in ("f16", "bf16", "f32_fast"):
-            if name in images:
-                d = (images[name] - ref).double()
-                result[f"relL2_{name}_vs_f32_exact_gpu"] = float(
-                    d.norm() / ref.double().norm())
-    if not args.no_cpu_baseline:
-        tsa_host = tsa.cpu().numpy()
-        imgs = {k: v.cpu().numpy() for k, v in images.items()}
-        base, errs = cpu_baseline(cg, ig, tsa_host, beta, imgs)
-        result["cpu_baseline"] = base
-        if errs:
-            result["errors"] = errs
+    if args.workload == "uneven_scans":
+        result = run_batch(args, dev)
     else:
-        result["cpu_baseline"] = None
+        result = run_single(args, dev)
     print(json.dumps(result), flush=True)
 
 

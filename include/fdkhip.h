@@ -159,6 +159,79 @@ int fdkhip_plan_destroy(fdkhip_plan *plan);
  * computed on the device.  Synchronous. */
 int fdkhip_plan_count_updates(fdkhip_plan *plan, int64_t *n_updates);
 
+/* Time every launch shape the build supports with each views-per-stage depth
+ * that fits LDS (`reps` launches each, on `stream`, into image_dev), keep the
+ * fastest for this plan and remember it for later plans of the same
+ * geometry, view count, format and math mode in this process.  The image is
+ * overwritten.  *best_ms (may be NULL) receives the best average time.
+ * Synchronous.  No counterpart in the reference, whose thread count is the
+ * caller's nthread. */
+int fdkhip_plan_autotune(fdkhip_plan *plan, const void *proj_dev,
+	const float *scale_dev, float *image_dev, void *stream, int32_t reps,
+	float *best_ms);
+
+/* Average device time in milliseconds of the plan's three launch steps over
+ * `reps` direct executions, from HIP events recorded on `stream` between
+ * them: ms[0] zero fill of slices no view reaches, ms[1] zero-padded copy of
+ * the views, ms[2] the backprojection kernel.  Synchronous.  No counterpart
+ * in the reference (its timing is jf,time.c wall clock around the mex). */
+int fdkhip_plan_profile(fdkhip_plan *plan, const void *proj_dev,
+	const float *scale_dev, float *image_dev, void *stream, int32_t reps,
+	float ms[3]);
+
+/*
+ * Batches of scans: many 'fdk,ts,back' calls -- different geometries, view
+ * counts and masks, one storage format -- backprojected by one persistent
+ * launch whose workgroups take (scan, column tile, z group) items from a
+ * device work queue, so scans of uneven size keep every CU busy.  Each scan
+ * i is what fdkhip_plan_execute would compute for geoms[i], na[i], beta[i]
+ * and mask[i] (mask may be NULL, or mask[i] NULL for all ones), with the same
+ * result bits for the same math mode.  The reference backprojects one scan
+ * per mex call (fbp/cbct_back.m:99-103) and has no batch object.
+ */
+typedef struct fdkhip_batch fdkhip_batch;
+
+int fdkhip_batch_create(fdkhip_batch **batch, int32_t nscan,
+	const fdkhip_geom *geoms, const int32_t *na, const double *const *beta,
+	const uint8_t *const *mask, int32_t fmt, int32_t math);
+
+/* proj[i], scale[i] (F16/BF16 only; scale may be NULL for F32) and image[i]
+ * are scan i's device buffers as in fdkhip_plan_execute.  Asynchronous on
+ * `stream`; replayed from a HIP graph while the pointers stay the same. */
+int fdkhip_batch_execute(fdkhip_batch *batch, const void *const *proj,
+	const float *const *scale, float *const *image, void *stream);
+
+/* Same as fdkhip_batch_execute but always launches directly (no graph). */
+int fdkhip_batch_execute_direct(fdkhip_batch *batch, const void *const *proj,
+	const float *const *scale, float *const *image, void *stream);
+
+/* cfg[0] = waves per workgroup, cfg[1] = slices per lane, cfg[2] = views per
+ * LDS stage, cfg[3] = floats per view slot, cfg[4] = persistent workgroups,
+ * cfg[5] = work items. */
+int fdkhip_batch_config(const fdkhip_batch *batch, int32_t cfg[6]);
+
+/* Launch shape for the whole batch (a pair the build supports) and an upper
+ * bound on views per LDS stage (0 = largest that fits). */
+int fdkhip_batch_set_config(fdkhip_batch *batch, int32_t waves, int32_t zper,
+	int32_t vstage);
+
+/* As fdkhip_plan_autotune, over the batch's launch shapes and stage depths. */
+int fdkhip_batch_autotune(fdkhip_batch *batch, const void *const *proj,
+	const float *const *scale, float *const *image, void *stream, int32_t reps,
+	float *best_ms);
+
+/* Average device milliseconds over `reps` direct executions, from HIP events
+ * on `stream`: ms[0] the per-scan zero fills and padded copies, ms[1] the
+ * persistent backprojection kernel (with its queue reset).  Synchronous. */
+int fdkhip_batch_profile(fdkhip_batch *batch, const void *const *proj,
+	const float *const *scale, float *const *image, void *stream, int32_t reps,
+	float ms[2]);
+
+/* Sum of fdkhip_plan_count_updates over the scans. */
+int fdkhip_batch_count_updates(fdkhip_batch *batch, int64_t *n_updates);
+
+int fdkhip_batch_destroy(fdkhip_batch *batch);
+
 /*
  * Per-view quantisation of FP32 projections on the device, the storage step
  * for the F16/BF16 paths: scale[ia] = 2^e with e the smallest integer such

@@ -68,6 +68,19 @@ _SIGS = {
     "fdkhip_plan_count_updates": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "fdkhip_quantize_views": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                         _P, _P, _P]),
+    "fdkhip_plan_profile": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32, _P]),
+    "fdkhip_plan_autotune": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32, C.POINTER(C.c_float)]),
+    "fdkhip_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(Geom), _P, _P, _P,
+                                      C.c_int32, C.c_int32]),
+    "fdkhip_batch_execute": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fdkhip_batch_execute_direct": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fdkhip_batch_config": (C.c_int, [_P, _P]),
+    "fdkhip_batch_set_config": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    "fdkhip_batch_autotune": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32,
+                                        C.POINTER(C.c_float)]),
+    "fdkhip_batch_profile": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32, _P]),
+    "fdkhip_batch_count_updates": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "fdkhip_batch_destroy": (C.c_int, [_P]),
 }
 
 _lib = None

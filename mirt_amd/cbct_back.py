@@ -332,10 +332,168 @@ class FdkPlan:
         _lib.check(rc)
         return out
 
+    def autotune(self, proj, scale=None, out=None, *, reps=3, stream=None):
+        """Time the supported launch shapes and stage depths on these buffers
+        and keep the fastest (fdkhip_plan_autotune); returns the best average
+        milliseconds.  ``out`` (or a scratch image) is overwritten."""
+        import torch
+        self._check_proj(proj, scale)
+        g = self.geom
+        if out is None:
+            out = torch.empty((g.ny, g.nx, g.nz), dtype=torch.float32, device=proj.device)
+        s = stream if stream is not None else torch.cuda.current_stream(proj.device)
+        best = C.c_float()
+        _lib.check(self.lib.fdkhip_plan_autotune(
+            self.handle, C.c_void_p(proj.data_ptr()),
+            C.c_void_p(scale.data_ptr()) if scale is not None else None,
+            C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream), int(reps),
+            C.byref(best)))
+        return float(best.value)
+
+    def profile(self, proj, scale=None, out=None, *, reps=5, stream=None):
+        """Average device milliseconds of the launch steps over ``reps`` direct
+        executions (fdkhip_plan_profile): ``{"fill", "pad", "back"}``."""
+        import torch
+        self._check_proj(proj, scale)
+        g = self.geom
+        if out is None:
+            out = torch.empty((g.ny, g.nx, g.nz), dtype=torch.float32, device=proj.device)
+        s = stream if stream is not None else torch.cuda.current_stream(proj.device)
+        ms = (C.c_float * 3)()
+        _lib.check(self.lib.fdkhip_plan_profile(
+            self.handle, C.c_void_p(proj.data_ptr()),
+            C.c_void_p(scale.data_ptr()) if scale is not None else None,
+            C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream), int(reps), ms))
+        return {"fill": ms[0], "pad": ms[1], "back": ms[2]}
+
     @staticmethod
     def image_xyz(img):
         """[nz nx ny] (tensor (ny, nx, nz)) viewed as [nx ny nz]."""
         return img.permute(1, 0, 2)
+
+
+class FdkBatch:
+    """Many scans backprojected by one persistent launch (libfdkhip batch).
+
+    ``scans`` is a sequence of ``(geom, beta, mask)`` (mask may be None).
+    :meth:`__call__` takes per-scan device projections (and scales for
+    F16/BF16) laid out as for :class:`FdkPlan` and returns per-scan
+    ``(ny, nx, nz)`` float32 images equal to what one :class:`FdkPlan` per
+    scan gives.
+    """
+
+    def __init__(self, scans, *, fmt="f32", math_mode="exact"):
+        self.lib = _lib.load()
+        self.fmt = _lib.FMT_NAMES[fmt] if isinstance(fmt, str) else int(fmt)
+        self.math = {"exact": _lib.MATH_EXACT, "fast": _lib.MATH_FAST}[math_mode] \
+            if isinstance(math_mode, str) else int(math_mode)
+        scans = list(scans)
+        n = len(scans)
+        if n == 0:
+            raise ValueError("a batch needs at least one scan")
+        self.geoms = [sc[0] for sc in scans]
+        garr = (Geom * n)(*self.geoms)
+        self._betas = [np.ascontiguousarray(np.atleast_1d(sc[1]), dtype=np.float64)
+                       for sc in scans]
+        self.na = [int(b.size) for b in self._betas]
+        self._masks = []
+        for (g, _, m) in scans:
+            if m is None:
+                self._masks.append(None)
+                continue
+            m = np.asarray(m)
+            if m.shape != (g.nx, g.ny):
+                raise ValueError("mask must be [nx ny]")
+            self._masks.append(np.asfortranarray(m != 0).astype(np.uint8, order="F"))
+        na_arr = (C.c_int32 * n)(*self.na)
+        beta_arr = (C.c_void_p * n)(*[b.ctypes.data for b in self._betas])
+        mask_arr = (C.c_void_p * n)(*[None if m is None else m.ctypes.data
+                                      for m in self._masks])
+        h = C.c_void_p()
+        _lib.check(self.lib.fdkhip_batch_create(C.byref(h), n, garr, na_arr, beta_arr,
+                                                mask_arr, self.fmt, self.math))
+        self.handle = h
+        self.nscan = n
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.fdkhip_batch_destroy(h)
+            self.handle = None
+
+    @property
+    def config(self):
+        cfg = (C.c_int32 * 6)()
+        _lib.check(self.lib.fdkhip_batch_config(self.handle, cfg))
+        keys = ("waves", "zper", "vstage", "slot", "workgroups", "items")
+        return dict(zip(keys, list(cfg)))
+
+    def set_config(self, waves, zper, vstage=0):
+        _lib.check(self.lib.fdkhip_batch_set_config(self.handle, waves, zper, vstage))
+
+    def count_updates(self) -> int:
+        n = C.c_int64()
+        _lib.check(self.lib.fdkhip_batch_count_updates(self.handle, C.byref(n)))
+        return int(n.value)
+
+    def _arrays(self, projs, scales, outs):
+        import torch
+        if len(projs) != self.nscan:
+            raise ValueError(f"expected {self.nscan} projection tensors")
+        want = {_lib.F32: torch.float32, _lib.F16: torch.float16,
+                _lib.BF16: torch.bfloat16}[self.fmt]
+        if outs is None:
+            outs = [torch.empty((g.ny, g.nx, g.nz), dtype=torch.float32, device=p.device)
+                    for g, p in zip(self.geoms, projs)]
+        for i, (g, p, o) in enumerate(zip(self.geoms, projs, outs)):
+            if p.dtype != want or not p.is_cuda or not p.is_contiguous() \
+                    or p.numel() != self.na[i] * g.ns * g.nt:
+                raise ValueError(f"scan {i}: proj must be a contiguous {want} [nt ns na]")
+            if o.dtype != torch.float32 or not o.is_contiguous() \
+                    or o.numel() != g.nx * g.ny * g.nz:
+                raise ValueError(f"scan {i}: out must be a contiguous float32 [nz nx ny]")
+        n = self.nscan
+        pa = (C.c_void_p * n)(*[p.data_ptr() for p in projs])
+        oa = (C.c_void_p * n)(*[o.data_ptr() for o in outs])
+        sa = None
+        if self.fmt != _lib.F32:
+            if scales is None or len(scales) != n:
+                raise ValueError("F16/BF16 batches need one (na,) float32 scale per scan")
+            for i, sc in enumerate(scales):
+                if sc.dtype != torch.float32 or not sc.is_cuda or sc.numel() != self.na[i]:
+                    raise ValueError(f"scan {i}: scale must be a (na,) float32 device tensor")
+            sa = (C.c_void_p * n)(*[sc.data_ptr() for sc in scales])
+        return outs, pa, sa, oa
+
+    def __call__(self, projs, scales=None, outs=None, *, stream=None, graph=True):
+        import torch
+        outs, pa, sa, oa = self._arrays(projs, scales, outs)
+        s = stream if stream is not None else torch.cuda.current_stream(projs[0].device)
+        fn = self.lib.fdkhip_batch_execute if graph else self.lib.fdkhip_batch_execute_direct
+        _lib.check(fn(self.handle, pa, sa, oa, C.c_void_p(s.cuda_stream)))
+        return outs
+
+    def profile(self, projs, scales=None, outs=None, *, reps=5, stream=None):
+        """Average device milliseconds (fdkhip_batch_profile):
+        ``{"prep": fills + padded copies, "back": persistent kernel}``."""
+        import torch
+        outs, pa, sa, oa = self._arrays(projs, scales, outs)
+        s = stream if stream is not None else torch.cuda.current_stream(projs[0].device)
+        ms = (C.c_float * 2)()
+        _lib.check(self.lib.fdkhip_batch_profile(self.handle, pa, sa, oa,
+                                                 C.c_void_p(s.cuda_stream), int(reps), ms))
+        return {"prep": ms[0], "back": ms[1]}
+
+    def autotune(self, projs, scales=None, outs=None, *, reps=3, stream=None):
+        """fdkhip_batch_autotune on these buffers; returns the best average ms."""
+        import torch
+        outs, pa, sa, oa = self._arrays(projs, scales, outs)
+        s = stream if stream is not None else torch.cuda.current_stream(projs[0].device)
+        best = C.c_float()
+        _lib.check(self.lib.fdkhip_batch_autotune(self.handle, pa, sa, oa,
+                                                  C.c_void_p(s.cuda_stream), int(reps),
+                                                  C.byref(best)))
+        return float(best.value)
 
 
 def quantize_views(proj, fmt="f16", *, stream=None):
@@ -361,5 +519,5 @@ def quantize_views(proj, fmt="f16", *, stream=None):
     return q, scale
 
 
-__all__ = ["fdk_mex", "cbct_back", "FdkPlan", "FdkMexError", "FdkhipError",
+__all__ = ["fdk_mex", "cbct_back", "FdkPlan", "FdkBatch", "FdkMexError", "FdkhipError",
            "quantize_views", "geom_of", "angle_scale"]

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -178,7 +179,8 @@ struct fdkhip_plan {
 	unsigned long long *d_count = nullptr;
 	int *d_maxarea = nullptr;
 	int *d_any = nullptr;
-	void *d_zeros = nullptr;      // zero words: source of staged pad samples
+	void *d_padded = nullptr;     // [na][ns+2][nt+2] zero-padded copy of the views
+	size_t padded_bytes = 0;
 	// launch configuration
 	int nw = 8, zl = 16, vstage = 8, slot = 0;
 	int z0 = 0, ngroups = 0, tiles_x = 0, tiles_y = 0;
@@ -203,7 +205,7 @@ static BackArgs make_args(const fdkhip_plan *p, const void *proj, const float *s
 	memset(&a, 0, sizeof a);
 	a.g = p->gf;
 	a.proj = proj;
-	a.zeros = p->d_zeros;
+	a.padded = p->d_padded;
 	a.scale = scale;
 	a.trig = p->d_trig;
 	a.mask = p->d_mask;
@@ -223,8 +225,8 @@ static BackArgs make_args(const fdkhip_plan *p, const void *proj, const float *s
 
 static void drop_graph(fdkhip_plan *p)
 {
-	if (p->gexec) hipGraphExecDestroy(p->gexec);
-	if (p->graph) hipGraphDestroy(p->graph);
+	if (p->gexec) (void) hipGraphExecDestroy(p->gexec);
+	if (p->graph) (void) hipGraphDestroy(p->graph);
 	p->gexec = nullptr;
 	p->graph = nullptr;
 	p->g_proj = nullptr;
@@ -291,16 +293,18 @@ static int plan_free(fdkhip_plan *p)
 	if (!p)
 		return FDKHIP_OK;
 	drop_graph(p);
-	if (p->cap_stream) hipStreamDestroy(p->cap_stream);
-	hipFree(p->d_trig);
-	hipFree(p->d_mask);
-	hipFree(p->d_count);
-	hipFree(p->d_maxarea);
-	hipFree(p->d_any);
-	hipFree(p->d_zeros);
+	if (p->cap_stream) (void) hipStreamDestroy(p->cap_stream);
+	(void) hipFree(p->d_trig);
+	(void) hipFree(p->d_mask);
+	(void) hipFree(p->d_count);
+	(void) hipFree(p->d_maxarea);
+	(void) hipFree(p->d_any);
+	(void) hipFree(p->d_padded);
 	delete p;
 	return FDKHIP_OK;
 }
+
+static int apply_tuned(fdkhip_plan *p);
 
 extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	int32_t na, const double *beta, const uint8_t *mask_host, int32_t fmt, int32_t math)
@@ -326,7 +330,10 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	p->fmt = fmt;
 	p->math = (fmt == FDKHIP_F32) ? math : FDKHIP_MATH_FAST;
 	p->gf = narrow_geom(g, na);
-	hipGetDevice(&p->device);
+	if (hipGetDevice(&p->device) != hipSuccess) {
+		delete p;
+		return fail(FDKHIP_EHIP, "no current HIP device");
+	}
 	int lds = 0;
 	if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device)
 		== hipSuccess && lds > 0)
@@ -355,8 +362,12 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	PLAN_TRY(hipMalloc(&p->d_count, sizeof(unsigned long long)));
 	PLAN_TRY(hipMalloc(&p->d_maxarea, sizeof(int)));
 	PLAN_TRY(hipMalloc(&p->d_any, sizeof(int)));
-	PLAN_TRY(hipMalloc(&p->d_zeros, 256));
-	PLAN_TRY(hipMemset(p->d_zeros, 0, 256));
+	{
+		const size_t esz = (fmt == FDKHIP_F32) ? 4 : 2;
+		p->padded_bytes = (size_t) na * (g->ns + 2) * (g->nt + 2) * esz;
+		if (p->padded_bytes > 0)
+			PLAN_TRY(hipMalloc(&p->d_padded, p->padded_bytes));
+	}
 #undef PLAN_TRY
 
 	support_z(g, mask_host, p->zs_lo, p->zs_hi);
@@ -366,6 +377,8 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	if (p->zs_hi - p->zs_lo <= 64)
 		nw = 4;
 	rc = configure(p, nw, zl, 0, 0);
+	if (rc == FDKHIP_OK)
+		rc = apply_tuned(p);
 	if (rc) {
 		plan_free(p);
 		return rc;
@@ -417,21 +430,32 @@ extern "C" int fdkhip_plan_count_updates(fdkhip_plan *p, int64_t *n)
 	return FDKHIP_OK;
 }
 
+// The plan's launch sequence: zero fill of the slices no view reaches, the
+// zero-padded copy of the views, the tiled backprojection.  With `ev`, the
+// events ev[0..3] are recorded before the fill and after each step.
 static int plan_launch(fdkhip_plan *p, const void *proj, const float *scale, float *image,
-	hipStream_t s)
+	hipStream_t s, hipEvent_t *ev = nullptr)
 {
 	const long ncol = (long) p->geom.nx * p->geom.ny;
 	const int zend = std::min(p->z0 + p->ngroups * p->nw * p->zl, p->geom.nz);
 	const int zbeg = p->ngroups > 0 ? p->z0 : 0;
-	if (ncol > 0 && p->geom.nz > 0)
-		HIP_TRY(launch_fill(image, ncol, p->geom.nz, zbeg, p->ngroups > 0 ? zend : 0, s));
-	if (p->ngroups > 0 && ncol > 0 && p->na > 0) {
+	const bool work = p->ngroups > 0 && ncol > 0 && p->na > 0;
+	if (ev)
+		HIP_TRY(hipEventRecord(ev[0], s));
+	if (ncol > 0 && p->geom.nz > 0)   // no views: the z-groups are zeroed too
+		HIP_TRY(launch_fill(image, ncol, p->geom.nz, work ? zbeg : 0, work ? zend : 0, s));
+	if (ev)
+		HIP_TRY(hipEventRecord(ev[1], s));
+	if (work)
+		HIP_TRY(launch_pad(proj, p->d_padded, p->geom.ns, p->geom.nt, p->na, p->fmt, s));
+	if (ev)
+		HIP_TRY(hipEventRecord(ev[2], s));
+	if (work) {
 		BackArgs a = make_args(p, proj, scale, image);
 		HIP_TRY(launch_back(a, p->nw, p->zl, p->fmt, p->math == FDKHIP_MATH_FAST, s));
-	} else if (p->ngroups > 0 && ncol > 0) {
-		// no views: zero the z-groups too
-		HIP_TRY(launch_fill(image, ncol, p->geom.nz, 0, 0, s));
 	}
+	if (ev)
+		HIP_TRY(hipEventRecord(ev[3], s));
 	return FDKHIP_OK;
 }
 
@@ -504,7 +528,7 @@ extern "C" int fdkhip_quantize_views(const float *proj, int32_t ns, int32_t nt, 
 	int *maxbits = nullptr;
 	HIP_TRY(hipMallocAsync((void **) &maxbits, sizeof(int) * na, s));
 	hipError_t e = launch_quantize(proj, ns, nt, na, fmt, out, scale, maxbits, s);
-	hipFreeAsync(maxbits, s);
+	(void) hipFreeAsync(maxbits, s);
 	if (e != hipSuccess)
 		return fail(FDKHIP_EHIP, "quantize: %s", hipGetErrorString(e));
 	return FDKHIP_OK;
@@ -551,7 +575,7 @@ static fdkhip_geom mex_geom(const int32_t *nxyz, const double *dxyz, const doubl
 // RAII device buffer for the synchronous entry points
 struct DevBuf {
 	void *p = nullptr;
-	~DevBuf() { if (p) hipFree(p); }
+	~DevBuf() { if (p) (void) hipFree(p); }
 };
 
 extern "C" int fdkhip_fdk_ts_back(
@@ -654,4 +678,548 @@ extern "C" int fdkhip_fdk_st_back(
 	}
 	HIP_TRY(hipMemcpy(image, dimg.p, nvox * sizeof(float), hipMemcpyDeviceToHost));
 	return FDKHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuning: launch shape and views per LDS stage, chosen by timing
+
+namespace {
+
+// launch shapes (consumer waves, slices per lane) the library is built with
+const int kShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}};
+const int kStages[] = {0, 4, 8, 12, 16};
+
+struct Tuned {
+	int nw, zl, vstage;
+};
+
+std::mutex g_tune_mu;
+std::vector<std::pair<std::string, Tuned>> g_tuned;   // geometry key -> choice
+
+std::string tune_key(const fdkhip_plan *p)
+{
+	char buf[512];
+	const fdkhip_geom &g = p->geom;
+	snprintf(buf, sizeof buf, "%d %d %d %a %a %a %a %a %a %a %a %a %d %d %a %a %a %a %d %d %d",
+		g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.offset_x, g.offset_y, g.offset_z, g.dso, g.dsd,
+		g.dfs, g.ns, g.nt, g.ds, g.dt, g.offset_s, g.offset_t, p->na, p->fmt, p->math);
+	return buf;
+}
+
+bool tuned_lookup(const std::string &key, Tuned &t)
+{
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	for (auto &kv : g_tuned)
+		if (kv.first == key) {
+			t = kv.second;
+			return true;
+		}
+	return false;
+}
+
+void tuned_store(const std::string &key, const Tuned &t)
+{
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	for (auto &kv : g_tuned)
+		if (kv.first == key) {
+			kv.second = t;
+			return;
+		}
+	g_tuned.emplace_back(key, t);
+}
+
+} // namespace
+
+// average milliseconds of `reps` calls of `run` on `s` after one warm-up call
+static int time_launches(const std::function<int(hipStream_t)> &run, hipStream_t s, int reps,
+	float *ms)
+{
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	HIP_TRY(hipEventCreate(&e0));
+	hipError_t e = hipEventCreate(&e1);
+	if (e != hipSuccess) {
+		(void) hipEventDestroy(e0);
+		return fail(FDKHIP_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+	}
+	int rc = run(s);
+	if (!rc && hipEventRecord(e0, s) != hipSuccess)
+		rc = fail(FDKHIP_EHIP, "hipEventRecord");
+	for (int r = 0; r < reps && !rc; ++r)
+		rc = run(s);
+	if (!rc && hipEventRecord(e1, s) != hipSuccess)
+		rc = fail(FDKHIP_EHIP, "hipEventRecord");
+	if (!rc && hipEventSynchronize(e1) != hipSuccess)
+		rc = fail(FDKHIP_EHIP, "hipEventSynchronize");
+	float t = 0.f;
+	if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+		rc = fail(FDKHIP_EHIP, "hipEventElapsedTime");
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	if (!rc)
+		*ms = t / (float) std::max(reps, 1);
+	return rc;
+}
+
+extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const float *scale,
+	float *image, void *stream, int32_t reps, float *best_ms)
+{
+	int rc = check_exec(p, proj, scale, image);
+	if (rc)
+		return rc;
+	if (reps < 1)
+		return fail(FDKHIP_EARG, "reps must be >= 1");
+	hipStream_t s = (hipStream_t) stream;
+	Tuned best{p->nw, p->zl, p->vstage};
+	float best_t = 3.4e38f;
+	for (const auto &sh : kShapes) {
+		int last_vstage = -1;
+		for (int vs : kStages) {
+			if (configure(p, sh[0], sh[1], vs, 0) != FDKHIP_OK)
+				continue;
+			if (p->vstage == last_vstage)
+				continue;   // the LDS budget clamped it to a depth already timed
+			last_vstage = p->vstage;
+			float t = 0.f;
+			rc = time_launches([&](hipStream_t st) {
+				return plan_launch(p, proj, scale, image, st);
+			}, s, reps, &t);
+			if (rc)
+				return rc;
+			if (t < best_t) {
+				best_t = t;
+				best = Tuned{sh[0], sh[1], p->vstage};
+			}
+		}
+	}
+	rc = configure(p, best.nw, best.zl, best.vstage, 0);
+	if (rc)
+		return rc;
+	tuned_store(tune_key(p), best);
+	if (best_ms)
+		*best_ms = best_t;
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_plan_profile(fdkhip_plan *p, const void *proj, const float *scale,
+	float *image, void *stream, int32_t reps, float ms[3])
+{
+	int rc = check_exec(p, proj, scale, image);
+	if (rc)
+		return rc;
+	if (reps < 1 || !ms)
+		return fail(FDKHIP_EARG, "reps must be >= 1 and ms non-NULL");
+	hipStream_t s = (hipStream_t) stream;
+	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+	for (auto &e : ev)
+		if (hipEventCreate(&e) != hipSuccess) {
+			rc = fail(FDKHIP_EHIP, "hipEventCreate");
+			break;
+		}
+	double sum[3] = {0, 0, 0};
+	for (int r = 0; r < reps && !rc; ++r) {
+		rc = plan_launch(p, proj, scale, image, s, ev);
+		if (!rc && hipEventSynchronize(ev[3]) != hipSuccess)
+			rc = fail(FDKHIP_EHIP, "hipEventSynchronize");
+		for (int k = 0; k < 3 && !rc; ++k) {
+			float t = 0.f;
+			if (hipEventElapsedTime(&t, ev[k], ev[k + 1]) != hipSuccess)
+				rc = fail(FDKHIP_EHIP, "hipEventElapsedTime");
+			sum[k] += t;
+		}
+	}
+	for (auto &e : ev)
+		if (e)
+			(void) hipEventDestroy(e);
+	if (!rc)
+		for (int k = 0; k < 3; ++k)
+			ms[k] = (float) (sum[k] / reps);
+	return rc;
+}
+
+// apply a choice made earlier in this process for the same geometry
+static int apply_tuned(fdkhip_plan *p)
+{
+	Tuned t;
+	if (!tuned_lookup(tune_key(p), t))
+		return FDKHIP_OK;
+	return configure(p, t.nw, t.zl, t.vstage, 0);
+}
+
+// ---------------------------------------------------------------------------
+// batches of scans: one persistent launch driven by a device work queue
+
+struct fdkhip_batch {
+	int nscan = 0, fmt = FMT_F32, math = FDKHIP_MATH_FAST;
+	std::vector<fdkhip_plan *> plans;
+	int nw = 8, zl = 16, vstage = 8, slot = 64, nwg = 256;
+	long long nitems = 0;
+	std::vector<long long> item_begin;
+	std::vector<BackArgs> host_args;
+	BackArgs *d_scans = nullptr;
+	long long *d_item_begin = nullptr;
+	unsigned long long *d_counter = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t gexec = nullptr;
+	hipStream_t cap_stream = nullptr;
+	std::vector<const void *> g_proj;
+	std::vector<const float *> g_scale;
+	std::vector<float *> g_image;
+};
+
+static void batch_drop_graph(fdkhip_batch *b)
+{
+	if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
+	if (b->graph) (void) hipGraphDestroy(b->graph);
+	b->gexec = nullptr;
+	b->graph = nullptr;
+	b->g_proj.clear();
+	b->g_scale.clear();
+	b->g_image.clear();
+}
+
+static int batch_free(fdkhip_batch *b)
+{
+	if (!b)
+		return FDKHIP_OK;
+	batch_drop_graph(b);
+	if (b->cap_stream) (void) hipStreamDestroy(b->cap_stream);
+	for (auto *p : b->plans)
+		plan_free(p);
+	(void) hipFree(b->d_scans);
+	(void) hipFree(b->d_item_begin);
+	(void) hipFree(b->d_counter);
+	delete b;
+	return FDKHIP_OK;
+}
+
+// common launch shape for every scan, LDS layout sized for the largest one
+static int batch_configure(fdkhip_batch *b, int nw, int zl, int vstage_req)
+{
+	if (!back_config_supported(nw, zl))
+		return fail(FDKHIP_EUNSUPPORTED, "configuration waves=%d zper=%d not built", nw, zl);
+	int maxarea = 64;
+	for (auto *p : b->plans) {
+		int rc = configure(p, nw, zl, 0, 0);
+		if (rc)
+			return rc;
+		maxarea = std::max(maxarea, p->maxarea);
+	}
+	const size_t lds_max = b->plans.empty() ? 65536 : b->plans[0]->lds_max;
+	const size_t lds_limit = (nw >= 8) ? lds_max : lds_max / 2;
+	int slot = (maxarea + 63) & ~63;
+	int vstage = vstage_req > 0 ? vstage_req : 16;
+	while (vstage > 1 && back_lds_bytes(nw, vstage, slot) > lds_limit)
+		--vstage;
+	while (slot > 64 && back_lds_bytes(nw, vstage, slot) > lds_limit)
+		slot -= 64;
+	if (back_lds_bytes(nw, vstage, slot) > lds_limit)
+		return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
+	b->nw = nw;
+	b->zl = zl;
+	b->vstage = vstage;
+	b->slot = slot;
+	b->item_begin.assign(1, 0);
+	for (auto *p : b->plans) {
+		p->vstage = vstage;
+		p->slot = slot;
+		const long long items = (long long) p->tiles_x * p->tiles_y
+			* ((p->na > 0) ? p->ngroups : 0);
+		b->item_begin.push_back(b->item_begin.back() + items);
+	}
+	b->nitems = b->item_begin.back();
+	int cus = 0, dev = 0;
+	HIP_TRY(hipGetDevice(&dev));
+	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess
+		|| cus <= 0)
+		cus = 256;
+	b->nwg = cus * batch_blocks_per_cu(nw, zl, vstage, slot, b->fmt,
+		b->math == FDKHIP_MATH_FAST);
+	(void) hipFree(b->d_item_begin);
+	b->d_item_begin = nullptr;
+	HIP_TRY(hipMalloc(&b->d_item_begin, sizeof(long long) * b->item_begin.size()));
+	HIP_TRY(hipMemcpy(b->d_item_begin, b->item_begin.data(),
+		sizeof(long long) * b->item_begin.size(), hipMemcpyHostToDevice));
+	batch_drop_graph(b);
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_batch_create(fdkhip_batch **out, int32_t nscan, const fdkhip_geom *geoms,
+	const int32_t *na, const double *const *beta, const uint8_t *const *mask, int32_t fmt,
+	int32_t math)
+{
+	if (!out)
+		return fail(FDKHIP_EARG, "batch output pointer is NULL");
+	*out = nullptr;
+	if (nscan < 1 || !geoms || !na || !beta)
+		return fail(FDKHIP_EARG, "a batch needs nscan >= 1 and per-scan geometry, na, beta");
+	fdkhip_batch *b = new fdkhip_batch;
+	b->nscan = nscan;
+	b->fmt = fmt;
+	b->math = (fmt == FDKHIP_F32) ? math : FDKHIP_MATH_FAST;
+	for (int i = 0; i < nscan; ++i) {
+		fdkhip_plan *p = nullptr;
+		int rc = fdkhip_plan_create(&p, &geoms[i], na[i], beta[i], mask ? mask[i] : nullptr,
+			fmt, math);
+		if (rc) {
+			batch_free(b);
+			return rc;
+		}
+		b->plans.push_back(p);
+	}
+	int rc = batch_configure(b, 8, 16, 0);
+	if (rc) {
+		batch_free(b);
+		return rc;
+	}
+	hipError_t e = hipMalloc(&b->d_scans, sizeof(BackArgs) * nscan);
+	if (e == hipSuccess)
+		e = hipMalloc(&b->d_counter, sizeof(unsigned long long));
+	if (e != hipSuccess) {
+		batch_free(b);
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	}
+	*out = b;
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_batch_destroy(fdkhip_batch *b)
+{
+	return batch_free(b);
+}
+
+extern "C" int fdkhip_batch_config(const fdkhip_batch *b, int32_t cfg[6])
+{
+	if (!b || !cfg)
+		return fail(FDKHIP_EARG, "NULL argument");
+	cfg[0] = b->nw;
+	cfg[1] = b->zl;
+	cfg[2] = b->vstage;
+	cfg[3] = b->slot;
+	cfg[4] = b->nwg;
+	cfg[5] = (int32_t) std::min(b->nitems, (long long) 0x7fffffff);
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_batch_set_config(fdkhip_batch *b, int32_t waves, int32_t zper,
+	int32_t vstage)
+{
+	if (!b)
+		return fail(FDKHIP_EARG, "batch is NULL");
+	if (vstage < 0 || vstage > 64)
+		return fail(FDKHIP_EARG, "views per stage out of range (%d)", vstage);
+	return batch_configure(b, waves, zper, vstage);
+}
+
+extern "C" int fdkhip_batch_count_updates(fdkhip_batch *b, int64_t *n)
+{
+	if (!b || !n)
+		return fail(FDKHIP_EARG, "NULL argument");
+	long long s = 0;
+	for (auto *p : b->plans)
+		s += p->n_updates;
+	*n = s;
+	return FDKHIP_OK;
+}
+
+// Per-scan zero fills and padded copies, then the persistent kernel (and its
+// queue reset).  With `ev`, ev[0..2] are recorded before the per-scan steps,
+// between them and the kernel, and after it.
+static int batch_launch(fdkhip_batch *b, const void *const *proj, const float *const *scale,
+	float *const *image, hipStream_t s, hipEvent_t *ev = nullptr)
+{
+	if (ev)
+		HIP_TRY(hipEventRecord(ev[0], s));
+	for (int i = 0; i < b->nscan; ++i) {
+		fdkhip_plan *p = b->plans[i];
+		const long ncol = (long) p->geom.nx * p->geom.ny;
+		if (ncol == 0 || p->geom.nz == 0)
+			continue;
+		const bool work = p->ngroups > 0 && p->na > 0;
+		const int zend = std::min(p->z0 + p->ngroups * p->nw * p->zl, p->geom.nz);
+		HIP_TRY(launch_fill(image[i], ncol, p->geom.nz, work ? p->z0 : 0, work ? zend : 0, s));
+		if (work)
+			HIP_TRY(launch_pad(proj[i], p->d_padded, p->geom.ns, p->geom.nt, p->na, p->fmt, s));
+	}
+	if (ev)
+		HIP_TRY(hipEventRecord(ev[1], s));
+	if (b->nitems == 0) {
+		if (ev)
+			HIP_TRY(hipEventRecord(ev[2], s));
+		return FDKHIP_OK;
+	}
+	BatchArgs ba;
+	ba.scans = b->d_scans;
+	ba.item_begin = b->d_item_begin;
+	ba.nscan = b->nscan;
+	ba.nitems = b->nitems;
+	ba.counter = b->d_counter;
+	HIP_TRY(launch_batch(ba, b->nwg, b->nw, b->zl, b->vstage, b->slot, b->fmt,
+		b->math == FDKHIP_MATH_FAST, s));
+	if (ev)
+		HIP_TRY(hipEventRecord(ev[2], s));
+	return FDKHIP_OK;
+}
+
+static int batch_check(fdkhip_batch *b, const void *const *proj, const float *const *scale,
+	float *const *image)
+{
+	if (!b)
+		return fail(FDKHIP_EARG, "batch is NULL");
+	if (!proj || !image || (b->fmt != FDKHIP_F32 && !scale))
+		return fail(FDKHIP_EARG, "per-scan proj/image (and scale for F16/BF16) arrays required");
+	for (int i = 0; i < b->nscan; ++i) {
+		int rc = check_exec(b->plans[i], proj[i], scale ? scale[i] : nullptr, image[i]);
+		if (rc)
+			return rc;
+	}
+	return FDKHIP_OK;
+}
+
+// per-scan kernel arguments for these buffers -> device (ordered on `s`)
+static int batch_upload_args(fdkhip_batch *b, const void *const *proj,
+	const float *const *scale, float *const *image, hipStream_t s)
+{
+	b->host_args.resize(b->nscan);
+	for (int i = 0; i < b->nscan; ++i)
+		b->host_args[i] = make_args(b->plans[i], proj[i], scale ? scale[i] : nullptr, image[i]);
+	HIP_TRY(hipMemcpyAsync(b->d_scans, b->host_args.data(), sizeof(BackArgs) * b->nscan,
+		hipMemcpyHostToDevice, s));
+	HIP_TRY(hipStreamSynchronize(s));   // host_args may be rewritten by the next call
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_batch_execute_direct(fdkhip_batch *b, const void *const *proj,
+	const float *const *scale, float *const *image, void *stream)
+{
+	int rc = batch_check(b, proj, scale, image);
+	if (rc)
+		return rc;
+	hipStream_t s = (hipStream_t) stream;
+	rc = batch_upload_args(b, proj, scale, image, s);
+	if (rc)
+		return rc;
+	batch_drop_graph(b);   // the uploaded arguments may differ from the graph's
+	return batch_launch(b, proj, scale, image, s);
+}
+
+extern "C" int fdkhip_batch_execute(fdkhip_batch *b, const void *const *proj,
+	const float *const *scale, float *const *image, void *stream)
+{
+	int rc = batch_check(b, proj, scale, image);
+	if (rc)
+		return rc;
+	hipStream_t s = (hipStream_t) stream;
+	bool same = b->gexec && (int) b->g_proj.size() == b->nscan;
+	for (int i = 0; same && i < b->nscan; ++i)
+		same = b->g_proj[i] == proj[i] && b->g_image[i] == image[i]
+			&& b->g_scale[i] == (scale ? scale[i] : nullptr);
+	if (!same) {
+		batch_drop_graph(b);
+		rc = batch_upload_args(b, proj, scale, image, s);
+		if (rc)
+			return rc;
+		if (!b->cap_stream)
+			HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+		HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+		rc = batch_launch(b, proj, scale, image, b->cap_stream);
+		hipGraph_t graph = nullptr;
+		hipError_t e = hipStreamEndCapture(b->cap_stream, &graph);
+		if (rc)
+			return rc;
+		if (e != hipSuccess)
+			return fail(FDKHIP_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+		b->graph = graph;
+		HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+		for (int i = 0; i < b->nscan; ++i) {
+			b->g_proj.push_back(proj[i]);
+			b->g_scale.push_back(scale ? scale[i] : nullptr);
+			b->g_image.push_back(image[i]);
+		}
+	}
+	HIP_TRY(hipGraphLaunch(b->gexec, s));
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_batch_autotune(fdkhip_batch *b, const void *const *proj,
+	const float *const *scale, float *const *image, void *stream, int32_t reps, float *best_ms)
+{
+	int rc = batch_check(b, proj, scale, image);
+	if (rc)
+		return rc;
+	if (reps < 1)
+		return fail(FDKHIP_EARG, "reps must be >= 1");
+	hipStream_t s = (hipStream_t) stream;
+	int bn = b->nw, bz = b->zl, bv = b->vstage;
+	float best_t = 3.4e38f;
+	for (const auto &sh : kShapes) {
+		int last_vstage = -1;
+		for (int vs : kStages) {
+			if (batch_configure(b, sh[0], sh[1], vs) != FDKHIP_OK)
+				continue;
+			if (b->vstage == last_vstage)
+				continue;
+			last_vstage = b->vstage;
+			rc = batch_upload_args(b, proj, scale, image, s);
+			if (rc)
+				return rc;
+			float t = 0.f;
+			rc = time_launches([&](hipStream_t st) {
+				return batch_launch(b, proj, scale, image, st);
+			}, s, reps, &t);
+			if (rc)
+				return rc;
+			if (t < best_t) {
+				best_t = t;
+				bn = sh[0];
+				bz = sh[1];
+				bv = b->vstage;
+			}
+		}
+	}
+	rc = batch_configure(b, bn, bz, bv);
+	if (rc)
+		return rc;
+	if (best_ms)
+		*best_ms = best_t;
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_batch_profile(fdkhip_batch *b, const void *const *proj,
+	const float *const *scale, float *const *image, void *stream, int32_t reps, float ms[2])
+{
+	int rc = batch_check(b, proj, scale, image);
+	if (rc)
+		return rc;
+	if (reps < 1 || !ms)
+		return fail(FDKHIP_EARG, "reps must be >= 1 and ms non-NULL");
+	hipStream_t s = (hipStream_t) stream;
+	rc = batch_upload_args(b, proj, scale, image, s);
+	if (rc)
+		return rc;
+	batch_drop_graph(b);
+	hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+	for (auto &e : ev)
+		if (hipEventCreate(&e) != hipSuccess) {
+			rc = fail(FDKHIP_EHIP, "hipEventCreate");
+			break;
+		}
+	double sum[2] = {0, 0};
+	for (int r = 0; r < reps && !rc; ++r) {
+		rc = batch_launch(b, proj, scale, image, s, ev);
+		if (!rc && hipEventSynchronize(ev[2]) != hipSuccess)
+			rc = fail(FDKHIP_EHIP, "hipEventSynchronize");
+		for (int k = 0; k < 2 && !rc; ++k) {
+			float t = 0.f;
+			if (hipEventElapsedTime(&t, ev[k], ev[k + 1]) != hipSuccess)
+				rc = fail(FDKHIP_EHIP, "hipEventElapsedTime");
+			sum[k] += t;
+		}
+	}
+	for (auto &e : ev)
+		if (e)
+			(void) hipEventDestroy(e);
+	if (!rc)
+		for (int k = 0; k < 2; ++k)
+			ms[k] = (float) (sum[k] / reps);
+	return rc;
 }

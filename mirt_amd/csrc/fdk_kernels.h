@@ -22,7 +22,7 @@ struct GeomF {
 struct BackArgs {
 	GeomF g;
 	const void *proj;        // [na][ns][nt] (t fastest), storage format fmt
-	const void *zeros;       // >= 4 zero bytes: source of the staged pad samples
+	const void *padded;      // [na][ns+2][nt+2] zero-padded copy of proj (same format)
 	const float *scale;      // [na] per-view scale (F16/BF16), else unused
 	const float2 *trig;      // [na] {cosf(beta), sinf(beta)} of the float beta
 	const uint8_t *mask;     // [ny][nx] or null
@@ -33,6 +33,16 @@ struct BackArgs {
 	unsigned long long *count;   // survey: reference loop iterations
 	int *maxarea;            // survey: largest staged footprint (floats)
 	int *fov_any;            // survey: some column meets some view
+};
+
+// a batch of scans for the persistent kernel: scans[i] holds scan i's
+// arguments; its work items are [item_begin[i], item_begin[i+1])
+struct BatchArgs {
+	const BackArgs *scans;         // [nscan] in device memory
+	const long long *item_begin;   // [nscan + 1]
+	int nscan;
+	long long nitems;
+	unsigned long long *counter;   // work-queue head, reset before each launch
 };
 
 struct StArgs {
@@ -47,8 +57,14 @@ size_t back_lds_bytes(int nw, int vstage, int slot);
 bool back_config_supported(int nw, int zl);
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s);
 hipError_t launch_survey(const BackArgs &a, int nw, int zl, hipStream_t s);
+hipError_t launch_batch(const BatchArgs &b, int nwg, int nw, int zl, int vstage, int slot,
+	int fmt, bool fast, hipStream_t s);
+// resident persistent workgroups per CU for this shape and LDS layout
+int batch_blocks_per_cu(int nw, int zl, int vstage, int slot, int fmt, bool fast);
 hipError_t launch_fill(float *image, long ncol, int nz, int zlo, int zhi, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
+hipError_t launch_pad(const void *src, void *dst, int ns, int nt, int na, int fmt,
+	hipStream_t s);
 hipError_t launch_zero_i32(int *p, int n, hipStream_t s);
 hipError_t launch_st(const StArgs &a, hipStream_t s);
 hipError_t launch_quantize(const float *proj, int ns, int nt, int na, int fmt,

@@ -178,6 +178,8 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 	// later chunks: ZL additions each (lanes past their range add harmlessly)
 #pragma unroll
 	for (int k = 1; k < NW; ++k) {
+		if (__all(zhi <= zg0 + k * ZL))
+			break;                                 // no active lane reaches chunk k
 		if (k > k0) {
 			Tk[k] = tt;
 #pragma unroll
@@ -206,20 +208,28 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 	return zlo < zhi;
 }
 
-// Wave-wide reductions (wave64).
+// Wave-wide reductions (wave64, every lane active) with DPP: quad swaps,
+// row half-mirror and mirror give each row of 16 its result, row_bcast:15
+// and :31 carry it across rows; lane 63 holds the wave's result.
+template <bool MAX>
+__device__ __forceinline__ int wave_reduce(int v)
+{
+	auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xb1, 0xf, 0xf, false));   // quad_perm 1,0,3,2
+	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4e, 0xf, 0xf, false));   // quad_perm 2,3,0,1
+	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xf, 0xf, false));  // row_half_mirror
+	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xf, 0xf, false));  // row_mirror
+	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+	return __builtin_amdgcn_readlane(v, 63);
+}
 __device__ __forceinline__ int wave_min(int v)
 {
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1)
-		v = min(v, __shfl_xor(v, o, 64));
-	return v;
+	return wave_reduce<false>(v);
 }
 __device__ __forceinline__ int wave_max(int v)
 {
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1)
-		v = max(v, __shfl_xor(v, o, 64));
-	return v;
+	return wave_reduce<true>(v);
 }
 
 // Logical block index with consecutive indices on the same XCD: the hardware
@@ -269,12 +279,26 @@ struct Lds {
 enum { MODE_LDS = 0, MODE_GLOBAL = 1, MODE_NONE = 2 };
 
 
+// Asynchronous copy of one sample per lane from HBM into the LDS word
+// `ldst + lane` (global_load_lds: the LDS address is wave-uniform plus
+// 4 bytes per lane; 16-bit samples land zero-extended in their word).
+template <int FMT>
+__device__ __forceinline__ void stage_word(const char *src, float *ldst)
+{
+	const auto g = (const void __attribute__((address_space(1))) *) src;
+	const auto l = (void __attribute__((address_space(3))) *) ldst;
+	if constexpr (FMT == FMT_F32)
+		__builtin_amdgcn_global_load_lds(g, l, 4, 0, 0);
+	else
+		__builtin_amdgcn_global_load_lds(g, l, 2, 0, 0);
+}
+
 // Producer side: per-(column, view) setup of `nv` views into the stage
 // buffer at word offset `bo`, and staging of their footprints, by producer
 // wave `pw` of NP.  The footprint goes to LDS with asynchronous
-// global-to-LDS loads (one dword per lane: the FP32 sample, or the 16-bit
-// sample zero-extended); pad samples are read from `zeros`.  The caller
-// waits for the loads (vmcnt) before the stage barrier.
+// global-to-LDS loads from the zero-padded copy of the views (one dword
+// per lane: the FP32 sample, or the 16-bit sample zero-extended).  The
+// caller waits for the loads (vmcnt) before the stage barrier.
 template <int NW, int NP, int ZL, int FMT, bool FAST>
 __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW> &L,
 	int v0, int nv, int pw, int lane, bool colvalid, float xx, float yy, int zg0, int zg1)
@@ -282,7 +306,6 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 	const GeomF &g = a.g;
 	float *buf = fdk_lds + bo;
 	int *bufi = reinterpret_cast<int *>(fdk_lds) + bo;
-	const long view_elems = (long) g.ns * g.nt;
 	for (int vv = pw; vv < nv; vv += NP) {
 		const int v = v0 + vv;
 		const float2 cs = a.trig[v];
@@ -293,14 +316,15 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 		float Tk[NW];
 		if (act)
 			act = setup_walk<NW, ZL>(g, c, zg0, zg1, Tk, zlo, zhi, t_first, t_last);
-		// footprint of this view over the tile and z-group (padded coords)
-		const int s_lo = wave_min(act ? c.is + 1 : 0x7fffffff);
-		const int s_hi = wave_max(act ? c.is + 2 : (int) 0x80000000);
-		const int t_lo = wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff);
+		// footprint of this view over the tile and z-group (padded coords),
+		// kept inside the padded view [0, ns+1] x [0, nt+1]
+		const int s_lo = max(wave_min(act ? c.is + 1 : 0x7fffffff), 0);
+		const int s_hi = min(wave_max(act ? c.is + 2 : (int) 0x80000000), g.ns + 1);
+		const int t_lo = max(wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff), 0);
 		const int t_hi = min(wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000),
 			g.nt + 1);
 		int mode = MODE_NONE, tw = 0, sw = 0;
-		if (s_lo <= s_hi) {
+		if (s_lo <= s_hi && t_lo <= t_hi) {
 			tw = t_hi - t_lo + 1;
 			sw = s_hi - s_lo + 1;
 			mode = ((long) sw * tw <= a.slot) ? MODE_LDS : MODE_GLOBAL;
@@ -321,9 +345,14 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 		bufi[L.zlo + idx] = zlo;
 		bufi[L.zhi + idx] = act ? zhi : zlo;
 		bufi[L.is + idx] = act ? c.is : 0;
+		// t_bin at the first active slice of each chunk; chunks the lane's
+		// range misses get t_first, so that their (discarded) reads stay
+		// inside the staged footprint
 #pragma unroll
-		for (int k = 0; k < NW; ++k)
-			buf[L.T + (vv * NW + k) * 64 + lane] = act ? Tk[k] : 0.f;
+		for (int k = 0; k < NW; ++k) {
+			const bool meets = zlo < zg0 + (k + 1) * ZL && zg0 + k * ZL < zhi;
+			buf[L.T + (vv * NW + k) * 64 + lane] = act ? (meets ? Tk[k] : t_first) : 0.f;
+		}
 		if (lane == 0) {
 			bufi[L.vinfo + vv * 4 + 0] = s_lo;
 			bufi[L.vinfo + vv * 4 + 1] = t_lo;
@@ -332,30 +361,36 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 		}
 		if (mode != MODE_LDS)
 			continue;
-		// stage the zero-padded footprint (fdk_ts_put_view, fdk,ts,t.c:13-21):
-		// LDS word e of the slot <- padded sample (s_lo + e / tw, t_lo + e % tw)
+		// stage the footprint of the zero-padded view (the padded copy plays
+		// fdk_ts_put_view, fdk,ts,t.c:13-21): LDS word e of the slot <- padded
+		// sample (s_lo + e / tw, t_lo + e % tw)
 		const int area = sw * tw;
-		const unsigned magic = 0xffffffffu / (unsigned) tw + 1u;   // e/tw, exact (tw >= 2)
 		const int slot = bo + vv * a.slot;
-		const char *vp = static_cast<const char *>(view_ptr<FMT>(a.proj, (long) v * view_elems));
+		const int pnt = g.nt + 2;
 		constexpr int esz = (FMT == FMT_F32) ? 4 : 2;
-		for (int e0 = 0; e0 < area; e0 += 64) {
-			const int e = e0 + lane;
-			int r = (int) __umulhi((unsigned) e, magic);
-			r -= (r * tw > e) ? 1 : 0;
-			const int tc = e - r * tw;
-			const int sc = s_lo + r - 1;      // unpadded column
-			const int tr = t_lo + tc - 1;     // unpadded row
-			const bool inb = sc >= 0 && sc < g.ns && tr >= 0 && tr < g.nt;
-			const char *src = inb ? vp + (size_t) (unsigned) (sc * g.nt + tr) * esz
-				: static_cast<const char *>(a.zeros);
-			if (e < area) {
-				const auto gsrc = (const void __attribute__((address_space(1))) *) src;
-				const auto ldst = (void __attribute__((address_space(3))) *) (fdk_lds + slot + e0);
-				if constexpr (FMT == FMT_F32)
-					__builtin_amdgcn_global_load_lds(gsrc, ldst, 4, 0, 0);
-				else
-					__builtin_amdgcn_global_load_lds(gsrc, ldst, 2, 0, 0);
+		const char *pv = static_cast<const char *>(a.padded)
+			+ ((size_t) v * (g.ns + 2) + s_lo) * pnt * esz + (size_t) t_lo * esz;
+		if (tw == pnt) {
+			// full detector height: the footprint is contiguous in the padded view
+			for (int e0 = 0; e0 < area; e0 += 64) {
+				const int e = e0 + lane;
+				if (e < area)
+					stage_word<FMT>(pv + (size_t) e * esz, fdk_lds + slot + e0);
+			}
+		} else {
+			// rows of tw samples: (r, tc) of element e, advanced by 64 per step
+			const unsigned magic = 0xffffffffu / (unsigned) tw + 1u;  // e/tw, exact (tw >= 2)
+			int r = (int) __umulhi((unsigned) lane, magic);
+			r -= (r * tw > lane) ? 1 : 0;
+			int tc = lane - r * tw;
+			const int dr = 64 / tw, dtc = 64 - dr * tw;
+			for (int e0 = 0; e0 < area; e0 += 64) {
+				if (e0 + lane < area)
+					stage_word<FMT>(pv + ((size_t) r * pnt + tc) * esz, fdk_lds + slot + e0);
+				tc += dtc;
+				const int wrap = tc >= tw ? 1 : 0;
+				tc -= wrap * tw;
+				r += dr + wrap;
 			}
 		}
 	}
@@ -441,8 +476,9 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW>
 				}
 			} else {
 				// partial chunk: slices outside [jlo, jhi) read at t0 (inside
-				// the staged footprint) and are discarded by a select
-				const unsigned nj = (unsigned) (jhi - jlo);
+				// the staged footprint) and are discarded by a select; a lane
+				// whose range misses this chunk (jhi <= jlo) adds nothing
+				const unsigned nj = (unsigned) max(jhi - jlo, 0);
 #pragma unroll
 				for (int j = 0; j < ZL; ++j) {
 					const bool r = (unsigned) (j - jlo) < nj;
@@ -453,11 +489,12 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW>
 				}
 			}
 		} else if (any) {
-			// footprint larger than the LDS slot: gather from HBM
+			// footprint larger than the LDS slot: gather from the padded copy
 			const int v = v0 + vv;
 			const int is = bufi[L.is + idx];
-			const float sc = (FMT != FMT_F32 && !FAST) ? a.scale[v] : 1.f;
-			const void *vp = view_ptr<FMT>(a.proj, (long) v * view_elems);
+			const int pnt = g.nt + 2;
+			const size_t vb = (size_t) v * (g.ns + 2) * pnt;
+			const int c1 = (is + 1) * pnt + 1, c2 = c1 + pnt;   // (s = is, t = 0), (is+1, 0)
 			float t = t0;
 #pragma unroll
 			for (int j = 0; j < ZL; ++j) {
@@ -465,21 +502,18 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW>
 					const int it = (int) floorf(t);
 					const float wu = t - (float) it;
 					const float wd = 1.f - wu;
-					float s4[4];
-#pragma unroll
-					for (int q = 0; q < 4; ++q) {
-						const int s = is + (q >> 1), tt = it + (q & 1);
-						s4[q] = (s >= 0 && s < g.ns && tt >= 0 && tt < g.nt)
-							? load_sample<FMT>(vp, s * g.nt + tt) * sc : 0.f;
-					}
+					const float a0 = load_sample<FMT>(a.padded, vb + c1 + it);
+					const float a1 = load_sample<FMT>(a.padded, vb + c1 + it + 1);
+					const float b0 = load_sample<FMT>(a.padded, vb + c2 + it);
+					const float b1 = load_sample<FMT>(a.padded, vb + c2 + it + 1);
 					if (FAST) {
-						const float q1 = fmaf(wr, s4[2], wl * s4[0]);
-						const float q2 = fmaf(wr, s4[3], wl * s4[1]);
+						const float q1 = fmaf(wr, b0, wl * a0);
+						const float q2 = fmaf(wr, b1, wl * a1);
 						acc[j] = fmaf(wd, q1, acc[j]);
 						acc[j] = fmaf(wu, q2, acc[j]);
 					} else {
-						const float q1 = wl * s4[0] + wr * s4[2];
-						const float q2 = wl * s4[1] + wr * s4[3];
+						const float q1 = wl * a0 + wr * b0;
+						const float q2 = wl * a1 + wr * b1;
 						const float p0 = wd * q1 + wu * q2;
 						acc[j] += w2 * p0;
 					}
@@ -495,18 +529,12 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW>
 // buffers: while the consumers add stage s from one buffer, the producers
 // set up and stage s+1 into the other; one barrier per stage.
 template <int NW, int NP, int ZL, int FMT, bool FAST>
-__global__ void __launch_bounds__((NW + NP) * 64)
-back_kernel(const BackArgs a)
+__device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int group)
 {
 	const GeomF &g = a.g;
 	const int tid = threadIdx.x;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int lane = tid & 63;
-
-	const int nblocks = a.ntiles * a.ngroups;
-	const int lb = xcd_block(blockIdx.x, nblocks);
-	const int tile = lb / a.ngroups;
-	const int group = lb - tile * a.ngroups;
 	const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
 	const int zg0 = a.z0 + group * (NW * ZL);
 	const int zg1 = min(zg0 + NW * ZL, g.nz);
@@ -561,6 +589,61 @@ back_kernel(const BackArgs a)
 			}
 		}
 	}
+}
+
+template <int NW, int NP, int ZL, int FMT, bool FAST>
+__global__ void __launch_bounds__((NW + NP) * 64)
+back_kernel(const BackArgs a)
+{
+	const int nblocks = a.ntiles * a.ngroups;
+	const int lb = xcd_block(blockIdx.x, nblocks);
+	const int tile = lb / a.ngroups;
+	tile_pipeline<NW, NP, ZL, FMT, FAST>(a, tile, lb - tile * a.ngroups);
+}
+
+// Persistent form for a batch of scans of uneven sizes: every workgroup takes
+// (scan, tile, z-group) items from a device counter until the queue is empty.
+// Item ranges are contiguous per scan (item_begin, a prefix sum), so the
+// scan of an item is found by binary search.  All scans share the launch
+// shape and the LDS layout (vstage, slot), which the host sets to the
+// largest any scan needs.
+template <int NW, int NP, int ZL, int FMT, bool FAST>
+__global__ void __launch_bounds__((NW + NP) * 64)
+batch_kernel(const BatchArgs b)
+{
+	__shared__ long long next_item;
+	for (;;) {
+		if (threadIdx.x == 0)
+			next_item = (long long) atomicAdd(b.counter, 1ull);
+		__syncthreads();
+		// the item is the same in every lane: keep it (and all that derives
+		// from it) in scalar registers
+		const long long it = next_item;
+		const long long item = (long long) (((unsigned long long) (unsigned)
+			__builtin_amdgcn_readfirstlane((int) (it >> 32)) << 32)
+			| (unsigned) __builtin_amdgcn_readfirstlane((int) it));
+		__syncthreads();
+		if (item >= b.nitems)
+			break;
+		int lo = 0, hi = b.nscan - 1;
+		while (lo < hi) {
+			const int mid = (lo + hi + 1) >> 1;
+			if (b.item_begin[mid] <= item)
+				lo = mid;
+			else
+				hi = mid - 1;
+		}
+		const BackArgs &a = b.scans[lo];
+		const int local = (int) (item - b.item_begin[lo]);
+		const int tile = local / a.ngroups;
+		tile_pipeline<NW, NP, ZL, FMT, FAST>(a, tile, local - tile * a.ngroups);
+	}
+}
+
+__global__ void reset_counter_kernel(unsigned long long *c)
+{
+	if (threadIdx.x == 0)
+		*c = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -641,6 +724,25 @@ __global__ void fill_kernel(float *image, long ncol, int nz, int zlo, int zhi)
 		const int r = (int) (i - col * nout);
 		const int z = (r < zlo) ? r : zhi + (r - zlo);
 		image[col * nz + z] = 0.f;
+	}
+}
+
+// zero-padded copy of the views: [na][ns][nt] -> [na][ns+2][nt+2] (the
+// one-sample zero border of fdk_ts_put_view, fdk,ts,t.c:13-21, for all views)
+template <typename T>
+__global__ void pad_kernel(const T *src, T *dst, int ns, int nt, long total)
+{
+	const int pnt = nt + 2, pns = ns + 2;
+	for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < total;
+		i += (long) gridDim.x * blockDim.x) {
+		const int t = (int) (i % pnt);
+		const long q = i / pnt;
+		const int s = (int) (q % pns);
+		const long v = q / pns;
+		T val = T(0);
+		if (s >= 1 && s <= ns && t >= 1 && t <= nt)
+			val = src[(v * ns + (s - 1)) * nt + (t - 1)];
+		dst[i] = val;
 	}
 }
 
@@ -816,12 +918,12 @@ size_t back_lds_bytes(int nw, int vstage, int slot)
 // producer waves per workgroup for a consumer count
 static int producers_for(int nw)
 {
-	return nw >= 8 ? 4 : 2;
+	return nw >= 8 ? 8 : 4;
 }
 
 #define FDK_DISPATCH_FMT(NW, ZL)                                                               \
 	do {                                                                                       \
-		constexpr int NP_ = (NW) >= 8 ? 4 : 2;                                                 \
+		constexpr int NP_ = (NW) >= 8 ? 8 : 4;                                                 \
 		if (fmt == FMT_F32 && !fast) return launch_back_t<NW, NP_, ZL, FMT_F32, false>(a, lds, s); \
 		if (fmt == FMT_F32 && fast) return launch_back_t<NW, NP_, ZL, FMT_F32, true>(a, lds, s);   \
 		if (fmt == FMT_F16) return launch_back_t<NW, NP_, ZL, FMT_F16, true>(a, lds, s);          \
@@ -842,6 +944,74 @@ hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hi
 		if (nw == 4) FDK_DISPATCH_FMT(4, 32);
 	}
 	return hipErrorInvalidValue;
+}
+
+template <int NW, int NP, int ZL, int FMT, bool FAST>
+static hipError_t launch_batch_t(const BatchArgs &b, int nwg, size_t lds_bytes, hipStream_t s)
+{
+	hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, b.counter);
+	hipLaunchKernelGGL((batch_kernel<NW, NP, ZL, FMT, FAST>), dim3(nwg), dim3((NW + NP) * 64),
+		lds_bytes, s, b);
+	return hipGetLastError();
+}
+
+#define FDK_DISPATCH_BATCH(NW, ZL)                                                          \
+	do {                                                                                    \
+		constexpr int NP_ = (NW) >= 8 ? 8 : 4;                                              \
+		if (fmt == FMT_F32 && !fast) return launch_batch_t<NW, NP_, ZL, FMT_F32, false>(b, nwg, lds, s); \
+		if (fmt == FMT_F32 && fast) return launch_batch_t<NW, NP_, ZL, FMT_F32, true>(b, nwg, lds, s);   \
+		if (fmt == FMT_F16) return launch_batch_t<NW, NP_, ZL, FMT_F16, true>(b, nwg, lds, s);          \
+		if (fmt == FMT_BF16) return launch_batch_t<NW, NP_, ZL, FMT_BF16, true>(b, nwg, lds, s);        \
+	} while (0)
+
+hipError_t launch_batch(const BatchArgs &b, int nwg, int nw, int zl, int vstage, int slot,
+	int fmt, bool fast, hipStream_t s)
+{
+	const size_t lds = back_lds_bytes(nw, vstage, slot);
+	if (zl == 16) {
+		if (nw == 4) FDK_DISPATCH_BATCH(4, 16);
+		if (nw == 8) FDK_DISPATCH_BATCH(8, 16);
+	} else if (zl == 8) {
+		if (nw == 4) FDK_DISPATCH_BATCH(4, 8);
+		if (nw == 8) FDK_DISPATCH_BATCH(8, 8);
+	} else if (zl == 32) {
+		if (nw == 4) FDK_DISPATCH_BATCH(4, 32);
+	}
+	return hipErrorInvalidValue;
+}
+
+template <int NW, int NP, int ZL, int FMT, bool FAST>
+static int batch_occupancy_t(size_t lds_bytes)
+{
+	int n = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<NW, NP, ZL, FMT, FAST>,
+			(NW + NP) * 64, lds_bytes) != hipSuccess)
+		return 1;
+	return n > 0 ? n : 1;
+}
+
+#define FDK_OCC_BATCH(NW, ZL)                                                               \
+	do {                                                                                    \
+		constexpr int NP_ = (NW) >= 8 ? 8 : 4;                                              \
+		if (fmt == FMT_F32 && !fast) return batch_occupancy_t<NW, NP_, ZL, FMT_F32, false>(lds); \
+		if (fmt == FMT_F32 && fast) return batch_occupancy_t<NW, NP_, ZL, FMT_F32, true>(lds);   \
+		if (fmt == FMT_F16) return batch_occupancy_t<NW, NP_, ZL, FMT_F16, true>(lds);          \
+		if (fmt == FMT_BF16) return batch_occupancy_t<NW, NP_, ZL, FMT_BF16, true>(lds);        \
+	} while (0)
+
+int batch_blocks_per_cu(int nw, int zl, int vstage, int slot, int fmt, bool fast)
+{
+	const size_t lds = back_lds_bytes(nw, vstage, slot);
+	if (zl == 16) {
+		if (nw == 4) FDK_OCC_BATCH(4, 16);
+		if (nw == 8) FDK_OCC_BATCH(8, 16);
+	} else if (zl == 8) {
+		if (nw == 4) FDK_OCC_BATCH(4, 8);
+		if (nw == 8) FDK_OCC_BATCH(8, 8);
+	} else if (zl == 32) {
+		if (nw == 4) FDK_OCC_BATCH(4, 32);
+	}
+	return 1;
 }
 
 bool back_config_supported(int nw, int zl)
@@ -873,6 +1043,20 @@ hipError_t launch_fill(float *image, long ncol, int nz, int zlo, int zhi, hipStr
 	const long total = ncol * (long) (zlo + (nz - zhi));
 	const int blocks = (int) min((total + 255) / 256, (long) 65536);
 	hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, image, ncol, nz, zlo, zhi);
+	return hipGetLastError();
+}
+
+hipError_t launch_pad(const void *src, void *dst, int ns, int nt, int na, int fmt,
+	hipStream_t s)
+{
+	const long total = (long) na * (ns + 2) * (nt + 2);
+	const int blocks = (int) min((total + 255) / 256, (long) 8192);
+	if (fmt == FMT_F32)
+		hipLaunchKernelGGL(pad_kernel<float>, dim3(blocks), dim3(256), 0, s,
+			(const float *) src, (float *) dst, ns, nt, total);
+	else
+		hipLaunchKernelGGL(pad_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s,
+			(const uint16_t *) src, (uint16_t *) dst, ns, nt, total);
 	return hipGetLastError();
 }
 

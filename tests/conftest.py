@@ -49,6 +49,13 @@ CASES = {
     "arc_tall": dict(nxyz=(24, 24, 200), dxyz=(0.7, -0.7, 0.625), cxyz=(0, 0, 0),
                      dso=541.0, dsd=949.075, dfs=0.0, ns=96, nt=64,
                      dst=(1.0239, 1.0964), ost=(1.25, 0.0), na=40),
+    # a volume wider than the field of view and a steep cone: within one 8x8
+    # tile some columns miss a view while the slice ranges of the others
+    # differ by more than a z-chunk (ranges that miss a chunk must add
+    # nothing)
+    "wide_cone": dict(nxyz=(48, 48, 160), dxyz=(6.0, -6.0, 0.5), cxyz=(0.5, 0, 0),
+                      dso=541.0, dsd=949.0, dfs=np.inf, ns=96, nt=48, dst=(4.0, 1.0),
+                      ost=(0.0, 0.0), na=24),
     # a volume inside the cone everywhere (all slices in support)
     "flat_full": dict(nxyz=(32, 32, 48), dxyz=(0.5, -0.5, 0.5), cxyz=(0, 0, 0),
                       dso=541.0, dsd=949.0, dfs=np.inf, ns=128, nt=96,

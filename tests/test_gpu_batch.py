@@ -1,0 +1,178 @@
+"""GPU: batches of scans in one persistent launch, and the autotuner.
+
+Bars:
+* EXACT FP32 batch: every scan bit-for-bit equal to the oracle's FP32
+  restatement of the reference kernel (which the single-scan tests pin to the
+  reference C) and to a single-scan plan;
+* FAST FP32 / FP16 batch: equal to single-scan FAST plans of the same format
+  within relative L2 1e-6 (same arithmetic, different launch);
+* graph replays with new projection data match direct launches bit for bit;
+* the autotuner leaves results unchanged (EXACT: bit for bit).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import CASES, make_case
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# uneven scans: different geometries, view counts, masks, and one scan whose
+# image misses the field of view entirely
+BATCH = [("flat", 3), ("arc", 4), ("par", 5), ("arc_tall", 6), ("flat_full", 7)]
+
+
+def _geom(g):
+    from mirt_amd._lib import Geom
+    return Geom(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.offset_x, g.offset_y, g.offset_z,
+                g.dso, g.dsd, g.dfs, g.ns, g.nt, g.ds, g.dt, g.offset_s, g.offset_t)
+
+
+def _cases(data="noise"):
+    return [make_case(name, seed=seed, data=data) for name, seed in BATCH]
+
+
+def _off_fov_case():
+    """A small arc scan whose image sits far outside the detector's reach."""
+    g, proj, beta, mask = make_case("arc", seed=9)
+    g = O.OrcGeom.from_buffer_copy(g)
+    g.offset_z = 500.0
+    return g, proj, beta, mask
+
+
+def _batch(cases, fmt="f32", math_mode="exact"):
+    from mirt_amd import FdkBatch
+    return FdkBatch([(_geom(g), beta, mask.T) for g, _, beta, mask in cases],
+                    fmt=fmt, math_mode=math_mode)
+
+
+def _plan(g, beta, mask, fmt="f32", math_mode="exact"):
+    from mirt_amd import FdkPlan
+    return FdkPlan(_geom(g), beta, mask.T, fmt=fmt, math_mode=math_mode)
+
+
+def test_batch_exact_bit_for_bit(device):
+    import torch
+    cases = _cases() + [_off_fov_case()]
+    print(f"batch exact: {len(cases)} scans", flush=True)
+    b = _batch(cases)
+    projs = [torch.from_numpy(p).to(device) for _, p, _, _ in cases]
+    outs = [o.cpu().numpy() for o in b(projs)]
+    for i, (g, proj, beta, mask) in enumerate(cases):
+        want = O.back_ts_f32(g, proj, beta, mask)
+        d = np.abs(outs[i] - want)
+        assert np.array_equal(outs[i], want), \
+            f"scan {i}: max|diff|={d.max():.3e} relL2={O.rel_l2(outs[i], want):.2e}"
+        plan_img = _plan(g, beta, mask)(projs[i]).cpu().numpy()
+        assert np.array_equal(outs[i], plan_img), f"scan {i}: batch != plan"
+    assert not outs[-1].any()   # out of the field of view: zeros
+
+
+def test_batch_count_updates(device):
+    cases = _cases()
+    b = _batch(cases)
+    total = 0
+    for g, _, beta, mask in cases:
+        total += _plan(g, beta, mask).count_updates()
+    assert b.count_updates() == total > 0
+    cfg = b.config
+    assert cfg["items"] > 0 and cfg["workgroups"] > 0
+
+
+@pytest.mark.parametrize("fmt", ["f32", "f16"])
+def test_batch_fast_matches_plans(device, fmt):
+    import torch
+    from mirt_amd import quantize_views
+    cases = _cases()
+    print(f"batch fast {fmt}", flush=True)
+    b = _batch(cases, fmt=fmt, math_mode="fast")
+    projs, scales = [], []
+    for _, p, _, _ in cases:
+        t = torch.from_numpy(p).to(device)
+        if fmt == "f16":
+            q, s = quantize_views(t, "f16")
+            projs.append(q)
+            scales.append(s)
+        else:
+            projs.append(t)
+    outs = b(projs, scales if scales else None)
+    for i, (g, _, beta, mask) in enumerate(cases):
+        plan = _plan(g, beta, mask, fmt, "fast")
+        want = plan(projs[i], scales[i] if scales else None)
+        err = O.rel_l2(outs[i].cpu().numpy(), want.cpu().numpy())
+        assert err <= 1e-6, (i, err)
+
+
+def test_batch_graph_replay_new_data(device):
+    import torch
+    cases = _cases()
+    b = _batch(cases)
+    projs = [torch.from_numpy(p).to(device) for _, p, _, _ in cases]
+    outs = [torch.empty((g.ny, g.nx, g.nz), dtype=torch.float32, device=device)
+            for g, _, _, _ in cases]
+    for rep in range(3):
+        # new data in the same buffers: the replayed graph must see it
+        for t, (g, _, _, _) in zip(projs, cases):
+            t.copy_(torch.from_numpy(
+                np.random.default_rng(100 + rep).standard_normal(t.shape, dtype=np.float32)))
+        b(projs, outs=outs, graph=True)
+        got = [o.cpu().numpy() for o in outs]
+        direct = [o.cpu().numpy() for o in b(projs, graph=False)]
+        for i in range(len(cases)):
+            assert np.array_equal(got[i], direct[i]), (rep, i)
+
+
+@pytest.mark.parametrize("waves,zper,vstage", [(8, 8, 4), (4, 16, 0), (4, 32, 8), (8, 16, 1)])
+def test_batch_launch_shapes(device, waves, zper, vstage):
+    import torch
+    cases = _cases()
+    print(f"batch shape waves={waves} zper={zper} vstage={vstage}", flush=True)
+    b = _batch(cases)
+    b.set_config(waves, zper, vstage)
+    projs = [torch.from_numpy(p).to(device) for _, p, _, _ in cases]
+    outs = [o.cpu().numpy() for o in b(projs)]
+    for i, (g, proj, beta, mask) in enumerate(cases):
+        assert np.array_equal(outs[i], O.back_ts_f32(g, proj, beta, mask)), i
+
+
+def test_plan_autotune_keeps_results(device):
+    import torch
+    g, proj, beta, mask = make_case("arc_tall", seed=11)
+    plan = _plan(g, beta, mask)
+    t = torch.from_numpy(proj).to(device)
+    ms = plan.autotune(t, reps=2)
+    assert ms > 0
+    cfg = plan.config
+    img = plan(t).cpu().numpy()
+    assert np.array_equal(img, O.back_ts_f32(g, proj, beta, mask)), cfg
+    # a new plan of the same geometry starts from the tuned choice
+    again = _plan(g, beta, mask)
+    assert again.config == cfg
+
+
+def test_batch_autotune_keeps_results(device):
+    import torch
+    cases = _cases()
+    b = _batch(cases)
+    projs = [torch.from_numpy(p).to(device) for _, p, _, _ in cases]
+    assert b.autotune(projs, reps=1) > 0
+    outs = [o.cpu().numpy() for o in b(projs)]
+    for i, (g, proj, beta, mask) in enumerate(cases):
+        assert np.array_equal(outs[i], O.back_ts_f32(g, proj, beta, mask)), i
+
+
+def test_batch_argument_errors(device):
+    import torch
+    from mirt_amd import FdkhipError
+    cases = _cases()[:2]
+    b = _batch(cases)
+    projs = [torch.from_numpy(p).to(device) for _, p, _, _ in cases]
+    with pytest.raises(ValueError):
+        b(projs[:1])
+    with pytest.raises(FdkhipError):
+        b.set_config(8, 32, 0)    # shape not built
+    b16 = _batch(cases, fmt="f16", math_mode="fast")
+    with pytest.raises(ValueError):
+        b16([p.half() for p in projs])   # no scales

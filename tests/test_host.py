@@ -77,6 +77,24 @@ def test_cabi_argument_errors_before_device():
     assert rc == _lib.FDKHIP_EARG
 
 
+def test_batch_argument_errors_before_device():
+    import ctypes as C
+    from mirt_amd import _lib
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.fdkhip_batch_create(None, 1, None, None, None, None, 0, 0) == _lib.FDKHIP_EARG
+    assert lib.fdkhip_batch_create(C.byref(h), 0, None, None, None, None, 0, 0) \
+        == _lib.FDKHIP_EARG
+    assert "nscan" in lib.fdkhip_last_error().decode()
+    assert h.value is None
+    assert lib.fdkhip_batch_execute(None, None, None, None, None) == _lib.FDKHIP_EARG
+    assert lib.fdkhip_batch_set_config(None, 8, 16, 0) == _lib.FDKHIP_EARG
+    assert lib.fdkhip_batch_count_updates(None, None) == _lib.FDKHIP_EARG
+    assert lib.fdkhip_batch_destroy(None) == _lib.FDKHIP_OK
+    assert lib.fdkhip_plan_autotune(None, None, None, None, None, 1, None) \
+        == _lib.FDKHIP_EARG
+
+
 def test_quantize_argument_errors():
     from mirt_amd import _lib
     lib = _lib.load()
