@@ -73,7 +73,7 @@ def profiled_traffic(workload, fmt, cfg):
     """HBM bytes per launch of the backprojection kernel from the committed
     rocprofv3 summary for this workload and path (FETCH_SIZE and WRITE_SIZE
     passes, scaled by the copy-kernel calibration), if the summary's launch
-    shape matches the one timed here; else None."""
+    shape (waves, slices per lane) matches the one timed here; else None."""
     path = os.path.join(ROOT, "profiles", f"back_kernel_{workload}_{fmt}.md")
     if not os.path.exists(path):
         return None
@@ -82,7 +82,8 @@ def profiled_traffic(workload, fmt, cfg):
     shape = re.search(r"^launch:\s*waves=(\d+) zper=(\d+) vstage=(\d+)", text, re.M)
     if not m or not shape:
         return None
-    if tuple(int(x) for x in shape.groups()) != (cfg["waves"], cfg["zper"], cfg["vstage"]):
+    waves, zper, _vstage = (int(x) for x in shape.groups())
+    if (waves, zper) != (cfg["waves"], cfg["zper"]):
         return None
     return float(m.group(1))
 
@@ -187,6 +188,11 @@ def run_single(args, dev):
         plan = FdkPlan(geom, beta, mask, fmt=fmt, math_mode=mm)
         a = prepare(tsa, fmt)
         out = torch.empty((ig.ny, ig.nx, ig.nz), dtype=torch.float32, device=dev)
+        if not args.no_autotune:
+            t0 = time.perf_counter()
+            best = plan.autotune(*a, out=out, reps=2)
+            log(f"{name}: autotune {time.perf_counter() - t0:.1f} s -> {plan.config} "
+                f"({best:.3f} ms)")
         paths[name] = time_steps(lambda: plan(*a, out=out), args.steps, args.warmup)
         kernel[name] = plan.profile(*a, out=out, reps=max(2, args.steps))
         images[name] = out
@@ -366,6 +372,11 @@ def main():
     ap.add_argument("--errors-scans", type=int, default=4,
                     help="uneven_scans: scans checked against the FP64 oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB three times (torch copy kernel): the reference "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the plans' default launch shape instead of the tuned one")
     ap.add_argument("--fmt-only", action="store_true",
                     help="time only the headline path (profiling runs)")
     args = ap.parse_args()
@@ -377,6 +388,15 @@ def main():
 
     import torch
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        a = torch.ones(2 ** 28, device=dev)         # 1 GiB of float32
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 4,
+                          "bytes_written": a.numel() * 4, "repeats": 3}), flush=True)
+        return
     if args.workload == "uneven_scans":
         result = run_batch(args, dev)
     else:

@@ -91,6 +91,8 @@ static GeomF narrow_geom(const fdkhip_geom *g, int na)
 	f.wz = (float) ((g->nz - 1) / 2. + oz);
 	f.ws = (float) ((g->ns - 1) / 2. + os);
 	f.wt = (float) ((g->nt - 1) / 2. + ot);
+	f.inv_ds = (float) (1.0 / g->ds);
+	f.dz_dt = (float) (g->dz / g->dt);
 	const float dfs = (float) g->dfs;
 	if (isinf(f.dsd) || isinf(f.dso))
 		f.kind = KIND_PAR;
@@ -237,6 +239,33 @@ static void drop_graph(fdkhip_plan *p)
 // Tiling for (nw, zl), a survey pass over it, then the LDS slot size.
 // vstage_req > 0 fixes the views per stage; slot_cap > 0 caps the slot (floats)
 // -- views whose footprint does not fit gather from HBM instead of LDS.
+// Stage depth (views per LDS stage) and slot size (floats per staged view)
+// within `lds_limit` bytes.  A requested depth is kept and the slot shrunk
+// to fit; otherwise the deepest stage up to 16 views whose slot holds the
+// largest footprint, but never fewer than 4 views per stage while a smaller
+// slot would allow 4: below that the producer waves, not the consumers, set
+// the pace.  Views whose footprint exceeds the slot gather from HBM.
+static bool choose_layout(int nw, int maxarea, int vstage_req, int slot_cap, size_t lds_limit,
+	int &vstage_out, int &slot_out)
+{
+	int slot = (std::max(maxarea, 64) + 63) & ~63;
+	if (slot_cap > 0)
+		slot = std::max(64, std::min(slot, slot_cap & ~63));
+	const int floor_vs = vstage_req > 0 ? vstage_req : 4;
+	int vstage = vstage_req > 0 ? vstage_req : 16;
+	while (vstage > floor_vs && back_lds_bytes(nw, vstage, slot) > lds_limit)
+		--vstage;
+	while (slot > 64 && back_lds_bytes(nw, vstage, slot) > lds_limit)
+		slot -= 64;
+	while (vstage > 1 && back_lds_bytes(nw, vstage, slot) > lds_limit)
+		--vstage;
+	if (back_lds_bytes(nw, vstage, slot) > lds_limit)
+		return false;
+	vstage_out = vstage;
+	slot_out = slot;
+	return true;
+}
+
 static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_cap)
 {
 	if (!back_config_supported(nw, zl))
@@ -257,7 +286,7 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 		HIP_TRY(launch_zero_i32(p->d_maxarea, 1, nullptr));
 		HIP_TRY(launch_zero_i32(p->d_any, 1, nullptr));
 		BackArgs a = make_args(p, nullptr, nullptr, nullptr);
-		HIP_TRY(launch_survey(a, nw, zl, nullptr));
+		HIP_TRY(launch_survey(a, nw, zl, p->fmt != FDKHIP_F32, nullptr));
 		unsigned long long cnt = 0;
 		HIP_TRY(hipMemcpy(&cnt, p->d_count, sizeof cnt, hipMemcpyDeviceToHost));
 		HIP_TRY(hipMemcpy(&p->maxarea, p->d_maxarea, sizeof(int), hipMemcpyDeviceToHost));
@@ -265,23 +294,10 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 		p->n_updates = (long long) cnt;
 	}
 	// views per stage and slot size within the LDS budget: 8 consumer waves
-	// (+4 producers) fill a CU with one workgroup, 4 (+2) fit two
+	// (+8 producers) fill a CU with one workgroup, 4 (+4) fit two
 	p->lds_limit = (nw >= 8) ? p->lds_max : p->lds_max / 2;
-	int slot = std::max(p->maxarea, 64);
-	slot = (slot + 63) & ~63;
-	if (slot_cap > 0)
-		slot = std::max(64, std::min(slot, slot_cap & ~63));
-	int vstage = vstage_req > 0 ? vstage_req : 16;
-	if (vstage_req > 0) {
-		// keep the requested stage depth, give up slot size first
-		while (slot > 64 && back_lds_bytes(nw, vstage, slot) > p->lds_limit)
-			slot -= 64;
-	}
-	while (vstage > 1 && back_lds_bytes(nw, vstage, slot) > p->lds_limit)
-		--vstage;
-	while (slot > 64 && back_lds_bytes(nw, vstage, slot) > p->lds_limit)
-		slot -= 64;
-	if (back_lds_bytes(nw, vstage, slot) > p->lds_limit)
+	int vstage = 0, slot = 0;
+	if (!choose_layout(nw, p->maxarea, vstage_req, slot_cap, p->lds_limit, vstage, slot))
 		return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
 	p->vstage = vstage;
 	p->slot = slot;
@@ -323,6 +339,8 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 		return fail(FDKHIP_EARG, "unknown projection format %d", fmt);
 	if (math != FDKHIP_MATH_EXACT && math != FDKHIP_MATH_FAST)
 		return fail(FDKHIP_EARG, "unknown math mode %d", math);
+	if (g->nz > 65535)
+		return fail(FDKHIP_EUNSUPPORTED, "nz=%d: at most 65535 slices per volume", g->nz);
 
 	fdkhip_plan *p = new fdkhip_plan;
 	p->geom = *g;
@@ -687,7 +705,7 @@ namespace {
 
 // launch shapes (consumer waves, slices per lane) the library is built with
 const int kShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}};
-const int kStages[] = {0, 4, 8, 12, 16};
+const int kStages[] = {0, 3, 4, 6, 8, 12, 16};
 
 struct Tuned {
 	int nw, zl, vstage;
@@ -906,13 +924,8 @@ static int batch_configure(fdkhip_batch *b, int nw, int zl, int vstage_req)
 	}
 	const size_t lds_max = b->plans.empty() ? 65536 : b->plans[0]->lds_max;
 	const size_t lds_limit = (nw >= 8) ? lds_max : lds_max / 2;
-	int slot = (maxarea + 63) & ~63;
-	int vstage = vstage_req > 0 ? vstage_req : 16;
-	while (vstage > 1 && back_lds_bytes(nw, vstage, slot) > lds_limit)
-		--vstage;
-	while (slot > 64 && back_lds_bytes(nw, vstage, slot) > lds_limit)
-		slot -= 64;
-	if (back_lds_bytes(nw, vstage, slot) > lds_limit)
+	int vstage = 0, slot = 0;
+	if (!choose_layout(nw, maxarea, vstage_req, 0, lds_limit, vstage, slot))
 		return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
 	b->nw = nw;
 	b->zl = zl;

@@ -17,6 +17,8 @@ struct GeomF {
 	float dx, dy, dz, dso, dsd, ds, dt;
 	float wx, wy, wz, ws, wt;
 	int kind;
+	// FP16/BF16 paths only (not in the reference): 1/ds and dz/dt
+	float inv_ds, dz_dt;
 };
 
 struct BackArgs {
@@ -56,7 +58,7 @@ struct StArgs {
 size_t back_lds_bytes(int nw, int vstage, int slot);
 bool back_config_supported(int nw, int zl);
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s);
-hipError_t launch_survey(const BackArgs &a, int nw, int zl, hipStream_t s);
+hipError_t launch_survey(const BackArgs &a, int nw, int zl, bool direct, hipStream_t s);
 hipError_t launch_batch(const BatchArgs &b, int nwg, int nw, int zl, int vstage, int slot,
 	int fmt, bool fast, hipStream_t s);
 // resident persistent workgroups per CU for this shape and LDS layout

@@ -131,6 +131,41 @@ __device__ __forceinline__ int column_setup(const GeomF &g, float xx, float yy,
 	return SETUP_ACTIVE;
 }
 
+// Per-column, per-view quantities for the FP16/BF16 paths, which are held to
+// the FP64 result rather than to the reference's bits: reciprocals instead
+// of divisions, float atan2 for the arc detector, FMA.  iz_min/iz_max are
+// only a first guess here; setup_direct settles the slice range exactly.
+__device__ __forceinline__ int column_setup_fast(const GeomF &g, float xx, float yy,
+	float cosb, float sinb, ColSetup &c)
+{
+	const float xbeta = fmaf(xx, cosb, yy * sinb);
+	const float ybetas = g.dso - fmaf(-xx, sinb, yy * cosb);
+	const float rb = __builtin_amdgcn_rcpf(ybetas);
+	const float mag = (g.kind == KIND_PAR) ? 1.f : g.dsd * rb;
+	const float ss = (g.kind == KIND_ARC) ? g.dsd * atan2f(xbeta, ybetas) : mag * xbeta;
+	const float ss_bin = fmaf(ss, g.inv_ds, g.ws);
+	if (!(ss_bin >= -1.f && ss_bin < (float) g.ns))   // also rejects NaN
+		return SETUP_OUT;
+	const float fl = floorf(ss_bin);
+	c.is = (int) fl;
+	c.wr = ss_bin - fl;
+	c.wl = 1.f - c.wr;
+	c.t_inc = mag * g.dz_dt;
+	if (!(c.t_inc > 0.f) || c.t_inc == INFINITY)
+		return SETUP_IDLE;
+	const float rti = __builtin_amdgcn_rcpf(c.t_inc);
+	const float zmin = floorf(fmaf(-(g.wt + 1.f), rti, g.wz)) + 1.f;
+	const float zmax = ceilf(fmaf((float) g.nt - g.wt, rti, g.wz));
+	c.iz_min = (int) fminf(fmaxf(zmin, -1.f), (float) g.nz + 1.f);
+	c.iz_max = (int) fminf(fmaxf(zmax, -1.f), (float) g.nz + 1.f);
+	c.t_bin = fmaf(c.t_inc, (float) c.iz_min - g.wz, g.wt);
+	if (g.kind == KIND_ARC)
+		c.w2 = (g.dsd * g.dsd) * __builtin_amdgcn_rcpf(fmaf(ybetas, ybetas, xbeta * xbeta));
+	else
+		c.w2 = mag * mag;
+	return SETUP_ACTIVE;
+}
+
 // Walk of the reference's t_bin sequence over one z-group [zg0, zg1):
 // t_bin starts at iz_min and grows by repeated addition of t_inc
 // (fdk,ts.c:169,244), so the value at slice z is reproduced only by the same
@@ -208,6 +243,76 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 	return zlo < zhi;
 }
 
+// Slice range of one z-group for the FP16/BF16 paths, in u = t_bin + 1 (the
+// padded detector row coordinate, > 0 for every slice that reads a sample).
+// u at slice z is computed as the consumer does: chunk k (slices
+// zg0 + k*ZL + j) reads u = fmaf(j, t_inc, U[k]), U[k] = u at the chunk's
+// first slice from the closed form u(iz_min) + (z - iz_min) * t_inc (one
+// rounding, nearer the exact value than the reference's running sum).  The
+// range [zlo, zhi) is every slice of the group with 0 < u < nt + 1, found
+// with that same arithmetic; first/last are u at zlo and zhi - 1.
+template <int NW, int ZL>
+__device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
+	int zg0, int zg1, float (&Uk)[NW], int &zlo, int &zhi, float &first, float &last)
+{
+	const float ti = c.t_inc;
+	const float ub = c.t_bin + 1.f;      // u at iz_min
+	const int zb = c.iz_min;
+#pragma unroll
+	for (int k = 0; k < NW; ++k)
+		Uk[k] = fmaf((float) (zg0 + k * ZL - zb), ti, ub);
+	auto uz = [&](int z) {
+		const int zs = zg0 + ((z - zg0) / ZL) * ZL;
+		return fmaf((float) (z - zs), ti, fmaf((float) (zs - zb), ti, ub));
+	};
+	const float uhi = (float) g.nt + 1.f;
+	int lo = max(c.iz_min - 1, zg0), hi = min(c.iz_max + 1, zg1);
+	while (lo < hi && uz(lo) <= 0.f)
+		++lo;
+	while (hi > lo && uz(hi - 1) >= uhi)
+		--hi;
+	zlo = lo;
+	zhi = hi;
+	first = lo < hi ? uz(lo) : 0.f;
+	last = lo < hi ? uz(hi - 1) : 0.f;
+	return lo < hi;
+}
+
+// One column's view over one z-group, in either mode: the per-column
+// quantities, the slice range [zlo, zhi), the vertical coordinate at each
+// chunk's start (t_bin for the reference walk, u = t_bin + 1 for DIRECT) and
+// the padded detector rows [row_lo, row_hi] its bilinear reads touch.
+struct ColRange {
+	ColSetup c;
+	int zlo, zhi;
+	float first, last;     // t (walk) or u (DIRECT) at zlo and zhi - 1
+	int row_lo, row_hi;    // padded rows read
+};
+
+template <int NW, int ZL, bool DIRECT>
+__device__ __forceinline__ bool col_range(const GeomF &g, float xx, float yy, float2 cs,
+	int zg0, int zg1, ColRange &r, float (&Tk)[NW])
+{
+	r.zlo = r.zhi = 0;
+	r.first = r.last = 0.f;
+	if constexpr (DIRECT) {
+		if (column_setup_fast(g, xx, yy, cs.x, cs.y, r.c) != SETUP_ACTIVE)
+			return false;
+		if (!setup_direct<NW, ZL>(g, r.c, zg0, zg1, Tk, r.zlo, r.zhi, r.first, r.last))
+			return false;
+		r.row_lo = (int) r.first;             // u > 0: truncation is floor
+		r.row_hi = (int) r.last + 1;
+	} else {
+		if (column_setup(g, xx, yy, cs.x, cs.y, r.c) != SETUP_ACTIVE)
+			return false;
+		if (!setup_walk<NW, ZL>(g, r.c, zg0, zg1, Tk, r.zlo, r.zhi, r.first, r.last))
+			return false;
+		r.row_lo = (int) floorf(r.first) + 1;
+		r.row_hi = (int) floorf(r.last) + 2;
+	}
+	return true;
+}
+
 // Wave-wide reductions (wave64, every lane active) with DPP: quad swaps,
 // row half-mirror and mirror give each row of 16 its result, row_bcast:15
 // and :31 carry it across rows; lane 63 holds the wave's result.
@@ -250,27 +355,23 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks)
 extern __shared__ float fdk_lds[];
 
 // LDS layout of one stage buffer (4-byte words); the kernel holds two.
-//   slots   [vstage][slot]       staged footprints (FP32)
-//   P_cb    [vstage][64]  int    LDS word index of sample (s = is, t = 0)
-//   P_wl, P_wr, P_w2, P_tinc [vstage][64] float
-//   P_zlo, P_zhi [vstage][64] int    effective slice range [zlo, zhi)
-//   P_is    [vstage][64]  int
-//   P_T     [vstage][NW][64] float   t_bin at each consumer wave's first slice
-//   V_info  [vstage][4]   int    s_lo, t_lo, tw, (mode << 16) | s-width
+//   slots [vstage][slot]        staged footprints
+//   P     [vstage][64][8]       per (view, column) record, two 16-byte halves
+//                               read with one ds_read_b128 each:
+//           P0 = {cb, wl, wr, w2 | u_first}   cb: LDS word of the lane's left
+//                                             column at row 1 (walk) / padded
+//                                             row 0 (DIRECT)
+//           P1 = {t_inc, zlo | zhi << 16, is, u_last}
+//   T     [vstage][NW][64]      t_bin (DIRECT: u) at each consumer wave's
+//                               first slice
+//   vinfo [vstage][4]           s_lo, t_lo, tw, (mode << 16) | s-width
 template <int NW>
 struct Lds {
-	int cb, wl, wr, w2, tinc, zlo, zhi, is, T, vinfo, total;
+	int P, T, vinfo, total;
 	__device__ __forceinline__ Lds(int vstage, int slot)
 	{
-		cb = vstage * slot;
-		wl = cb + vstage * 64;
-		wr = wl + vstage * 64;
-		w2 = wr + vstage * 64;
-		tinc = w2 + vstage * 64;
-		zlo = tinc + vstage * 64;
-		zhi = zlo + vstage * 64;
-		is = zhi + vstage * 64;
-		T = is + vstage * 64;
+		P = vstage * slot;
+		T = P + vstage * 64 * 8;
 		vinfo = T + vstage * NW * 64;
 		total = vinfo + vstage * 4;
 	}
@@ -304,33 +405,29 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 	int v0, int nv, int pw, int lane, bool colvalid, float xx, float yy, int zg0, int zg1)
 {
 	const GeomF &g = a.g;
-	float *buf = fdk_lds + bo;
-	int *bufi = reinterpret_cast<int *>(fdk_lds) + bo;
+	constexpr bool DIRECT = FMT != FMT_F32;
 	for (int vv = pw; vv < nv; vv += NP) {
 		const int v = v0 + vv;
-		const float2 cs = a.trig[v];
-		ColSetup c;
-		bool act = colvalid && column_setup(g, xx, yy, cs.x, cs.y, c) == SETUP_ACTIVE;
-		int zlo = 0, zhi = 0;
-		float t_first = 0.f, t_last = 0.f;
+		ColRange cr;
 		float Tk[NW];
-		if (act)
-			act = setup_walk<NW, ZL>(g, c, zg0, zg1, Tk, zlo, zhi, t_first, t_last);
+		const bool act = colvalid
+			&& col_range<NW, ZL, DIRECT>(g, xx, yy, a.trig[v], zg0, zg1, cr, Tk);
+		const ColSetup &c = cr.c;
 		// footprint of this view over the tile and z-group (padded coords),
 		// kept inside the padded view [0, ns+1] x [0, nt+1]
 		const int s_lo = max(wave_min(act ? c.is + 1 : 0x7fffffff), 0);
 		const int s_hi = min(wave_max(act ? c.is + 2 : (int) 0x80000000), g.ns + 1);
-		const int t_lo = max(wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff), 0);
-		const int t_hi = min(wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000),
-			g.nt + 1);
+		const int t_lo = max(wave_min(act ? cr.row_lo : 0x7fffffff), 0);
+		const int t_hi = min(wave_max(act ? cr.row_hi : (int) 0x80000000), g.nt + 1);
 		int mode = MODE_NONE, tw = 0, sw = 0;
 		if (s_lo <= s_hi && t_lo <= t_hi) {
 			tw = t_hi - t_lo + 1;
 			sw = s_hi - s_lo + 1;
 			mode = ((long) sw * tw <= a.slot) ? MODE_LDS : MODE_GLOBAL;
 		}
-		const int idx = vv * 64 + lane;
-		bufi[L.cb + idx] = act ? vv * a.slot + (c.is + 1 - s_lo) * tw + (1 - t_lo) : 0;
+		// LDS word of padded row 0 (DIRECT) or of row 1, t = 0 (walk) in the
+		// lane's left column
+		const int cb = act ? vv * a.slot + (c.is + 1 - s_lo) * tw + (DIRECT ? 0 : 1) - t_lo : 0;
 		float wl = act ? c.wl : 0.f, wr = act ? c.wr : 0.f;
 		const float w2 = act ? c.w2 : 0.f;
 		if (FAST) {
@@ -338,26 +435,31 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 			wl = w2 * wl * sc;
 			wr = w2 * wr * sc;
 		}
-		buf[L.wl + idx] = wl;
-		buf[L.wr + idx] = wr;
-		buf[L.w2 + idx] = w2;
-		buf[L.tinc + idx] = act ? c.t_inc : 0.f;
-		bufi[L.zlo + idx] = zlo;
-		bufi[L.zhi + idx] = act ? zhi : zlo;
-		bufi[L.is + idx] = act ? c.is : 0;
-		// t_bin at the first active slice of each chunk; chunks the lane's
-		// range misses get t_first, so that their (discarded) reads stay
-		// inside the staged footprint
+		const int zhi_st = act ? cr.zhi : cr.zlo;
+		float4 *P4 = reinterpret_cast<float4 *>(fdk_lds + bo + L.P) + (vv * 64 + lane) * 2;
+		P4[0] = make_float4(__int_as_float(cb), wl, wr, DIRECT ? cr.first : w2);
+		P4[1] = make_float4(act ? c.t_inc : 0.f, __int_as_float(cr.zlo | (zhi_st << 16)),
+			__int_as_float(act ? c.is : 0), cr.last);
+		// vertical coordinate at each chunk's first active slice.  Walk: chunks
+		// the lane's range misses get t_first, so that their (discarded)
+		// reads stay inside the staged footprint.  DIRECT: u at the chunk's
+		// first slice; the consumer clamps u to [first, last].
 #pragma unroll
 		for (int k = 0; k < NW; ++k) {
-			const bool meets = zlo < zg0 + (k + 1) * ZL && zg0 + k * ZL < zhi;
-			buf[L.T + (vv * NW + k) * 64 + lane] = act ? (meets ? Tk[k] : t_first) : 0.f;
+			const bool meets = cr.zlo < zg0 + (k + 1) * ZL && zg0 + k * ZL < cr.zhi;
+			float tk;
+			if constexpr (DIRECT)
+				tk = act ? Tk[k] : 0.f;
+			else
+				tk = act ? (meets ? Tk[k] : cr.first) : 0.f;
+			fdk_lds[bo + L.T + (vv * NW + k) * 64 + lane] = tk;
 		}
 		if (lane == 0) {
-			bufi[L.vinfo + vv * 4 + 0] = s_lo;
-			bufi[L.vinfo + vv * 4 + 1] = t_lo;
-			bufi[L.vinfo + vv * 4 + 2] = tw;
-			bufi[L.vinfo + vv * 4 + 3] = (mode << 16) | (sw & 0xffff);
+			int *vi = reinterpret_cast<int *>(fdk_lds) + bo + L.vinfo + vv * 4;
+			vi[0] = s_lo;
+			vi[1] = t_lo;
+			vi[2] = tw;
+			vi[3] = (mode << 16) | (sw & 0xffff);
 		}
 		if (mode != MODE_LDS)
 			continue;
@@ -435,89 +537,161 @@ __device__ __forceinline__ float slice_update(float acc, float t, int c1, int c2
 	}
 }
 
+// The same update for the FP16/BF16 paths in u = t_bin + 1 (> 0): the
+// padded row is the truncation of u, the vertical weight its fraction; c1/c2
+// address padded row 0 of the two staged columns; w2 and the view scale are
+// folded into wl/wr.
+template <int FMT>
+__device__ __forceinline__ float slice_update_u(float acc, float u, int c1, int c2,
+	float wl, float wr)
+{
+	const int iu = (int) u;
+	const float wu = __builtin_amdgcn_fractf(u);
+	const float wd = 1.f - wu;
+	const float a0 = lds_sample<FMT>(c1 + iu), a1 = lds_sample<FMT>(c1 + iu + 1);
+	const float b0 = lds_sample<FMT>(c2 + iu), b1 = lds_sample<FMT>(c2 + iu + 1);
+	// fmaf(w, x, 0) keeps the FP16 -> FP32 conversion inside v_fma_mix
+	const float q1 = fmaf(wr, b0, fmaf(wl, a0, 0.f));
+	const float q2 = fmaf(wr, b1, fmaf(wl, a1, 0.f));
+	acc = fmaf(wd, q1, acc);
+	return fmaf(wu, q2, acc);
+}
+
 // Consumer side: consumer wave `wave` adds the views of the stage buffer at
 // `bo` to its ZL register accumulators (slices [zc, zc + ZL) of its column).
+// The next view's record is read while the current one is added, so the
+// per-view LDS round trips overlap the slice arithmetic.
 template <int NW, int ZL, int FMT, bool FAST>
 __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW> &L,
 	int v0, int nv, int wave, int lane, int zc, float (&acc)[ZL])
 {
+	constexpr bool DIRECT = FMT != FMT_F32;
 	const GeomF &g = a.g;
-	const float *buf = fdk_lds + bo;
-	const int *bufi = reinterpret_cast<const int *>(fdk_lds) + bo;
-	const long view_elems = (long) g.ns * g.nt;
+	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
+	const float *Tw = fdk_lds + bo + L.T + wave * 64 + lane;
+	const int2 *VI = reinterpret_cast<const int2 *>(fdk_lds + bo + L.vinfo + 2);
+	float4 np0 = P4[0], np1 = P4[1];
+	float nt0 = Tw[0];
+	int2 nvi = VI[0];
 	for (int vv = 0; vv < nv; ++vv) {
-		const int mode = bufi[L.vinfo + vv * 4 + 3] >> 16;
+		const float4 p0 = np0, p1 = np1;
+		const float t0 = nt0;
+		const int2 vi = nvi;
+		if (vv + 1 < nv) {
+			np0 = P4[(vv + 1) * 128];
+			np1 = P4[(vv + 1) * 128 + 1];
+			nt0 = Tw[(vv + 1) * NW * 64];
+			nvi = VI[(vv + 1) * 2];
+		}
+		const int mode = __builtin_amdgcn_readfirstlane(vi.y) >> 16;
 		if (mode == MODE_NONE)
 			continue;
-		const int idx = vv * 64 + lane;
-		const int zlo = bufi[L.zlo + idx];
-		const int zhi = bufi[L.zhi + idx];
+		const int zlohi = __float_as_int(p1.y);
+		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
 		const int jlo = max(zlo - zc, 0);
 		const int jhi = min(zhi - zc, ZL);
 		const bool any = jlo < jhi;
 		if (!__any(any))
 			continue;                  // no lane of this wave meets the view
-		const float t0 = buf[L.T + (vv * NW + wave) * 64 + lane];
-		const float ti = buf[L.tinc + idx];
-		const float wl = buf[L.wl + idx];
-		const float wr = buf[L.wr + idx];
-		const float w2 = buf[L.w2 + idx];
+		const int cb = __float_as_int(p0.x);
+		const float wl = p0.y, wr = p0.z, w2 = p0.w;
+		const float ti = p1.x;
 		if (mode == MODE_LDS) {
-			const int tw = bufi[L.vinfo + vv * 4 + 2];
-			const int c1 = bo + bufi[L.cb + idx];
+			const int tw = __builtin_amdgcn_readfirstlane(vi.x);
+			const int c1 = bo + cb;
 			const int c2 = c1 + tw;
-			float t = t0;
 			if (__all(jlo == 0 && jhi == ZL)) {
 				// every lane covers the whole chunk: straight-line slices
+				if constexpr (DIRECT) {
 #pragma unroll
-				for (int j = 0; j < ZL; ++j) {
-					acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
-					t += ti;
+					for (int j = 0; j < ZL; ++j)
+						acc[j] = slice_update_u<FMT>(acc[j], fmaf((float) j, ti, t0), c1, c2,
+							wl, wr);
+				} else {
+					float t = t0;
+#pragma unroll
+					for (int j = 0; j < ZL; ++j) {
+						acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
+						t += ti;
+					}
 				}
 			} else {
-				// partial chunk: slices outside [jlo, jhi) read at t0 (inside
-				// the staged footprint) and are discarded by a select; a lane
-				// whose range misses this chunk (jhi <= jlo) adds nothing
+				// partial chunk: slices outside [jlo, jhi) read inside the
+				// staged footprint and are discarded by a select; a lane whose
+				// range misses this chunk (jhi <= jlo) adds nothing
 				const unsigned nj = (unsigned) max(jhi - jlo, 0);
+				if constexpr (DIRECT) {
+					// u clamped into the lane's [u_first, u_last]
+					const float uf = w2, ul = p1.w;
 #pragma unroll
-				for (int j = 0; j < ZL; ++j) {
-					const bool r = (unsigned) (j - jlo) < nj;
-					const float tv = r ? t : t0;
-					const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
-					acc[j] = r ? up : acc[j];
-					t = r ? t + ti : t;
+					for (int j = 0; j < ZL; ++j) {
+						const bool r = (unsigned) (j - jlo) < nj;
+						const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), uf, ul);
+						const float up = slice_update_u<FMT>(acc[j], u, c1, c2, wl, wr);
+						acc[j] = r ? up : acc[j];
+					}
+				} else {
+					// at t0, the lane's first active slice of this chunk
+					float t = t0;
+#pragma unroll
+					for (int j = 0; j < ZL; ++j) {
+						const bool r = (unsigned) (j - jlo) < nj;
+						const float tv = r ? t : t0;
+						const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
+						acc[j] = r ? up : acc[j];
+						t = r ? t + ti : t;
+					}
 				}
 			}
 		} else if (any) {
 			// footprint larger than the LDS slot: gather from the padded copy
-			const int v = v0 + vv;
-			const int is = bufi[L.is + idx];
+			const int is = __float_as_int(p1.z);
 			const int pnt = g.nt + 2;
-			const size_t vb = (size_t) v * (g.ns + 2) * pnt;
-			const int c1 = (is + 1) * pnt + 1, c2 = c1 + pnt;   // (s = is, t = 0), (is+1, 0)
-			float t = t0;
+			const size_t col = ((size_t) (v0 + vv) * (g.ns + 2) + is + 1) * pnt;
+			if constexpr (DIRECT) {
+				const size_t c1 = col, c2 = col + pnt;     // padded row 0
 #pragma unroll
-			for (int j = 0; j < ZL; ++j) {
-				if (j >= jlo && j < jhi) {
-					const int it = (int) floorf(t);
-					const float wu = t - (float) it;
-					const float wd = 1.f - wu;
-					const float a0 = load_sample<FMT>(a.padded, vb + c1 + it);
-					const float a1 = load_sample<FMT>(a.padded, vb + c1 + it + 1);
-					const float b0 = load_sample<FMT>(a.padded, vb + c2 + it);
-					const float b1 = load_sample<FMT>(a.padded, vb + c2 + it + 1);
-					if (FAST) {
-						const float q1 = fmaf(wr, b0, wl * a0);
-						const float q2 = fmaf(wr, b1, wl * a1);
-						acc[j] = fmaf(wd, q1, acc[j]);
+				for (int j = 0; j < ZL; ++j) {
+					if (j >= jlo && j < jhi) {
+						const float u = fmaf((float) j, ti, t0);
+						const int iu = (int) u;
+						const float wu = __builtin_amdgcn_fractf(u);
+						const float a0 = load_sample<FMT>(a.padded, c1 + iu);
+						const float a1 = load_sample<FMT>(a.padded, c1 + iu + 1);
+						const float b0 = load_sample<FMT>(a.padded, c2 + iu);
+						const float b1 = load_sample<FMT>(a.padded, c2 + iu + 1);
+						const float q1 = fmaf(wr, b0, fmaf(wl, a0, 0.f));
+						const float q2 = fmaf(wr, b1, fmaf(wl, a1, 0.f));
+						acc[j] = fmaf(1.f - wu, q1, acc[j]);
 						acc[j] = fmaf(wu, q2, acc[j]);
-					} else {
-						const float q1 = wl * a0 + wr * b0;
-						const float q2 = wl * a1 + wr * b1;
-						const float p0 = wd * q1 + wu * q2;
-						acc[j] += w2 * p0;
 					}
-					t += ti;
+				}
+			} else {
+				const size_t c1 = col + 1, c2 = c1 + pnt;  // (s = is, t = 0), (is+1, 0)
+				float t = t0;
+#pragma unroll
+				for (int j = 0; j < ZL; ++j) {
+					if (j >= jlo && j < jhi) {
+						const int it = (int) floorf(t);
+						const float wu = t - (float) it;
+						const float wd = 1.f - wu;
+						const float a0 = load_sample<FMT>(a.padded, c1 + it);
+						const float a1 = load_sample<FMT>(a.padded, c1 + it + 1);
+						const float b0 = load_sample<FMT>(a.padded, c2 + it);
+						const float b1 = load_sample<FMT>(a.padded, c2 + it + 1);
+						if (FAST) {
+							const float q1 = fmaf(wr, b0, wl * a0);
+							const float q2 = fmaf(wr, b1, wl * a1);
+							acc[j] = fmaf(wd, q1, acc[j]);
+							acc[j] = fmaf(wu, q2, acc[j]);
+						} else {
+							const float q1 = wl * a0 + wr * b0;
+							const float q2 = wl * a1 + wr * b1;
+							const float p0v = wd * q1 + wu * q2;
+							acc[j] += w2 * p0v;
+						}
+						t += ti;
+					}
 				}
 			}
 		}
@@ -633,7 +807,17 @@ batch_kernel(const BatchArgs b)
 			else
 				hi = mid - 1;
 		}
-		const BackArgs &a = b.scans[lo];
+		// the scan table is read-only for the kernel's lifetime: read it
+		// through the constant address space so that it lands in SGPRs
+		BackArgs a;
+		{
+			constexpr int nw32 = sizeof(BackArgs) / 4;
+			const auto *src = (const __attribute__((address_space(4))) int *) (b.scans + lo);
+			int *dst = reinterpret_cast<int *>(&a);
+#pragma unroll
+			for (int w = 0; w < nw32; ++w)
+				dst[w] = src[w];
+		}
 		const int local = (int) (item - b.item_begin[lo]);
 		const int tile = local / a.ngroups;
 		tile_pipeline<NW, NP, ZL, FMT, FAST>(a, tile, local - tile * a.ngroups);
@@ -651,7 +835,7 @@ __global__ void reset_counter_kernel(unsigned long long *c)
 // reference loop's (voxel, view) iterations, the largest staged footprint and
 // whether any column meets any view (for the dz/dt < 0 error).
 
-template <int NW, int ZL>
+template <int NW, int ZL, bool DIRECT>
 __global__ void __launch_bounds__(64)
 survey_kernel(const BackArgs a)
 {
@@ -679,21 +863,19 @@ survey_kernel(const BackArgs a)
 		ColSetup c;
 		const int code = colvalid ? column_setup(g, xx, yy, cs.x, cs.y, c) : SETUP_OUT;
 		any |= (code != SETUP_OUT) ? 1 : 0;
-		bool act = code == SETUP_ACTIVE;
 		// reference loop iterations of this column-view (whole range, once)
-		if (act && group == 0)
+		if (code == SETUP_ACTIVE && group == 0)
 			cnt += (unsigned long long) max(c.iz_max - c.iz_min, 0);
-		int zlo = 0, zhi = 0;
-		float t_first = 0.f, t_last = 0.f;
+		// the footprint exactly as the backprojection kernel stages it
+		ColRange cr;
 		float Tk[NW];
-		if (act)
-			act = setup_walk<NW, ZL>(g, c, zg0, zg1, Tk, zlo, zhi, t_first, t_last);
-		const int s_lo = wave_min(act ? c.is + 1 : 0x7fffffff);
-		const int s_hi = wave_max(act ? c.is + 2 : (int) 0x80000000);
-		const int t_lo = wave_min(act ? (int) floorf(t_first) + 1 : 0x7fffffff);
-		const int t_hi = min(wave_max(act ? (int) floorf(t_last) + 2 : (int) 0x80000000),
-			g.nt + 1);
-		if (s_lo <= s_hi) {
+		const bool act = colvalid
+			&& col_range<NW, ZL, DIRECT>(g, xx, yy, cs, zg0, zg1, cr, Tk);
+		const int s_lo = max(wave_min(act ? cr.c.is + 1 : 0x7fffffff), 0);
+		const int s_hi = min(wave_max(act ? cr.c.is + 2 : (int) 0x80000000), g.ns + 1);
+		const int t_lo = max(wave_min(act ? cr.row_lo : 0x7fffffff), 0);
+		const int t_hi = min(wave_max(act ? cr.row_hi : (int) 0x80000000), g.nt + 1);
+		if (s_lo <= s_hi && t_lo <= t_hi) {
 			const long area = (long) (s_hi - s_lo + 1) * (t_hi - t_lo + 1);
 			maxarea = max(maxarea, (int) min(area, (long) 0x7fffffff));
 		}
@@ -1020,12 +1202,15 @@ bool back_config_supported(int nw, int zl)
 		|| (zl == 32 && nw == 4);
 }
 
-hipError_t launch_survey(const BackArgs &a, int nw, int zl, hipStream_t s)
+hipError_t launch_survey(const BackArgs &a, int nw, int zl, bool direct, hipStream_t s)
 {
 	const int nblocks = a.ntiles * a.ngroups;
 #define FDK_SURVEY(NW, ZL) \
 	if (nw == NW && zl == ZL) { \
-		hipLaunchKernelGGL((survey_kernel<NW, ZL>), dim3(nblocks), dim3(64), 0, s, a); \
+		if (direct) \
+			hipLaunchKernelGGL((survey_kernel<NW, ZL, true>), dim3(nblocks), dim3(64), 0, s, a); \
+		else \
+			hipLaunchKernelGGL((survey_kernel<NW, ZL, false>), dim3(nblocks), dim3(64), 0, s, a); \
 		return hipGetLastError(); }
 	FDK_SURVEY(4, 16)
 	FDK_SURVEY(8, 16)

@@ -8,11 +8,10 @@ rest of the volume is held to size-independent properties:
   restatement of the reference kernel (whole volume at ``cpu_small``, also
   against the reference C when oracle/_ref is built);
 * FP16 storage with per-view scale: relative L2 <= 1e-3 against the FP64
-  oracle on the sampled columns for the workload's FDK-filtered phantom
-  views, and <= 1e-3 against the FP32 path over the whole volume for white
-  noise views (where the FP32 reference itself departs from FP64 by more
-  than 1e-3 at the largest size: its t_bin recurrence drifts over 512
-  slices and noise has no smoothness to hide it);
+  oracle on the sampled columns, for white-noise views and for the
+  workload's FDK-filtered phantom views (the FP32 reference itself departs
+  from FP64 by 2.3e-3 on noise at the largest size: its running t_bin sum
+  drifts over 512 slices; the FP16 path evaluates t_bin in closed form);
 * homogeneity: back(2 p) == 2 back(p) bit for bit over the whole volume
   (scaling by two is exact in every operation of the kernel);
 * additivity: back(p + q) vs back(p) + back(q) within relative L2 1e-5
@@ -86,13 +85,14 @@ def test_fullsize_parity(device, name):
     img2 = exact(tsa * 2)
     assert torch.equal(img2, img * 2)
 
-    # FP16 storage alone: against the FP32 FAST path on the same views, whole
-    # volume (white noise is the hardest case for the storage format)
-    fast = FdkPlan(geom, beta, None, fmt="f32", math_mode="fast")(tsa)
+    # FP16 storage on white noise views (the hardest case for the format)
+    # against the FP64 oracle on the sampled columns
     q, s = quantize_views(tsa, "f16")
-    img16 = FdkPlan(geom, beta, None, fmt="f16")(q, s)
-    err = float((img16.double() - fast.double()).norm() / fast.double().norm())
-    print(f"  f16 vs f32 fast (noise views, whole volume): {err:.2e}", flush=True)
+    img16 = FdkPlan(geom, beta, None, fmt="f16")(q, s).cpu().numpy().reshape(-1, ig.nz)
+    r64 = O.back_ts_f64(g, host, beta, mask, cols=cols)
+    err = O.rel_l2(img16[cols], r64)
+    print(f"  f16 vs fp64 (noise views): {err:.2e}; reference f32 vs fp64: "
+          f"{O.rel_l2(want, r64):.2e}", flush=True)
     assert err <= 1e-3, err
 
 
