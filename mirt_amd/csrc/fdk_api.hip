@@ -181,7 +181,7 @@ struct fdkhip_plan {
 	unsigned long long *d_count = nullptr;
 	int *d_maxarea = nullptr;
 	int *d_any = nullptr;
-	void *d_padded = nullptr;     // [na][ns+2][nt+2] zero-padded copy of the views
+	void *d_padded = nullptr;     // zero-padded copy of the views (see launch_pad)
 	size_t padded_bytes = 0;
 	// launch configuration
 	int nw = 8, zl = 16, vstage = 8, slot = 0;
@@ -245,21 +245,21 @@ static void drop_graph(fdkhip_plan *p)
 // largest footprint, but never fewer than 4 views per stage while a smaller
 // slot would allow 4: below that the producer waves, not the consumers, set
 // the pace.  Views whose footprint exceeds the slot gather from HBM.
-static bool choose_layout(int nw, int maxarea, int vstage_req, int slot_cap, size_t lds_limit,
-	int &vstage_out, int &slot_out)
+static bool choose_layout(int nw, int fmt, int maxarea, int vstage_req, int slot_cap,
+	size_t lds_limit, int &vstage_out, int &slot_out)
 {
 	int slot = (std::max(maxarea, 64) + 63) & ~63;
 	if (slot_cap > 0)
 		slot = std::max(64, std::min(slot, slot_cap & ~63));
 	const int floor_vs = vstage_req > 0 ? vstage_req : 4;
 	int vstage = vstage_req > 0 ? vstage_req : 16;
-	while (vstage > floor_vs && back_lds_bytes(nw, vstage, slot) > lds_limit)
+	while (vstage > floor_vs && back_lds_bytes(nw, vstage, slot, fmt) > lds_limit)
 		--vstage;
-	while (slot > 64 && back_lds_bytes(nw, vstage, slot) > lds_limit)
+	while (slot > 64 && back_lds_bytes(nw, vstage, slot, fmt) > lds_limit)
 		slot -= 64;
-	while (vstage > 1 && back_lds_bytes(nw, vstage, slot) > lds_limit)
+	while (vstage > 1 && back_lds_bytes(nw, vstage, slot, fmt) > lds_limit)
 		--vstage;
-	if (back_lds_bytes(nw, vstage, slot) > lds_limit)
+	if (back_lds_bytes(nw, vstage, slot, fmt) > lds_limit)
 		return false;
 	vstage_out = vstage;
 	slot_out = slot;
@@ -297,7 +297,7 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 	// (+8 producers) fill a CU with one workgroup, 4 (+4) fit two
 	p->lds_limit = (nw >= 8) ? p->lds_max : p->lds_max / 2;
 	int vstage = 0, slot = 0;
-	if (!choose_layout(nw, p->maxarea, vstage_req, slot_cap, p->lds_limit, vstage, slot))
+	if (!choose_layout(nw, p->fmt, p->maxarea, vstage_req, slot_cap, p->lds_limit, vstage, slot))
 		return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
 	p->vstage = vstage;
 	p->slot = slot;
@@ -381,8 +381,10 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	PLAN_TRY(hipMalloc(&p->d_maxarea, sizeof(int)));
 	PLAN_TRY(hipMalloc(&p->d_any, sizeof(int)));
 	{
-		const size_t esz = (fmt == FDKHIP_F32) ? 4 : 2;
-		p->padded_bytes = (size_t) na * (g->ns + 2) * (g->nt + 2) * esz;
+		// FP32: [na][ns+2][nt+2] floats; FP16/BF16: [na][ns+1][nt+2] column pairs
+		p->padded_bytes = (fmt == FDKHIP_F32)
+			? (size_t) na * (g->ns + 2) * (g->nt + 2) * sizeof(float)
+			: (size_t) na * (g->ns + 1) * (g->nt + 2) * sizeof(uint32_t);
 		if (p->padded_bytes > 0)
 			PLAN_TRY(hipMalloc(&p->d_padded, p->padded_bytes));
 	}
@@ -925,7 +927,7 @@ static int batch_configure(fdkhip_batch *b, int nw, int zl, int vstage_req)
 	const size_t lds_max = b->plans.empty() ? 65536 : b->plans[0]->lds_max;
 	const size_t lds_limit = (nw >= 8) ? lds_max : lds_max / 2;
 	int vstage = 0, slot = 0;
-	if (!choose_layout(nw, maxarea, vstage_req, 0, lds_limit, vstage, slot))
+	if (!choose_layout(nw, b->fmt, maxarea, vstage_req, 0, lds_limit, vstage, slot))
 		return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
 	b->nw = nw;
 	b->zl = zl;

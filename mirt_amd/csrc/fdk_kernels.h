@@ -55,7 +55,7 @@ struct StArgs {
 	float *image;            // [nz][ny][nx]
 };
 
-size_t back_lds_bytes(int nw, int vstage, int slot);
+size_t back_lds_bytes(int nw, int vstage, int slot, int fmt);
 bool back_config_supported(int nw, int zl);
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s);
 hipError_t launch_survey(const BackArgs &a, int nw, int zl, bool direct, hipStream_t s);

@@ -245,25 +245,24 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 
 // Slice range of one z-group for the FP16/BF16 paths, in u = t_bin + 1 (the
 // padded detector row coordinate, > 0 for every slice that reads a sample).
-// u at slice z is computed as the consumer does: chunk k (slices
-// zg0 + k*ZL + j) reads u = fmaf(j, t_inc, U[k]), U[k] = u at the chunk's
-// first slice from the closed form u(iz_min) + (z - iz_min) * t_inc (one
-// rounding, nearer the exact value than the reference's running sum).  The
-// range [zlo, zhi) is every slice of the group with 0 < u < nt + 1, found
-// with that same arithmetic; first/last are u at zlo and zhi - 1.
+// u is evaluated in closed form, as the consumer evaluates it:
+//   U0   = u(iz_min) + (zg0 - iz_min) * t_inc          (the group's first slice)
+//   U[k] = U0 + k * ZL * t_inc                          (chunk k's first slice)
+//   u    = U[k] + j * t_inc                             (slice zg0 + k*ZL + j)
+// each step one fmaf (one rounding each; nearer the exact value than the
+// reference's running sum).  The range [zlo, zhi) is every slice of the
+// group with 0 < u < nt + 1, found with that same arithmetic; first/last
+// are u at zlo and zhi - 1.  Uk[0] receives U0.
 template <int NW, int ZL>
 __device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
 	int zg0, int zg1, float (&Uk)[NW], int &zlo, int &zhi, float &first, float &last)
 {
 	const float ti = c.t_inc;
-	const float ub = c.t_bin + 1.f;      // u at iz_min
-	const int zb = c.iz_min;
-#pragma unroll
-	for (int k = 0; k < NW; ++k)
-		Uk[k] = fmaf((float) (zg0 + k * ZL - zb), ti, ub);
+	const float U0 = fmaf((float) (zg0 - c.iz_min), ti, c.t_bin + 1.f);
+	Uk[0] = U0;
 	auto uz = [&](int z) {
-		const int zs = zg0 + ((z - zg0) / ZL) * ZL;
-		return fmaf((float) (z - zs), ti, fmaf((float) (zs - zb), ti, ub));
+		const int j0 = ((z - zg0) / ZL) * ZL;
+		return fmaf((float) (z - zg0 - j0), ti, fmaf((float) j0, ti, U0));
 	};
 	const float uhi = (float) g.nt + 1.f;
 	int lo = max(c.iz_min - 1, zg0), hi = min(c.iz_max + 1, zg1);
@@ -360,19 +359,20 @@ extern __shared__ float fdk_lds[];
 //                               read with one ds_read_b128 each:
 //           P0 = {cb, wl, wr, w2 | u_first}   cb: LDS word of the lane's left
 //                                             column at row 1 (walk) / padded
-//                                             row 0 (DIRECT)
-//           P1 = {t_inc, zlo | zhi << 16, is, u_last}
-//   T     [vstage][NW][64]      t_bin (DIRECT: u) at each consumer wave's
-//                               first slice
-//   vinfo [vstage][4]           s_lo, t_lo, tw, (mode << 16) | s-width
-template <int NW>
+//                                             row 0 (DIRECT); for views
+//                                             gathered from HBM: is
+//           P1 = {t_inc, zlo | zhi << 16, is | U0, u_last}
+//   T     [vstage][NW][64]      walk only: t_bin at each consumer wave's first
+//                               active slice (DIRECT evaluates it from U0)
+//   vinfo [vstage][4]           s_lo, t_lo, tw, (mode << 16) | staged columns
+template <int NW, bool TTAB>
 struct Lds {
 	int P, T, vinfo, total;
 	__device__ __forceinline__ Lds(int vstage, int slot)
 	{
 		P = vstage * slot;
 		T = P + vstage * 64 * 8;
-		vinfo = T + vstage * NW * 64;
+		vinfo = T + (TTAB ? vstage * NW * 64 : 0);
 		total = vinfo + vstage * 4;
 	}
 };
@@ -401,7 +401,8 @@ __device__ __forceinline__ void stage_word(const char *src, float *ldst)
 // per lane: the FP32 sample, or the 16-bit sample zero-extended).  The
 // caller waits for the loads (vmcnt) before the stage barrier.
 template <int NW, int NP, int ZL, int FMT, bool FAST>
-__device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW> &L,
+__device__ __forceinline__ void produce(const BackArgs &a, int bo,
+	const Lds<NW, FMT == FMT_F32> &L,
 	int v0, int nv, int pw, int lane, bool colvalid, float xx, float yy, int zg0, int zg1)
 {
 	const GeomF &g = a.g;
@@ -422,7 +423,7 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 		int mode = MODE_NONE, tw = 0, sw = 0;
 		if (s_lo <= s_hi && t_lo <= t_hi) {
 			tw = t_hi - t_lo + 1;
-			sw = s_hi - s_lo + 1;
+			sw = s_hi - s_lo + (DIRECT ? 0 : 1);     // staged columns (DIRECT: pairs)
 			mode = ((long) sw * tw <= a.slot) ? MODE_LDS : MODE_GLOBAL;
 		}
 		// LDS word of padded row 0 (DIRECT) or of row 1, t = 0 (walk) in the
@@ -436,23 +437,23 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 			wr = w2 * wr * sc;
 		}
 		const int zhi_st = act ? cr.zhi : cr.zlo;
+		const int is_st = act ? c.is : 0;
 		float4 *P4 = reinterpret_cast<float4 *>(fdk_lds + bo + L.P) + (vv * 64 + lane) * 2;
-		P4[0] = make_float4(__int_as_float(cb), wl, wr, DIRECT ? cr.first : w2);
+		P4[0] = make_float4(__int_as_float(mode == MODE_LDS ? cb : is_st), wl, wr,
+			DIRECT ? cr.first : w2);
 		P4[1] = make_float4(act ? c.t_inc : 0.f, __int_as_float(cr.zlo | (zhi_st << 16)),
-			__int_as_float(act ? c.is : 0), cr.last);
-		// vertical coordinate at each chunk's first active slice.  Walk: chunks
-		// the lane's range misses get t_first, so that their (discarded)
-		// reads stay inside the staged footprint.  DIRECT: u at the chunk's
-		// first slice; the consumer clamps u to [first, last].
+			DIRECT ? (act ? Tk[0] : 0.f) : __int_as_float(is_st), cr.last);
+		// walk: t_bin at each chunk's first active slice; chunks the lane's
+		// range misses get t_first, so that their (discarded) reads stay
+		// inside the staged footprint.  DIRECT: the consumer evaluates u at
+		// its chunk from U0 and clamps it to [first, last].
+		if constexpr (!DIRECT) {
 #pragma unroll
-		for (int k = 0; k < NW; ++k) {
-			const bool meets = cr.zlo < zg0 + (k + 1) * ZL && zg0 + k * ZL < cr.zhi;
-			float tk;
-			if constexpr (DIRECT)
-				tk = act ? Tk[k] : 0.f;
-			else
-				tk = act ? (meets ? Tk[k] : cr.first) : 0.f;
-			fdk_lds[bo + L.T + (vv * NW + k) * 64 + lane] = tk;
+			for (int k = 0; k < NW; ++k) {
+				const bool meets = cr.zlo < zg0 + (k + 1) * ZL && zg0 + k * ZL < cr.zhi;
+				fdk_lds[bo + L.T + (vv * NW + k) * 64 + lane] =
+					act ? (meets ? Tk[k] : cr.first) : 0.f;
+			}
 		}
 		if (lane == 0) {
 			int *vi = reinterpret_cast<int *>(fdk_lds) + bo + L.vinfo + vv * 4;
@@ -465,34 +466,43 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo, const Lds<NW>
 			continue;
 		// stage the footprint of the zero-padded view (the padded copy plays
 		// fdk_ts_put_view, fdk,ts,t.c:13-21): LDS word e of the slot <- padded
-		// sample (s_lo + e / tw, t_lo + e % tw)
+		// word (s_lo + e / tw, t_lo + e % tw) -- an FP32 sample, or for
+		// FP16/BF16 the column pair (s, s + 1)
 		const int area = sw * tw;
 		const int slot = bo + vv * a.slot;
 		const int pnt = g.nt + 2;
-		constexpr int esz = (FMT == FMT_F32) ? 4 : 2;
+		constexpr int esz = 4;
+		const int pcols = DIRECT ? g.ns + 1 : g.ns + 2;
 		const char *pv = static_cast<const char *>(a.padded)
-			+ ((size_t) v * (g.ns + 2) + s_lo) * pnt * esz + (size_t) t_lo * esz;
+			+ ((size_t) v * pcols + s_lo) * pnt * esz + (size_t) t_lo * esz;
+		// byte offsets from the view's footprint corner stay below 2^32 (one
+		// view), so the loads take a scalar base plus a 32-bit lane offset
 		if (tw == pnt) {
 			// full detector height: the footprint is contiguous in the padded view
-			for (int e0 = 0; e0 < area; e0 += 64) {
-				const int e = e0 + lane;
-				if (e < area)
-					stage_word<FMT>(pv + (size_t) e * esz, fdk_lds + slot + e0);
-			}
+			unsigned off = (unsigned) lane * esz;
+			for (int e0 = 0; e0 < area; e0 += 64, off += 64 * esz)
+				if (e0 + lane < area)
+					stage_word<FMT_F32>(pv + off, fdk_lds + slot + e0);
 		} else {
-			// rows of tw samples: (r, tc) of element e, advanced by 64 per step
+			// rows of tw samples: (r, tc) of element e, advanced by 64 per step;
+			// off = (r * pnt + tc) * esz
 			const unsigned magic = 0xffffffffu / (unsigned) tw + 1u;  // e/tw, exact (tw >= 2)
 			int r = (int) __umulhi((unsigned) lane, magic);
 			r -= (r * tw > lane) ? 1 : 0;
 			int tc = lane - r * tw;
 			const int dr = 64 / tw, dtc = 64 - dr * tw;
+			unsigned off = (unsigned) (r * pnt + tc) * esz;
+			const unsigned step = (unsigned) (dr * pnt + dtc) * esz;
+			const unsigned wrap_step = (unsigned) (pnt - tw) * esz;
 			for (int e0 = 0; e0 < area; e0 += 64) {
 				if (e0 + lane < area)
-					stage_word<FMT>(pv + ((size_t) r * pnt + tc) * esz, fdk_lds + slot + e0);
+					stage_word<FMT_F32>(pv + off, fdk_lds + slot + e0);
 				tc += dtc;
-				const int wrap = tc >= tw ? 1 : 0;
-				tc -= wrap * tw;
-				r += dr + wrap;
+				off += step;
+				if (tc >= tw) {
+					tc -= tw;
+					off += wrap_step;
+				}
 			}
 		}
 	}
@@ -537,22 +547,39 @@ __device__ __forceinline__ float slice_update(float acc, float t, int c1, int c2
 	}
 }
 
-// The same update for the FP16/BF16 paths in u = t_bin + 1 (> 0): the
-// padded row is the truncation of u, the vertical weight its fraction; c1/c2
-// address padded row 0 of the two staged columns; w2 and the view scale are
-// folded into wl/wr.
+// FP32 values of the two halves of a column-pair word (FP16 or BF16)
 template <int FMT>
-__device__ __forceinline__ float slice_update_u(float acc, float u, int c1, int c2,
-	float wl, float wr)
+__device__ __forceinline__ float pair_lo(uint32_t w)
+{
+	if constexpr (FMT == FMT_F16)
+		return __half2float(__ushort_as_half((unsigned short) (w & 0xffffu)));
+	else
+		return __uint_as_float(w << 16);
+}
+template <int FMT>
+__device__ __forceinline__ float pair_hi(uint32_t w)
+{
+	if constexpr (FMT == FMT_F16)
+		return __half2float(__ushort_as_half((unsigned short) (w >> 16)));
+	else
+		return __uint_as_float(w & 0xffff0000u);
+}
+
+// The same update for the FP16/BF16 paths in u = t_bin + 1 (> 0): the
+// padded row is the truncation of u, the vertical weight its fraction; c1
+// addresses padded row 0 of the lane's staged column pair (is, is + 1); w2
+// and the view scale are folded into wl/wr.
+template <int FMT>
+__device__ __forceinline__ float slice_update_u(float acc, float u, int c1, float wl, float wr)
 {
 	const int iu = (int) u;
 	const float wu = __builtin_amdgcn_fractf(u);
 	const float wd = 1.f - wu;
-	const float a0 = lds_sample<FMT>(c1 + iu), a1 = lds_sample<FMT>(c1 + iu + 1);
-	const float b0 = lds_sample<FMT>(c2 + iu), b1 = lds_sample<FMT>(c2 + iu + 1);
+	const uint32_t w0 = __float_as_uint(fdk_lds[c1 + iu]);
+	const uint32_t w1 = __float_as_uint(fdk_lds[c1 + iu + 1]);
 	// fmaf(w, x, 0) keeps the FP16 -> FP32 conversion inside v_fma_mix
-	const float q1 = fmaf(wr, b0, fmaf(wl, a0, 0.f));
-	const float q2 = fmaf(wr, b1, fmaf(wl, a1, 0.f));
+	const float q1 = fmaf(wr, pair_hi<FMT>(w0), fmaf(wl, pair_lo<FMT>(w0), 0.f));
+	const float q2 = fmaf(wr, pair_hi<FMT>(w1), fmaf(wl, pair_lo<FMT>(w1), 0.f));
 	acc = fmaf(wd, q1, acc);
 	return fmaf(wu, q2, acc);
 }
@@ -562,7 +589,8 @@ __device__ __forceinline__ float slice_update_u(float acc, float u, int c1, int 
 // The next view's record is read while the current one is added, so the
 // per-view LDS round trips overlap the slice arithmetic.
 template <int NW, int ZL, int FMT, bool FAST>
-__device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW> &L,
+__device__ __forceinline__ void consume(const BackArgs &a, int bo,
+	const Lds<NW, FMT == FMT_F32> &L,
 	int v0, int nv, int wave, int lane, int zc, float (&acc)[ZL])
 {
 	constexpr bool DIRECT = FMT != FMT_F32;
@@ -570,17 +598,20 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW>
 	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
 	const float *Tw = fdk_lds + bo + L.T + wave * 64 + lane;
 	const int2 *VI = reinterpret_cast<const int2 *>(fdk_lds + bo + L.vinfo + 2);
+	const float jc = (float) (wave * ZL);   // DIRECT: this wave's first slice in the group
 	float4 np0 = P4[0], np1 = P4[1];
-	float nt0 = Tw[0];
+	float nt0 = DIRECT ? 0.f : Tw[0];
 	int2 nvi = VI[0];
 	for (int vv = 0; vv < nv; ++vv) {
 		const float4 p0 = np0, p1 = np1;
-		const float t0 = nt0;
 		const int2 vi = nvi;
+		// t_bin (walk) or u (DIRECT) at this wave's first slice
+		const float t0 = DIRECT ? fmaf(jc, p1.x, p1.z) : nt0;
 		if (vv + 1 < nv) {
 			np0 = P4[(vv + 1) * 128];
 			np1 = P4[(vv + 1) * 128 + 1];
-			nt0 = Tw[(vv + 1) * NW * 64];
+			if constexpr (!DIRECT)
+				nt0 = Tw[(vv + 1) * NW * 64];
 			nvi = VI[(vv + 1) * 2];
 		}
 		const int mode = __builtin_amdgcn_readfirstlane(vi.y) >> 16;
@@ -605,8 +636,7 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW>
 				if constexpr (DIRECT) {
 #pragma unroll
 					for (int j = 0; j < ZL; ++j)
-						acc[j] = slice_update_u<FMT>(acc[j], fmaf((float) j, ti, t0), c1, c2,
-							wl, wr);
+						acc[j] = slice_update_u<FMT>(acc[j], fmaf((float) j, ti, t0), c1, wl, wr);
 				} else {
 					float t = t0;
 #pragma unroll
@@ -627,7 +657,7 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW>
 					for (int j = 0; j < ZL; ++j) {
 						const bool r = (unsigned) (j - jlo) < nj;
 						const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), uf, ul);
-						const float up = slice_update_u<FMT>(acc[j], u, c1, c2, wl, wr);
+						const float up = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
 						acc[j] = r ? up : acc[j];
 					}
 				} else {
@@ -645,28 +675,27 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo, const Lds<NW>
 			}
 		} else if (any) {
 			// footprint larger than the LDS slot: gather from the padded copy
-			const int is = __float_as_int(p1.z);
+			const int is = cb;                    // the record holds is for these views
 			const int pnt = g.nt + 2;
-			const size_t col = ((size_t) (v0 + vv) * (g.ns + 2) + is + 1) * pnt;
 			if constexpr (DIRECT) {
-				const size_t c1 = col, c2 = col + pnt;     // padded row 0
+				// column pair (is, is + 1), padded row 0
+				const uint32_t *pc = static_cast<const uint32_t *>(a.padded)
+					+ ((size_t) (v0 + vv) * (g.ns + 1) + is + 1) * pnt;
 #pragma unroll
 				for (int j = 0; j < ZL; ++j) {
 					if (j >= jlo && j < jhi) {
 						const float u = fmaf((float) j, ti, t0);
 						const int iu = (int) u;
 						const float wu = __builtin_amdgcn_fractf(u);
-						const float a0 = load_sample<FMT>(a.padded, c1 + iu);
-						const float a1 = load_sample<FMT>(a.padded, c1 + iu + 1);
-						const float b0 = load_sample<FMT>(a.padded, c2 + iu);
-						const float b1 = load_sample<FMT>(a.padded, c2 + iu + 1);
-						const float q1 = fmaf(wr, b0, fmaf(wl, a0, 0.f));
-						const float q2 = fmaf(wr, b1, fmaf(wl, a1, 0.f));
+						const uint32_t w0 = pc[iu], w1 = pc[iu + 1];
+						const float q1 = fmaf(wr, pair_hi<FMT>(w0), fmaf(wl, pair_lo<FMT>(w0), 0.f));
+						const float q2 = fmaf(wr, pair_hi<FMT>(w1), fmaf(wl, pair_lo<FMT>(w1), 0.f));
 						acc[j] = fmaf(1.f - wu, q1, acc[j]);
 						acc[j] = fmaf(wu, q2, acc[j]);
 					}
 				}
 			} else {
+				const size_t col = ((size_t) (v0 + vv) * (g.ns + 2) + is + 1) * pnt;
 				const size_t c1 = col + 1, c2 = c1 + pnt;  // (s = is, t = 0), (is+1, 0)
 				float t = t0;
 #pragma unroll
@@ -715,7 +744,7 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 	const int ix = tx * 8 + (lane & 7);
 	const int iy = ty * 8 + (lane >> 3);
 
-	const Lds<NW> L(a.vstage, a.slot);
+	const Lds<NW, FMT == FMT_F32> L(a.vstage, a.slot);
 	const int nstage = (g.na + a.vstage - 1) / a.vstage;
 
 	if (wave >= NW) {
@@ -765,8 +794,10 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 	}
 }
 
+// 4 consumer + 4 producer waves: two workgroups share a CU (at most 128
+// VGPRs); 8 + 8: one
 template <int NW, int NP, int ZL, int FMT, bool FAST>
-__global__ void __launch_bounds__((NW + NP) * 64)
+__global__ void __launch_bounds__((NW + NP) * 64, NW >= 8 ? 1 : 2)
 back_kernel(const BackArgs a)
 {
 	const int nblocks = a.ntiles * a.ngroups;
@@ -876,7 +907,7 @@ survey_kernel(const BackArgs a)
 		const int t_lo = max(wave_min(act ? cr.row_lo : 0x7fffffff), 0);
 		const int t_hi = min(wave_max(act ? cr.row_hi : (int) 0x80000000), g.nt + 1);
 		if (s_lo <= s_hi && t_lo <= t_hi) {
-			const long area = (long) (s_hi - s_lo + 1) * (t_hi - t_lo + 1);
+			const long area = (long) (s_hi - s_lo + (DIRECT ? 0 : 1)) * (t_hi - t_lo + 1);
 			maxarea = max(maxarea, (int) min(area, (long) 0x7fffffff));
 		}
 	}
@@ -925,6 +956,31 @@ __global__ void pad_kernel(const T *src, T *dst, int ns, int nt, long total)
 		if (s >= 1 && s <= ns && t >= 1 && t <= nt)
 			val = src[(v * ns + (s - 1)) * nt + (t - 1)];
 		dst[i] = val;
+	}
+}
+
+// FP16/BF16 views as column pairs: word (c, t) of view v holds the padded
+// samples (c, t) in its low and (c + 1, t) in its high half, c in [0, ns],
+// t in [0, nt+1], so one 32-bit read serves both columns of a bilinear
+// interpolation.  [na][ns+1][nt+2] words.
+__global__ void pair_pad_kernel(const uint16_t *src, uint32_t *dst, int ns, int nt, long total)
+{
+	const int pnt = nt + 2, pcols = ns + 1;
+	for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < total;
+		i += (long) gridDim.x * blockDim.x) {
+		const int t = (int) (i % pnt);
+		const long q = i / pnt;
+		const int c = (int) (q % pcols);
+		const long v = q / pcols;
+		uint32_t lo = 0, hi = 0;
+		if (t >= 1 && t <= nt) {
+			const uint16_t *col = src + v * ns * (long) nt + (t - 1);
+			if (c >= 1)          // padded column c = sample column c - 1
+				lo = col[(long) (c - 1) * nt];
+			if (c + 1 <= ns)     // padded column c + 1 = sample column c
+				hi = col[(long) c * nt];
+		}
+		dst[i] = lo | (hi << 16);
 	}
 }
 
@@ -1090,10 +1146,10 @@ static hipError_t launch_back_t(const BackArgs &a, size_t lds_bytes, hipStream_t
 }
 
 // two stage buffers of: slots + 8 per-(view,column) arrays + T + vinfo (words)
-size_t back_lds_bytes(int nw, int vstage, int slot)
+size_t back_lds_bytes(int nw, int vstage, int slot, int fmt)
 {
 	const size_t words = (size_t) vstage * slot + (size_t) vstage * 64 * 8
-		+ (size_t) vstage * nw * 64 + (size_t) vstage * 4;
+		+ (fmt == FMT_F32 ? (size_t) vstage * nw * 64 : 0) + (size_t) vstage * 4;
 	return 2 * words * 4;
 }
 
@@ -1115,7 +1171,7 @@ static int producers_for(int nw)
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s)
 {
 	(void) producers_for;
-	const size_t lds = back_lds_bytes(nw, a.vstage, a.slot);
+	const size_t lds = back_lds_bytes(nw, a.vstage, a.slot, fmt);
 	if (zl == 16) {
 		if (nw == 4) FDK_DISPATCH_FMT(4, 16);
 		if (nw == 8) FDK_DISPATCH_FMT(8, 16);
@@ -1149,7 +1205,7 @@ static hipError_t launch_batch_t(const BatchArgs &b, int nwg, size_t lds_bytes, 
 hipError_t launch_batch(const BatchArgs &b, int nwg, int nw, int zl, int vstage, int slot,
 	int fmt, bool fast, hipStream_t s)
 {
-	const size_t lds = back_lds_bytes(nw, vstage, slot);
+	const size_t lds = back_lds_bytes(nw, vstage, slot, fmt);
 	if (zl == 16) {
 		if (nw == 4) FDK_DISPATCH_BATCH(4, 16);
 		if (nw == 8) FDK_DISPATCH_BATCH(8, 16);
@@ -1183,7 +1239,7 @@ static int batch_occupancy_t(size_t lds_bytes)
 
 int batch_blocks_per_cu(int nw, int zl, int vstage, int slot, int fmt, bool fast)
 {
-	const size_t lds = back_lds_bytes(nw, vstage, slot);
+	const size_t lds = back_lds_bytes(nw, vstage, slot, fmt);
 	if (zl == 16) {
 		if (nw == 4) FDK_OCC_BATCH(4, 16);
 		if (nw == 8) FDK_OCC_BATCH(8, 16);
@@ -1234,14 +1290,17 @@ hipError_t launch_fill(float *image, long ncol, int nz, int zlo, int zhi, hipStr
 hipError_t launch_pad(const void *src, void *dst, int ns, int nt, int na, int fmt,
 	hipStream_t s)
 {
-	const long total = (long) na * (ns + 2) * (nt + 2);
-	const int blocks = (int) min((total + 255) / 256, (long) 8192);
-	if (fmt == FMT_F32)
+	if (fmt == FMT_F32) {
+		const long total = (long) na * (ns + 2) * (nt + 2);
+		const int blocks = (int) min((total + 255) / 256, (long) 8192);
 		hipLaunchKernelGGL(pad_kernel<float>, dim3(blocks), dim3(256), 0, s,
 			(const float *) src, (float *) dst, ns, nt, total);
-	else
-		hipLaunchKernelGGL(pad_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s,
-			(const uint16_t *) src, (uint16_t *) dst, ns, nt, total);
+	} else {
+		const long total = (long) na * (ns + 1) * (nt + 2);
+		const int blocks = (int) min((total + 255) / 256, (long) 8192);
+		hipLaunchKernelGGL(pair_pad_kernel, dim3(blocks), dim3(256), 0, s,
+			(const uint16_t *) src, (uint32_t *) dst, ns, nt, total);
+	}
 	return hipGetLastError();
 }
 
