@@ -577,9 +577,17 @@ __device__ __forceinline__ float slice_update_u(float acc, float u, int c1, floa
 	const float wd = 1.f - wu;
 	const uint32_t w0 = __float_as_uint(fdk_lds[c1 + iu]);
 	const uint32_t w1 = __float_as_uint(fdk_lds[c1 + iu + 1]);
-	// fmaf(w, x, 0) keeps the FP16 -> FP32 conversion inside v_fma_mix
-	const float q1 = fmaf(wr, pair_hi<FMT>(w0), fmaf(wl, pair_lo<FMT>(w0), 0.f));
-	const float q2 = fmaf(wr, pair_hi<FMT>(w1), fmaf(wl, pair_lo<FMT>(w1), 0.f));
+	float q1, q2;
+	if constexpr (FMT == FMT_F16) {
+		// fmaf(w, x, 0) keeps the FP16 -> FP32 conversion inside v_fma_mix;
+		// the two products are independent so that their mixed-precision
+		// FMAs overlap instead of forming a chain
+		q1 = fmaf(wl, pair_lo<FMT>(w0), 0.f) + fmaf(wr, pair_hi<FMT>(w0), 0.f);
+		q2 = fmaf(wl, pair_lo<FMT>(w1), 0.f) + fmaf(wr, pair_hi<FMT>(w1), 0.f);
+	} else {
+		q1 = fmaf(wr, pair_hi<FMT>(w0), wl * pair_lo<FMT>(w0));
+		q2 = fmaf(wr, pair_hi<FMT>(w1), wl * pair_lo<FMT>(w1));
+	}
 	acc = fmaf(wd, q1, acc);
 	return fmaf(wu, q2, acc);
 }
