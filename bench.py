@@ -188,7 +188,9 @@ def run_single(args, dev):
         plan = FdkPlan(geom, beta, mask, fmt=fmt, math_mode=mm)
         a = prepare(tsa, fmt)
         out = torch.empty((ig.ny, ig.nx, ig.nz), dtype=torch.float32, device=dev)
-        if not args.no_autotune:
+        if args.launch:
+            plan.set_config(*args.launch)
+        elif not args.no_autotune:
             t0 = time.perf_counter()
             best = plan.autotune(*a, out=out, reps=2)
             log(f"{name}: autotune {time.perf_counter() - t0:.1f} s -> {plan.config} "
@@ -377,6 +379,9 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the plans' default launch shape instead of the tuned one")
+    ap.add_argument("--launch", type=lambda v: tuple(int(x) for x in v.split(",")),
+                    default=None, metavar="WAVES,ZPER,VSTAGE",
+                    help="fixed launch shape (no autotune), e.g. 4,32,4 (profiling runs)")
     ap.add_argument("--fmt-only", action="store_true",
                     help="time only the headline path (profiling runs)")
     args = ap.parse_args()

@@ -160,12 +160,13 @@ int fdkhip_plan_destroy(fdkhip_plan *plan);
 int fdkhip_plan_count_updates(fdkhip_plan *plan, int64_t *n_updates);
 
 /* Time every launch shape the build supports with each views-per-stage depth
- * that fits LDS (`reps` launches each, on `stream`, into image_dev), keep the
- * fastest for this plan and remember it for later plans of the same
- * geometry, view count, format and math mode in this process.  The image is
- * overwritten.  *best_ms (may be NULL) receives the best average time.
- * Synchronous.  No counterpart in the reference, whose thread count is the
- * caller's nthread. */
+ * that fits LDS (`reps` launches of the backprojection kernel each, over the
+ * first 128 views, on `stream`), keep the fastest for this plan and remember
+ * it for later plans of the same geometry, view count, format and math mode
+ * in this process.  image_dev is overwritten (its contents are undefined
+ * afterwards).  *best_ms (may be NULL) receives the best average time of a
+ * 128-view launch.  Synchronous.  No counterpart in the reference, whose only
+ * knob is the caller's nthread. */
 int fdkhip_plan_autotune(fdkhip_plan *plan, const void *proj_dev,
 	const float *scale_dev, float *image_dev, void *stream, int32_t reps,
 	float *best_ms);
@@ -215,7 +216,8 @@ int fdkhip_batch_config(const fdkhip_batch *batch, int32_t cfg[6]);
 int fdkhip_batch_set_config(fdkhip_batch *batch, int32_t waves, int32_t zper,
 	int32_t vstage);
 
-/* As fdkhip_plan_autotune, over the batch's launch shapes and stage depths. */
+/* As fdkhip_plan_autotune, over the batch's launch shapes and stage depths
+ * (the persistent kernel over the first 128 views of every scan). */
 int fdkhip_batch_autotune(fdkhip_batch *batch, const void *const *proj,
 	const float *const *scale, float *const *image, void *stream, int32_t reps,
 	float *best_ms);

@@ -334,8 +334,9 @@ class FdkPlan:
 
     def autotune(self, proj, scale=None, out=None, *, reps=3, stream=None):
         """Time the supported launch shapes and stage depths on these buffers
-        and keep the fastest (fdkhip_plan_autotune); returns the best average
-        milliseconds.  ``out`` (or a scratch image) is overwritten."""
+        (the backprojection kernel over the first 128 views) and keep the
+        fastest (fdkhip_plan_autotune); returns the best average milliseconds
+        of such a launch.  ``out`` (or a scratch image) is overwritten."""
         import torch
         self._check_proj(proj, scale)
         g = self.geom
@@ -485,7 +486,9 @@ class FdkBatch:
         return {"prep": ms[0], "back": ms[1]}
 
     def autotune(self, projs, scales=None, outs=None, *, reps=3, stream=None):
-        """fdkhip_batch_autotune on these buffers; returns the best average ms."""
+        """fdkhip_batch_autotune on these buffers (the persistent kernel over
+        the first 128 views of every scan); returns the best average ms of
+        such a launch.  ``outs`` are overwritten."""
         import torch
         outs, pa, sa, oa = self._arrays(projs, scales, outs)
         s = stream if stream is not None else torch.cuda.current_stream(projs[0].device)

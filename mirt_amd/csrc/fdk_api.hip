@@ -182,6 +182,12 @@ struct fdkhip_plan {
 	int *d_maxarea = nullptr;
 	int *d_any = nullptr;
 	void *d_padded = nullptr;     // zero-padded copy of the views (see launch_pad)
+	// survey results per launch shape (nw, zl): maxarea, fov_any, n_updates
+	struct Survey {
+		int nw, zl, maxarea, fov_any;
+		long long n_updates;
+	};
+	std::vector<Survey> surveys;
 	size_t padded_bytes = 0;
 	// launch configuration
 	int nw = 8, zl = 16, vstage = 8, slot = 0;
@@ -281,7 +287,15 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 	p->maxarea = 0;
 	p->fov_any = 0;
 	p->n_updates = 0;
-	if (p->ngroups > 0 && p->tiles_x > 0 && p->tiles_y > 0 && p->na > 0) {
+	bool cached = false;
+	for (const auto &sv : p->surveys)
+		if (sv.nw == nw && sv.zl == zl) {
+			p->maxarea = sv.maxarea;
+			p->fov_any = sv.fov_any;
+			p->n_updates = sv.n_updates;
+			cached = true;
+		}
+	if (!cached && p->ngroups > 0 && p->tiles_x > 0 && p->tiles_y > 0 && p->na > 0) {
 		HIP_TRY(launch_zero_u64(p->d_count, 1, nullptr));
 		HIP_TRY(launch_zero_i32(p->d_maxarea, 1, nullptr));
 		HIP_TRY(launch_zero_i32(p->d_any, 1, nullptr));
@@ -292,6 +306,7 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 		HIP_TRY(hipMemcpy(&p->maxarea, p->d_maxarea, sizeof(int), hipMemcpyDeviceToHost));
 		HIP_TRY(hipMemcpy(&p->fov_any, p->d_any, sizeof(int), hipMemcpyDeviceToHost));
 		p->n_updates = (long long) cnt;
+		p->surveys.push_back({nw, zl, p->maxarea, p->fov_any, p->n_updates});
 	}
 	// views per stage and slot size within the LDS budget: 8 consumer waves
 	// (+8 producers) fill a CU with one workgroup, 4 (+4) fit two
@@ -708,6 +723,7 @@ namespace {
 // launch shapes (consumer waves, slices per lane) the library is built with
 const int kShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}};
 const int kStages[] = {0, 3, 4, 6, 8, 12, 16};
+const int kTuneViews = 128;   // views per timed launch while tuning
 
 struct Tuned {
 	int nw, zl, vstage;
@@ -791,6 +807,11 @@ extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const floa
 	hipStream_t s = (hipStream_t) stream;
 	Tuned best{p->nw, p->zl, p->vstage};
 	float best_t = 3.4e38f;
+	// the padded views once; then the backprojection kernel alone over the
+	// first kTuneViews views (its time is proportional to the view count)
+	rc = plan_launch(p, proj, scale, image, s);
+	if (rc)
+		return rc;
 	for (const auto &sh : kShapes) {
 		int last_vstage = -1;
 		for (int vs : kStages) {
@@ -799,9 +820,14 @@ extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const floa
 			if (p->vstage == last_vstage)
 				continue;   // the LDS budget clamped it to a depth already timed
 			last_vstage = p->vstage;
+			if (p->ngroups == 0 || p->na == 0)
+				continue;
 			float t = 0.f;
 			rc = time_launches([&](hipStream_t st) {
-				return plan_launch(p, proj, scale, image, st);
+				BackArgs a = make_args(p, proj, scale, image);
+				a.g.na = std::min(p->na, kTuneViews);
+				HIP_TRY(launch_back(a, p->nw, p->zl, p->fmt, p->math == FDKHIP_MATH_FAST, st));
+				return FDKHIP_OK;
 			}, s, reps, &t);
 			if (rc)
 				return rc;
@@ -1039,6 +1065,22 @@ extern "C" int fdkhip_batch_count_updates(fdkhip_batch *b, int64_t *n)
 // Per-scan zero fills and padded copies, then the persistent kernel (and its
 // queue reset).  With `ev`, ev[0..2] are recorded before the per-scan steps,
 // between them and the kernel, and after it.
+// the persistent kernel alone (with its queue reset), on the uploaded arguments
+static int batch_kernel_launch(fdkhip_batch *b, hipStream_t s)
+{
+	if (b->nitems == 0)
+		return FDKHIP_OK;
+	BatchArgs ba;
+	ba.scans = b->d_scans;
+	ba.item_begin = b->d_item_begin;
+	ba.nscan = b->nscan;
+	ba.nitems = b->nitems;
+	ba.counter = b->d_counter;
+	HIP_TRY(launch_batch(ba, b->nwg, b->nw, b->zl, b->vstage, b->slot, b->fmt,
+		b->math == FDKHIP_MATH_FAST, s));
+	return FDKHIP_OK;
+}
+
 static int batch_launch(fdkhip_batch *b, const void *const *proj, const float *const *scale,
 	float *const *image, hipStream_t s, hipEvent_t *ev = nullptr)
 {
@@ -1057,19 +1099,9 @@ static int batch_launch(fdkhip_batch *b, const void *const *proj, const float *c
 	}
 	if (ev)
 		HIP_TRY(hipEventRecord(ev[1], s));
-	if (b->nitems == 0) {
-		if (ev)
-			HIP_TRY(hipEventRecord(ev[2], s));
-		return FDKHIP_OK;
-	}
-	BatchArgs ba;
-	ba.scans = b->d_scans;
-	ba.item_begin = b->d_item_begin;
-	ba.nscan = b->nscan;
-	ba.nitems = b->nitems;
-	ba.counter = b->d_counter;
-	HIP_TRY(launch_batch(ba, b->nwg, b->nw, b->zl, b->vstage, b->slot, b->fmt,
-		b->math == FDKHIP_MATH_FAST, s));
+	int rc = batch_kernel_launch(b, s);
+	if (rc)
+		return rc;
 	if (ev)
 		HIP_TRY(hipEventRecord(ev[2], s));
 	return FDKHIP_OK;
@@ -1092,11 +1124,14 @@ static int batch_check(fdkhip_batch *b, const void *const *proj, const float *co
 
 // per-scan kernel arguments for these buffers -> device (ordered on `s`)
 static int batch_upload_args(fdkhip_batch *b, const void *const *proj,
-	const float *const *scale, float *const *image, hipStream_t s)
+	const float *const *scale, float *const *image, hipStream_t s, int na_cap = 0)
 {
 	b->host_args.resize(b->nscan);
-	for (int i = 0; i < b->nscan; ++i)
+	for (int i = 0; i < b->nscan; ++i) {
 		b->host_args[i] = make_args(b->plans[i], proj[i], scale ? scale[i] : nullptr, image[i]);
+		if (na_cap > 0)
+			b->host_args[i].g.na = std::min(b->host_args[i].g.na, na_cap);
+	}
 	HIP_TRY(hipMemcpyAsync(b->d_scans, b->host_args.data(), sizeof(BackArgs) * b->nscan,
 		hipMemcpyHostToDevice, s));
 	HIP_TRY(hipStreamSynchronize(s));   // host_args may be rewritten by the next call
@@ -1166,6 +1201,13 @@ extern "C" int fdkhip_batch_autotune(fdkhip_batch *b, const void *const *proj,
 	hipStream_t s = (hipStream_t) stream;
 	int bn = b->nw, bz = b->zl, bv = b->vstage;
 	float best_t = 3.4e38f;
+	// zero fills and padded views once; then the persistent kernel alone over
+	// the first kTuneViews views of every scan
+	rc = batch_upload_args(b, proj, scale, image, s);
+	if (!rc)
+		rc = batch_launch(b, proj, scale, image, s);
+	if (rc)
+		return rc;
 	for (const auto &sh : kShapes) {
 		int last_vstage = -1;
 		for (int vs : kStages) {
@@ -1174,12 +1216,12 @@ extern "C" int fdkhip_batch_autotune(fdkhip_batch *b, const void *const *proj,
 			if (b->vstage == last_vstage)
 				continue;
 			last_vstage = b->vstage;
-			rc = batch_upload_args(b, proj, scale, image, s);
+			rc = batch_upload_args(b, proj, scale, image, s, kTuneViews);
 			if (rc)
 				return rc;
 			float t = 0.f;
 			rc = time_launches([&](hipStream_t st) {
-				return batch_launch(b, proj, scale, image, st);
+				return batch_kernel_launch(b, st);
 			}, s, reps, &t);
 			if (rc)
 				return rc;
