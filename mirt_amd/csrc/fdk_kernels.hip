@@ -662,22 +662,35 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 					// u clamped into the lane's [u_first, u_last]
 					const float uf = w2, ul = p1.w;
 #pragma unroll
-					for (int j = 0; j < ZL; ++j) {
-						const bool r = (unsigned) (j - jlo) < nj;
-						const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), uf, ul);
-						const float up = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
-						acc[j] = r ? up : acc[j];
+					for (int g0 = 0; g0 < ZL; g0 += 8) {
+						// groups of 8 slices that no lane of the wave adds are skipped
+						if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
+							continue;
+#pragma unroll
+						for (int j = g0; j < g0 + 8; ++j) {
+							const bool r = (unsigned) (j - jlo) < nj;
+							const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), uf, ul);
+							const float up = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
+							acc[j] = r ? up : acc[j];
+						}
 					}
 				} else {
 					// at t0, the lane's first active slice of this chunk
 					float t = t0;
 #pragma unroll
-					for (int j = 0; j < ZL; ++j) {
-						const bool r = (unsigned) (j - jlo) < nj;
-						const float tv = r ? t : t0;
-						const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
-						acc[j] = r ? up : acc[j];
-						t = r ? t + ti : t;
+					for (int g0 = 0; g0 < ZL; g0 += 8) {
+						// groups of 8 slices that no lane of the wave adds are
+						// skipped (t advances only on added slices)
+						if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
+							continue;
+#pragma unroll
+						for (int j = g0; j < g0 + 8; ++j) {
+							const bool r = (unsigned) (j - jlo) < nj;
+							const float tv = r ? t : t0;
+							const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
+							acc[j] = r ? up : acc[j];
+							t = r ? t + ti : t;
+						}
 					}
 				}
 			}
