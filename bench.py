@@ -111,12 +111,23 @@ def time_steps(fn, steps, warmup):
     return e0.elapsed_time(e1) / steps      # ms per step
 
 
-def cpu_reference_rate(g, tsa_host, beta, mask, nx_ny_nz, target_s=15.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_reference_rate(g, tsa_host, beta, mask, nx_ny_nz, target_s=15.0, nthread=None):
     """Reference C on this host on an evenly spaced sample of views."""
     from oracle import oracle as O
     if not O.have_reference("fixed"):
         return None
-    nthread = max(1, min(16, host_cores()))
+    if nthread is None:
+        nthread = max(1, min(16, host_cores()))
     na = tsa_host.shape[0]
     probe = np.arange(0, na, max(1, na // 4))[:4]
     O.ref_back_ts("fixed", g, tsa_host[probe], beta[probe], mask, nthread=nthread)  # warm-up
@@ -139,8 +150,24 @@ def cpu_reference_rate(g, tsa_host, beta, mask, nx_ny_nz, target_s=15.0):
             "cores": nthread, "kind": "reference",
             "sample": f"reference C (fdk-helix fdk_ts_back_t, gcc -O3, {nthread} pthreads) "
                       f"on {len(views)} of {na} views (every {step}th), full "
-                      f"{nx}x{ny}x{nz} volume; median of {len(times)} runs of {t:.2f} s",
+                      f"{nx}x{ny}x{nz} volume; median of {len(times)} runs of {t:.2f} s; "
+                      f"host CPU {cpu_model()}, {host_cores()} cores available",
             "seconds": t}
+
+
+def end_to_end_ms(g_mex, tsa_host, beta, reps=2):
+    """Host buffers in, host image out through the drop-in C-ABI entry
+    (fdkhip_fdk_ts_back: H2D views, plan set-up, FP32 EXACT kernel, D2H
+    image): wall milliseconds, best of `reps`."""
+    from mirt_amd import fdk_mex
+    ptsa = np.asfortranarray(tsa_host.transpose(2, 1, 0))   # [nt ns na]
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fdk_mex("fdk,ts,back", *g_mex, ptsa, beta, np.array([1], np.int32))
+        t = (time.perf_counter() - t0) * 1e3
+        best = t if best is None else min(best, t)
+    return best
 
 
 def column_errors(g, tsa_host, beta, mask, images, ncol=1024, seed=7):
@@ -263,6 +290,24 @@ def run_single(args, dev):
         t0 = time.perf_counter()
         result["cpu_baseline"] = cpu_reference_rate(g, tsa_host, beta, m,
                                                     (ig.nx, ig.ny, ig.nz))
+        one = cpu_reference_rate(g, tsa_host, beta, m, (ig.nx, ig.ny, ig.nz),
+                                 target_s=5.0, nthread=1)
+        if result["cpu_baseline"] is not None and one is not None:
+            result["cpu_baseline"]["value_1thread"] = one["value"]
+            result["cpu_baseline"]["sample_1thread"] = one["sample"]
+        g_mex = (np.array([ig.nx, ig.ny, ig.nz], np.int32),
+                 np.array([ig.dx, ig.dy, ig.dz], np.float64),
+                 np.array([ig.offset_x, ig.offset_y, ig.offset_z], np.float64),
+                 np.asfortranarray(np.asarray(mask) != 0).astype(np.uint8),
+                 float(cg.dso), float(cg.dsd), float(cg.dfs),
+                 np.array([cg.ds, cg.dt], np.float64),
+                 np.array([cg.offset_s, cg.offset_t], np.float64))
+        e2e = end_to_end_ms(g_mex, tsa_host, np.ascontiguousarray(beta, np.float64))
+        result["end_to_end"] = {
+            "ms": e2e, "value": U / (e2e * 1e-3), "unit": "voxel-view updates/s",
+            "path": "host buffers through fdkhip_fdk_ts_back (H2D views, plan set-up, "
+                    "FP32 EXACT kernel, D2H image); not the headline value"}
+        log(f"end to end (host buffers, FP32 EXACT): {e2e:.1f} ms")
         imgs = {k: v.cpu().numpy() for k, v in images.items()}
         result["errors"] = column_errors(g, tsa_host, beta, m, imgs)
         log(f"cpu baseline + oracle errors in {time.perf_counter() - t0:.1f} s")

@@ -97,6 +97,22 @@ int fdkhip_fdk_ts_back(
 	float *image);
 
 /*
+ * The whole use_mex = 1 path of cbct_back_mex (fbp/cbct_back.m:74-137) on host
+ * buffers in MATLAB order: proj_sta is `single(proj)`, [ns nt na] (s fastest,
+ * g->nt detector rows); views 1:ia_skip:na with angles beta[0], beta[ia_skip],
+ * ...; the edge rows are replicated extrapolate_t times at both ends
+ * (cbct_back.m:86-94, which refuses offset_t != 0 then); the result is
+ * image_xyz [nx ny nz] (x fastest) times `scale` (narrowed to single, as
+ * MATLAB multiplies a single array by a double scalar; the reference's scale
+ * is 0.5 * deg2rad(|orbit|) / (na / ia_skip), or 1 without scale_dang).  The
+ * permutes and the row replication run on the device; the backprojection is
+ * the EXACT kernel of fdkhip_fdk_ts_back.  mask may be NULL.  Synchronous.
+ */
+int fdkhip_cbct_back(const fdkhip_geom *g, const float *proj_sta, int32_t na,
+	const double *beta, int32_t ia_skip, const uint8_t *mask, int32_t extrapolate_t,
+	double scale, float *image_xyz);
+
+/*
  * 'fdk,st,back' on host memory (use_mex = 3 in fbp/cbct_back.m:121-127):
  * fdk_st_back1 (mex/src/fdk/fdk-orig/fdk,st.c:38-141) over all views, proj
  * [ns nt na] (s fastest), image [nx ny nz] (x fastest), no zero padding,

@@ -173,6 +173,7 @@ def cbct_back(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: int = 1
         use_mex = 1
     if nthread is None:
         nthread = 1
+    device_path = back_call is None and use_mex == 1 and 1 <= int(nthread) <= 250
     if back_call is None:
         back_call = fdk_mex
     if not use_mex:
@@ -194,6 +195,11 @@ def cbct_back(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: int = 1
         proj = proj[:, :, None]
     if proj.shape != (ns, nt, na):
         raise ValueError(f"proj shape {proj.shape} != [ns nt na] = {(ns, nt, na)}")
+    if device_path:
+        # permutes, edge-row replication and the angle scale on the device
+        # (fdkhip_cbct_back); same arithmetic as the host route below
+        return _cbct_back_device(proj, cg, ig, betas, ia_skip, int(extrapolate_t),
+                                 scale_dang)
     nthread = np.array([nthread], dtype=np.int32)
     offset_t = cg.offset_t
     if extrapolate_t > 0:
@@ -231,6 +237,24 @@ def cbct_back(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: int = 1
         raise ValueError("bug")
     if scale_dang:
         img = angle_scale(cg, ia_skip) * img
+    return img
+
+
+def _cbct_back_device(proj, cg, ig, betas, ia_skip, extrapolate_t, scale_dang):
+    """cbct_back's use_mex = 1 route through fdkhip_cbct_back: ``proj``
+    [ns nt na] float32 in, [nx ny nz] float32 out."""
+    lib = _lib.load()
+    if extrapolate_t > 0 and cg.offset_t != 0:
+        raise ValueError("offset_t not yet done for extrapolation")
+    geom = geom_of(cg, ig)                       # detector rows before replication
+    proj = np.asfortranarray(proj, dtype=np.float32)
+    betas = np.ascontiguousarray(betas, dtype=np.float64)
+    mask = np.asfortranarray(ig.mask_or != 0).astype(np.uint8, order="F")
+    img = np.empty((ig.nx, ig.ny, ig.nz), dtype=np.float32, order="F")
+    scale = float(angle_scale(cg, ia_skip)) if scale_dang else 1.0
+    _lib.check(lib.fdkhip_cbct_back(C.byref(geom), _ptr(proj), int(cg.na), _ptr(betas),
+                                    int(ia_skip), _ptr(mask), int(extrapolate_t), scale,
+                                    _ptr(img)))
     return img
 
 

@@ -661,6 +661,96 @@ extern "C" int fdkhip_fdk_ts_back(
 	return rc;
 }
 
+// cbct_back_mex's whole use_mex = 1 path (fbp/cbct_back.m:74-137) on host
+// buffers in MATLAB order, with the permutes, the edge-row replication and
+// the angle scale done on the device.
+extern "C" int fdkhip_cbct_back(const fdkhip_geom *g, const float *proj_sta, int32_t na,
+	const double *beta, int32_t ia_skip, const uint8_t *mask, int32_t extrapolate_t,
+	double scale, float *image_xyz)
+{
+	if (!g)
+		return fail(FDKHIP_EARG, "geometry is NULL");
+	int rc = check_geom(g);
+	if (rc)
+		return rc;
+	if (na < 0)
+		return fail(FDKHIP_EARG, "na must be >= 0 (na=%d)", na);
+	if (ia_skip < 1)
+		return fail(FDKHIP_EARG, "ia_skip must be >= 1 (ia_skip=%d)", ia_skip);
+	if (extrapolate_t < 0)
+		return fail(FDKHIP_EARG, "extrapolate_t must be >= 0");
+	if (extrapolate_t > 0 && g->offset_t != 0)   // cbct_back.m:91-93
+		return fail(FDKHIP_EARG, "offset_t not yet done for extrapolation");
+	const size_t nvox = (size_t) g->nx * g->ny * g->nz;
+	if (nvox > 0 && !image_xyz)
+		return fail(FDKHIP_EARG, "image is NULL");
+	if (na > 0 && (!beta || (!proj_sta && (size_t) g->ns * g->nt > 0)))
+		return fail(FDKHIP_EARG, "proj or beta is NULL");
+	if (g->nz > 65535)
+		return fail(FDKHIP_EUNSUPPORTED, "nz=%d: at most 65535 slices per volume", g->nz);
+	if (nvox == 0)
+		return FDKHIP_OK;
+	const int e = extrapolate_t;
+	const int ns = g->ns, nt = g->nt, nte = nt + 2 * e;
+	const int nsub = (na + ia_skip - 1) / ia_skip;   // views 1:ia_skip:na
+	fdkhip_geom gk = *g;
+	gk.nt = nte;
+	std::vector<double> bsub((size_t) nsub);
+	for (int i = 0; i < nsub; ++i)
+		bsub[i] = beta[(size_t) i * ia_skip];
+	fdkhip_plan *p = nullptr;
+	rc = fdkhip_plan_create(&p, &gk, nsub, bsub.data(), mask, FDKHIP_F32, FDKHIP_MATH_EXACT);
+	if (rc)
+		return rc;
+	const size_t view = (size_t) ns * nt, viewe = (size_t) ns * nte;
+	DevBuf dsta, dtsa, dzxy, dxyz;
+	hipError_t err = hipSuccess;
+	if (nsub > 0 && view > 0) {
+		err = hipMalloc(&dsta.p, view * nsub * sizeof(float));
+		if (err == hipSuccess)
+			err = hipMalloc(&dtsa.p, viewe * nsub * sizeof(float));
+	}
+	if (err == hipSuccess)
+		err = hipMalloc(&dzxy.p, nvox * sizeof(float));
+	if (err == hipSuccess)
+		err = hipMalloc(&dxyz.p, nvox * sizeof(float));
+	if (err != hipSuccess) {
+		plan_free(p);
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(err));
+	}
+#define CB_TRY(expr)                                                             \
+	do {                                                                         \
+		hipError_t e_ = (expr);                                                  \
+		if (e_ != hipSuccess) {                                                  \
+			plan_free(p);                                                        \
+			return fail(FDKHIP_EHIP, "%s: %s", #expr, hipGetErrorString(e_));    \
+		}                                                                        \
+	} while (0)
+	if (nsub > 0 && view > 0) {
+		// views 0, ia_skip, 2 ia_skip, ... of [ns nt na] in one strided copy
+		CB_TRY(hipMemcpy2D(dsta.p, view * sizeof(float), proj_sta,
+			view * ia_skip * sizeof(float), view * sizeof(float), nsub,
+			hipMemcpyHostToDevice));
+		// [ns nt] -> [nt+2e ns] per view, edge rows replicated (cbct_back.m:86-98)
+		CB_TRY(launch_transpose((const float *) dsta.p, (float *) dtsa.p, nsub, nt, ns,
+			(long) view, ns, (long) viewe, nte, nte, e, 1.f, nullptr));
+	}
+	rc = plan_launch(p, dtsa.p, nullptr, (float *) dzxy.p, nullptr);
+	if (rc) {
+		plan_free(p);
+		return rc;
+	}
+	// [nz nx ny] -> [nx ny nz] with the angle scale (cbct_back.m:104,133-135);
+	// the double scale multiplies a single image in single, as MATLAB does
+	CB_TRY(launch_transpose((const float *) dzxy.p, (float *) dxyz.p, g->ny, g->nx, g->nz,
+		(long) g->nx * g->nz, g->nz, g->nx, (long) g->nx * g->ny, g->nx, 0, (float) scale,
+		nullptr));
+	CB_TRY(hipMemcpy(image_xyz, dxyz.p, nvox * sizeof(float), hipMemcpyDeviceToHost));
+#undef CB_TRY
+	plan_free(p);
+	return FDKHIP_OK;
+}
+
 extern "C" int fdkhip_fdk_st_back(
 	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
 	const uint8_t *mask, double dso, double dsd, double dfs,

@@ -65,6 +65,9 @@ hipError_t launch_batch(const BatchArgs &b, int nwg, int nw, int zl, int vstage,
 int batch_blocks_per_cu(int nw, int zl, int vstage, int slot, int fmt, bool fast);
 hipError_t launch_fill(float *image, long ncol, int nz, int zlo, int zhi, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
+// out[b*out_b + c*out_c + r] = scale * in[b*in_b + clamp(r - rshift, 0, R-1)*in_r + c]
+hipError_t launch_transpose(const float *in, float *out, int B, int R, int C, long in_b,
+	long in_r, long out_b, long out_c, int Rout, int rshift, float scale, hipStream_t s);
 hipError_t launch_pad(const void *src, void *dst, int ns, int nt, int na, int fmt,
 	hipStream_t s);
 hipError_t launch_zero_i32(int *p, int n, hipStream_t s);

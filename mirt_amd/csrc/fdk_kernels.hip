@@ -1005,6 +1005,42 @@ __global__ void pair_pad_kernel(const uint16_t *src, uint32_t *dst, int ns, int 
 	}
 }
 
+// Tiled transpose with row shift/clamp and scale, the device form of the
+// host-side permutes around the mex call (fbp/cbct_back.m:86-98,104,133-135):
+//   out[b*out_b + c*out_c + r] = scale * in[b*in_b + clamp(r - rshift, 0, R-1)*in_r + c]
+// for r in [0, Rout), c in [0, C), b in [0, B).  32 x 32 tiles through LDS so
+// that reads (c fastest) and writes (r fastest) are both coalesced.
+__global__ void __launch_bounds__(256)
+transpose_kernel(const float *in, float *out, int B, int R, int C, long in_b, long in_r,
+	long out_b, long out_c, int Rout, int rshift, float scale)
+{
+	__shared__ float tile[32][33];
+	const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+	for (int b = blockIdx.z; b < B; b += gridDim.z) {
+		const float *ib = in + (long) b * in_b;
+		float *ob = out + (long) b * out_b;
+#pragma unroll
+		for (int k = 0; k < 32; k += 8) {
+			const int r = r0 + ty + k, c = c0 + tx;
+			float v = 0.f;
+			if (r < Rout && c < C) {
+				const int rs = min(max(r - rshift, 0), R - 1);
+				v = ib[(long) rs * in_r + c];
+			}
+			tile[ty + k][tx] = v;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < 32; k += 8) {
+			const int c = c0 + ty + k, r = r0 + tx;
+			if (r < Rout && c < C)
+				ob[(long) c * out_c + r] = scale * tile[tx][ty + k];
+		}
+		__syncthreads();
+	}
+}
+
 __global__ void zero_u64_kernel(unsigned long long *p, int n)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1322,6 +1358,18 @@ hipError_t launch_pad(const void *src, void *dst, int ns, int nt, int na, int fm
 		hipLaunchKernelGGL(pair_pad_kernel, dim3(blocks), dim3(256), 0, s,
 			(const uint16_t *) src, (uint32_t *) dst, ns, nt, total);
 	}
+	return hipGetLastError();
+}
+
+hipError_t launch_transpose(const float *in, float *out, int B, int R, int C, long in_b,
+	long in_r, long out_b, long out_c, int Rout, int rshift, float scale, hipStream_t s)
+{
+	if (B <= 0 || R <= 0 || C <= 0 || Rout <= 0)
+		return hipSuccess;
+	const dim3 grid((unsigned) ((C + 31) / 32), (unsigned) ((Rout + 31) / 32),
+		(unsigned) min(B, 65535));
+	hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, in, out, B, R, C, in_b, in_r,
+		out_b, out_c, Rout, rshift, scale);
 	return hipGetLastError();
 }
 

@@ -95,6 +95,27 @@ def test_batch_argument_errors_before_device():
         == _lib.FDKHIP_EARG
 
 
+def test_cbct_back_entry_argument_errors_before_device():
+    import ctypes as C
+    from mirt_amd import _lib
+    from mirt_amd._lib import Geom
+    lib = _lib.load()
+    img = np.zeros(8, np.float32)
+    g = Geom(2, 2, 2, 1.0, -1.0, 1.0, 0, 0, 0, 541.0, 949.0, 0.0, 8, 4, 1.0, 1.0, 0.0, 0.5)
+    beta = np.zeros(2)
+    proj = np.zeros(64, np.float32)
+    P = lambda a: C.c_void_p(a.ctypes.data)
+    assert lib.fdkhip_cbct_back(None, P(proj), 2, P(beta), 1, None, 0, 1.0, P(img)) \
+        == _lib.FDKHIP_EARG
+    rc = lib.fdkhip_cbct_back(C.byref(g), P(proj), 2, P(beta), 1, None, 2, 1.0, P(img))
+    assert rc == _lib.FDKHIP_EARG
+    assert lib.fdkhip_last_error().decode() == "offset_t not yet done for extrapolation"
+    assert lib.fdkhip_cbct_back(C.byref(g), P(proj), 2, P(beta), 0, None, 0, 1.0, P(img)) \
+        == _lib.FDKHIP_EARG
+    assert lib.fdkhip_cbct_back(C.byref(g), P(proj), 2, None, 1, None, 0, 1.0, P(img)) \
+        == _lib.FDKHIP_EARG
+
+
 def test_quantize_argument_errors():
     from mirt_amd import _lib
     lib = _lib.load()
