@@ -275,6 +275,9 @@ def run_single(args, dev):
                              "stream); traffic = HBM bytes per launch from the "
                              "rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/"},
     }
+    if args.slabs and args.slabs > 1:
+        result["slabs"] = slab_times(args, geom, beta, mask, ig, tsa, head, cfg, "z")
+        result["strips"] = slab_times(args, geom, beta, mask, ig, tsa, head, cfg, "y")
     ref = images.get("f32_exact")
     if ref is not None:
         for name in images:
@@ -312,6 +315,49 @@ def run_single(args, dev):
         result["errors"] = column_errors(g, tsa_host, beta, m, imgs)
         log(f"cpu baseline + oracle errors in {time.perf_counter() - t0:.1f} s")
     return result
+
+
+def slab_times(args, geom, beta, mask, ig, tsa, head, cfg, axis):
+    """A G-way split of the volume for G GPUs, shown on this GPU: part g is
+    the same scan with nz/G slices (axis "z": z-slabs) or ny/G image rows
+    (axis "y": strips of columns), the image centre moved so that the part
+    covers slices / rows g*n/G ... (g+1)*n/G - 1 of the full volume.  Each part
+    is backprojected by its own plan with the full volume's launch shape; the
+    slowest part is what G GPUs would take (no reduction: parts concatenate).
+    """
+    import torch
+    from mirt_amd import FdkPlan
+    from mirt_amd._lib import Geom
+    G = args.slabs
+    fmt, mm = PATHS[head]
+    a = prepare(tsa, fmt)
+    n = ig.nz if axis == "z" else ig.ny
+    ns_ = n // G
+    rows = []
+    for gi in range(G):
+        gs = Geom.from_buffer_copy(geom)
+        msk = mask
+        if axis == "z":
+            gs.nz = ns_
+            gs.offset_z = geom.offset_z + (ig.nz - ns_) / 2.0 - gi * ns_
+        else:
+            # y is the image's second axis; dy < 0 flips nothing here: the
+            # part is rows gi*ns_ ... of the mask and of the volume
+            gs.ny = ns_
+            gs.offset_y = geom.offset_y + (ig.ny - ns_) / 2.0 - gi * ns_
+            msk = np.asarray(mask)[:, gi * ns_:(gi + 1) * ns_]
+        plan = FdkPlan(gs, beta, msk, fmt=fmt, math_mode=mm)
+        plan.set_config(cfg["waves"], cfg["zper"], cfg["vstage"])
+        out = torch.empty((gs.ny, gs.nx, gs.nz), dtype=torch.float32, device=tsa.device)
+        ms = plan.profile(*a, out=out, reps=max(2, args.steps))["back"]
+        rows.append({"part": gi, axis: [gi * ns_, (gi + 1) * ns_], "kernel_ms": ms,
+                     "updates_effective": plan.count_updates(), "launch": plan.config})
+        del plan, out
+    slowest = max(r["kernel_ms"] for r in rows)
+    log(f"{G} parts along {axis} of {ns_}: slowest {slowest:.3f} ms, "
+        f"sum {sum(r['kernel_ms'] for r in rows):.3f} ms")
+    return {"G": G, "axis": axis, "per_part": ns_, "slowest_kernel_ms": slowest,
+            "parts": rows}
 
 
 def run_batch(args, dev):
@@ -416,6 +462,8 @@ def main():
     ap.add_argument("--path", choices=list(PATHS), default=None,
                     help="storage/math path of the headline (default per workload)")
     ap.add_argument("--nscan", type=int, default=32, help="uneven_scans: number of scans")
+    ap.add_argument("--slabs", type=int, default=0, metavar="G",
+                    help="also time the G z-slabs a G-GPU split would give each GPU")
     ap.add_argument("--errors-scans", type=int, default=4,
                     help="uneven_scans: scans checked against the FP64 oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
