@@ -69,12 +69,12 @@ def oracle_geom(cg, ig):
                   cg.ns, cg.nt, (cg.ds, cg.dt), (cg.offset_s, cg.offset_t))
 
 
-def profiled_traffic(workload, fmt, cfg):
+def profiled_traffic(workload, fmt, cfg, kernel="back_kernel"):
     """HBM bytes per launch of the backprojection kernel from the committed
     rocprofv3 summary for this workload and path (FETCH_SIZE and WRITE_SIZE
     passes, scaled by the copy-kernel calibration), if the summary's launch
     shape (waves, slices per lane) matches the one timed here; else None."""
-    path = os.path.join(ROOT, "profiles", f"back_kernel_{workload}_{fmt}.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{fmt}.md")
     if not os.path.exists(path):
         return None
     text = open(path).read()
@@ -388,9 +388,13 @@ def run_batch(args, dev):
     batch = FdkBatch(list(zip(geoms, betas, masks)), fmt=fmt, math_mode=mm)
     outs = [torch.empty((sc.ig.ny, sc.ig.nx, sc.ig.nz), dtype=torch.float32, device=dev)
             for sc in scans]
-    t0 = time.perf_counter()
-    tuned = batch.autotune(projs, scales, outs, reps=2)
-    log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned:.3f} ms, config {batch.config}")
+    if args.launch:
+        batch.set_config(*args.launch)
+    elif not args.no_autotune:
+        t0 = time.perf_counter()
+        tuned = batch.autotune(projs, scales, outs, reps=2)
+        log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned:.3f} ms, "
+            f"config {batch.config}")
     ms = time_steps(lambda: batch(projs, scales, outs), args.steps, args.warmup)
     prof = batch.profile(projs, scales, outs, reps=max(2, args.steps))
     U = sum(sc.ig.nx * sc.ig.ny * sc.ig.nz * sc.cg.na for sc in scans)
@@ -398,16 +402,17 @@ def run_batch(args, dev):
     log(f"batch: {ms:.3f} ms/step, kernel {prof['back']:.3f} ms, {U / ms * 1e3:.3e} upd/s")
 
     # the same scans one plan after another (each replayed from its graph)
-    plans = []
-    for i in range(len(scans)):
-        plans.append(FdkPlan(geoms[i], betas[i], masks[i], fmt=fmt, math_mode=mm))
+    ms_seq = None
+    if not args.fmt_only:
+        plans = [FdkPlan(geoms[i], betas[i], masks[i], fmt=fmt, math_mode=mm)
+                 for i in range(len(scans))]
 
-    def sequential():
-        for i, p in enumerate(plans):
-            p(projs[i], scales[i] if scales else None, out=outs[i])
-    ms_seq = time_steps(sequential, args.steps, args.warmup)
-    log(f"sequential plans: {ms_seq:.3f} ms/step")
-    del plans
+        def sequential():
+            for i, p in enumerate(plans):
+                p(projs[i], scales[i] if scales else None, out=outs[i])
+        ms_seq = time_steps(sequential, args.steps, args.warmup)
+        log(f"sequential plans: {ms_seq:.3f} ms/step")
+        del plans
     batch(projs, scales, outs)
     torch.cuda.synchronize()
 
@@ -431,9 +436,11 @@ def run_batch(args, dev):
         "kernel_ms": prof,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None,
+                     "traffic": profiled_traffic("uneven_scans", head, batch.config,
+                                                 "batch_kernel"),
                      "note": f"{FLOP_PER_UPDATE} FP32 flop x {U_eff} in-support updates / "
-                             f"{kms:.3f} ms batch_kernel (HIP events)"},
+                             f"{kms:.3f} ms batch_kernel (HIP events); traffic from the "
+                             "rocprofv3 summary in profiles/"},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline and hosts:

@@ -8,26 +8,29 @@
 // explicit fmaf() (only in the FAST paths).
 //
 // Work decomposition of the main kernel (back_kernel):
-//   * a workgroup owns an 8x8 tile of (x,y) columns and a z-group of
-//     NW*ZL slices; wave w of the workgroup owns slices
-//     [zg0 + w*ZL, zg0 + (w+1)*ZL) of all 64 columns, lane = column, and keeps
-//     those ZL voxels in FP32 registers for the whole view loop;
-//   * views are processed in stages of `vstage` views:
-//       setup   -- one lane per (view, column) computes the reference's
-//                  per-column quantities (s bin, weights, z range, t_bin) in
-//                  the reference's float operation order, walks t_bin by
-//                  repeated addition exactly as the reference loop does and
-//                  records the value at each wave's first slice; a wave
-//                  reduction gives the view's detector footprint for the tile;
-//       stage   -- the footprint (zero-padded view, as fdk_ts_put_view does,
-//                  mex/src/fdk/fdk-orig/fdk,ts,t.c:13-21) is copied from HBM
-//                  into LDS, converted to FP32 (F16/BF16 times the view scale);
-//       update  -- every lane runs its ZL slices: bilinear interpolation from
-//                  LDS in registers, FP32 accumulation in registers;
+//   * a workgroup owns an 8x8 tile of (x,y) columns and a z-group of NW*ZL
+//     slices; consumer wave w owns slices [zg0 + w*ZL, zg0 + (w+1)*ZL) of
+//     all 64 columns, lane = column, and keeps those ZL voxels in FP32
+//     registers for the whole view loop; NP producer waves work ahead of
+//     them, one view per producer wave;
+//   * views go in stages of `vstage` views through two LDS buffers:
+//       setup   -- one lane per (view, column) computes the per-column
+//                  quantities (s bin, weights, slice range, t_bin), packs
+//                  them in a 32-byte record, and a DPP wave reduction gives
+//                  the view's detector footprint over the tile;
+//       stage   -- the footprint of the zero-padded view (the padded copy
+//                  plays fdk_ts_put_view, mex/src/fdk/fdk-orig/fdk,ts,t.c:13-21)
+//                  is copied from HBM into LDS with asynchronous
+//                  global-to-LDS loads (FP32 samples, or FP16/BF16 column
+//                  pairs);
+//       update  -- every consumer lane runs its ZL slices: bilinear
+//                  interpolation from LDS, FP32 accumulation in registers;
 //   * one write per voxel at the end, into the [nz nx ny] image.
-// The reference's t_bin sequence (t_bin += t_inc per slice) is reproduced
-// exactly, which together with the reference operation order makes the EXACT
-// mode bit-for-bit equal to the reference C on IEEE FP32.
+// Arithmetic modes: EXACT (FP32) reproduces the reference's t_bin sequence
+// (t_bin += t_inc per slice) and operation order, which makes it bit-for-bit
+// equal to the reference C; FAST (FP32) folds w2 into the weights and uses
+// FMAs; DIRECT (FP16/BF16 storage) evaluates t_bin in closed form and is held
+// to the FP64 result.
 //
 // No MFMA: the operation is a gather plus a few multiply-adds per voxel-view.
 
@@ -49,29 +52,10 @@ __device__ __forceinline__ int to_int_x86(float f)
 	return (f >= -2147483648.0f && f < 2147483648.0f) ? (int) f : (int) 0x80000000u;
 }
 
-__device__ __forceinline__ float bf16_to_f32(uint16_t h)
+// FP32 sample idx of the zero-padded copy of FP32 views
+__device__ __forceinline__ float load_sample(const void *padded, long idx)
 {
-	return __uint_as_float(((uint32_t) h) << 16);
-}
-
-template <int FMT>
-__device__ __forceinline__ const void *view_ptr(const void *proj, long elem)
-{
-	if constexpr (FMT == FMT_F32)
-		return static_cast<const float *>(proj) + elem;
-	else
-		return static_cast<const uint16_t *>(proj) + elem;
-}
-
-template <int FMT>
-__device__ __forceinline__ float load_sample(const void *proj, long idx)
-{
-	if constexpr (FMT == FMT_F32)
-		return static_cast<const float *>(proj)[idx];
-	else if constexpr (FMT == FMT_F16)
-		return __half2float(static_cast<const __half *>(proj)[idx]);
-	else
-		return bf16_to_f32(static_cast<const uint16_t *>(proj)[idx]);
+	return static_cast<const float *>(padded)[idx];
 }
 
 // Per-column, per-view quantities of fdk,ts.c:120-177 (axial branch,
@@ -508,32 +492,21 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 	}
 }
 
-// FP32 value of a staged LDS word
-template <int FMT>
-__device__ __forceinline__ float lds_sample(int w)
-{
-	if constexpr (FMT == FMT_F32)
-		return fdk_lds[w];
-	else if constexpr (FMT == FMT_F16)
-		return __half2float(__ushort_as_half((unsigned short) __float_as_uint(fdk_lds[w])));
-	else
-		return __uint_as_float(__float_as_uint(fdk_lds[w]) << 16);
-}
-
-// One slice update: bilinear interpolation at (is, t) from the two staged
-// columns c1 (s = is) and c2 (s = is+1), added to acc.  FAST folds w2 (and
-// the view scale) into wl/wr and uses fused multiply-adds; EXACT follows
+// One slice update of the FP32 paths: bilinear interpolation at (is, t) from
+// the two staged columns c1 (s = is) and c2 (s = is+1), added to acc.  FAST
+// folds w2 into wl/wr and uses fused multiply-adds; EXACT follows
 // fdk,ts.c:199-241 operation by operation.
 template <int FMT, bool FAST>
 __device__ __forceinline__ float slice_update(float acc, float t, int c1, int c2,
 	float wl, float wr, float w2)
 {
+	static_assert(FMT == FMT_F32, "FP16/BF16 views take slice_update_u");
 	const float fl = floorf(t);
 	const int it = (int) fl;
 	const float wu = t - fl;
 	const float wd = 1.f - wu;
-	const float a0 = lds_sample<FMT>(c1 + it), a1 = lds_sample<FMT>(c1 + it + 1);
-	const float b0 = lds_sample<FMT>(c2 + it), b1 = lds_sample<FMT>(c2 + it + 1);
+	const float a0 = fdk_lds[c1 + it], a1 = fdk_lds[c1 + it + 1];
+	const float b0 = fdk_lds[c2 + it], b1 = fdk_lds[c2 + it + 1];
 	if (FAST) {
 		const float q1 = fmaf(wr, b0, fmaf(wl, a0, 0.f));
 		const float q2 = fmaf(wr, b1, fmaf(wl, a1, 0.f));
@@ -725,10 +698,10 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 						const int it = (int) floorf(t);
 						const float wu = t - (float) it;
 						const float wd = 1.f - wu;
-						const float a0 = load_sample<FMT>(a.padded, c1 + it);
-						const float a1 = load_sample<FMT>(a.padded, c1 + it + 1);
-						const float b0 = load_sample<FMT>(a.padded, c2 + it);
-						const float b1 = load_sample<FMT>(a.padded, c2 + it + 1);
+						const float a0 = load_sample(a.padded, c1 + it);
+						const float a1 = load_sample(a.padded, c1 + it + 1);
+						const float b0 = load_sample(a.padded, c2 + it);
+						const float b1 = load_sample(a.padded, c2 + it + 1);
 						if (FAST) {
 							const float q1 = fmaf(wr, b0, wl * a0);
 							const float q2 = fmaf(wr, b1, wl * a1);
@@ -1202,18 +1175,13 @@ static hipError_t launch_back_t(const BackArgs &a, size_t lds_bytes, hipStream_t
 	return hipGetLastError();
 }
 
-// two stage buffers of: slots + 8 per-(view,column) arrays + T + vinfo (words)
+// two stage buffers of: slots + 8-word (view, column) records + the walk
+// modes' t_bin table + view headers (4-byte words)
 size_t back_lds_bytes(int nw, int vstage, int slot, int fmt)
 {
 	const size_t words = (size_t) vstage * slot + (size_t) vstage * 64 * 8
 		+ (fmt == FMT_F32 ? (size_t) vstage * nw * 64 : 0) + (size_t) vstage * 4;
 	return 2 * words * 4;
-}
-
-// producer waves per workgroup for a consumer count
-static int producers_for(int nw)
-{
-	return nw >= 8 ? 8 : 4;
 }
 
 #define FDK_DISPATCH_FMT(NW, ZL)                                                               \
@@ -1227,7 +1195,6 @@ static int producers_for(int nw)
 
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s)
 {
-	(void) producers_for;
 	const size_t lds = back_lds_bytes(nw, a.vstage, a.slot, fmt);
 	if (zl == 16) {
 		if (nw == 4) FDK_DISPATCH_FMT(4, 16);

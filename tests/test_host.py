@@ -396,3 +396,41 @@ def test_ellipsoid_proj_cone_chords():
                     dist2 = w @ w - (w @ d) ** 2
                     chord = 2 * val * math.sqrt(max(r * r - dist2, 0))
                     assert p[i, j, ia] == pytest.approx(chord, abs=1e-9)
+
+
+# --------------------------------------------------------------------------
+# workload definitions (the benchmark configurations)
+
+def test_workloads_match_the_benchmark_configurations():
+    from mirt_amd.workloads import WORKLOADS, uneven_scans
+    want = {   # ns, nt, na, ds, dt, arc, (nx, ny, nz), dx, dz, ellipsoids, inserts, I0
+        "cpu_small": (128, 96, 64, 1.0, 1.0, False, (64, 64, 32), 1.0, 1.0, 10, 0, None),
+        "plain_flat": (512, 384, 360, 0.8, 0.8, False, (256, 256, 128), 0.5, 0.5, 20, 0, 1e5),
+        "clinical_arc": (888, 64, 984, 1.0239, 1.0964, True, (512, 512, 256), 0.7, 0.625, 30,
+                         0, 2e5),
+        "large_flat": (1024, 768, 1440, 0.4, 0.4, False, (1024, 1024, 512), 0.25, 0.25, 40,
+                       12, 1e5),
+    }
+    for name, (ns, nt, na, ds, dt, arc, nxyz, dx, dz, nell, nins, i0) in want.items():
+        sc = WORKLOADS[name]()
+        cg, ig = sc.cg, sc.ig
+        assert (cg.ns, cg.nt, cg.na) == (ns, nt, na), name
+        assert (cg.ds, cg.dt, cg.dso, cg.dsd) == (ds, dt, 541.0, 949.0), name
+        assert (cg.dfs == 0) == arc and (arc or np.isinf(cg.dfs)), name
+        assert cg.orbit == 360 and (ig.nx, ig.ny, ig.nz) == nxyz, name
+        assert (ig.dx, ig.dy, ig.dz) == (dx, -dx, dz), name
+        assert len(sc.ellipsoids) == nell + nins and sc.i0 == i0, name
+        mu = sc.ellipsoids[:nell, 8]
+        assert np.all((mu >= 0.005) & (mu <= 0.02)), name
+        assert np.all(sc.ellipsoids[nell:, 8] == 0.05), name
+    assert WORKLOADS["clinical_arc"]().cg.offset_s == 1.25
+    scans = uneven_scans(32)
+    assert len(scans) == 32
+    for sc in scans:
+        e = sc.ig.nx
+        assert e in (128, 192, 256, 384) and sc.ig.ny == e and sc.ig.nz == e // 2
+        assert 180 <= sc.cg.na <= 720 and 256 <= sc.cg.ns <= 768 and 128 <= sc.cg.nt <= 512
+        assert 10 <= len(sc.ellipsoids) <= 30 and np.isinf(sc.cg.dfs)
+    # seeded: the same scans every time
+    again = uneven_scans(32)
+    assert all(np.array_equal(a.ellipsoids, b.ellipsoids) for a, b in zip(scans, again))
