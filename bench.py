@@ -317,6 +317,106 @@ def run_single(args, dev):
     return result
 
 
+# per in-range voxel-view update of cbct_back_mat, in MATLAB's expression
+# order: tprime (sub, mul), bt (div, add), wu, wd, p1 (2 mul + add), p2 (same),
+# p0 (2 mul + add), the weight (1 mul flat / mul + div arc), img2 += p0
+MAT_FLOP_PER_UPDATE = {"flat": 17, "arc": 18}
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector peak, AMD spec
+
+
+def mat_cpu_rate(g, proj_sta, beta, mask, target_s=15.0):
+    """The oracle's numpy restatement of cbct_back_mat (one thread) on a slab
+    of slices and a sample of views: voxel-view updates/s."""
+    from oracle import oracle as O
+
+    def slab(nzs):
+        gs = O.geom((g.nx, g.ny, nzs), (g.dx, g.dy, g.dz), (g.offset_x, g.offset_y, 0.0),
+                    g.dso, g.dsd, g.dfs, g.ns, g.nt, (g.ds, g.dt), (g.offset_s, g.offset_t))
+        return gs
+
+    t0 = time.perf_counter()
+    O.back_mat_f64(slab(4), proj_sta[:, :, :1], beta[:1], mask=mask)
+    per = (time.perf_counter() - t0) / 4                    # s per slice-view
+    nzs = int(max(4, min(g.nz, target_s / per / 4)))
+    nv = int(max(1, min(proj_sta.shape[2], target_s / per / nzs)))
+    views = np.linspace(0, proj_sta.shape[2] - 1, nv).astype(int)
+    t0 = time.perf_counter()
+    O.back_mat_f64(slab(nzs), proj_sta[:, :, views], beta[views], mask=mask)
+    t = time.perf_counter() - t0
+    return {"value": g.nx * g.ny * nzs * nv / t, "unit": "voxel-view updates/s",
+            "cores": 1, "kind": "port",
+            "sample": f"numpy restatement of cbct_back_mat (oracle.back_mat_f64, one thread) "
+                      f"on {nv} of {proj_sta.shape[2]} views x a {g.nx}x{g.ny}x{nzs} slab, "
+                      f"{t:.1f} s; host CPU {cpu_model()}",
+            "seconds": t}
+
+
+def run_mat(args, dev):
+    """cbct_back_mat (use_mex = 0) on the workload's filtered views: the FP64
+    MATLAB-rule backprojector, views [ns nt na] float32 and the float64 image
+    resident on the device (fdkhip_cbct_back_mat_direct)."""
+    import ctypes as C
+    import torch
+    from mirt_amd import _lib, geom_of
+    from mirt_amd.cbct_back import _views
+    from mirt_amd.workloads import WORKLOADS, filtered_views
+
+    lib = _lib.load()
+    scan = WORKLOADS[args.workload]()
+    cg, ig = scan.cg, scan.ig
+    tsa = filtered_views(scan, dev)                       # (na, ns, nt)
+    sta = tsa.permute(0, 2, 1).contiguous()               # (na, nt, ns) = [ns nt na]
+    del tsa
+    beta = np.ascontiguousarray(_views(cg, cg.na), np.float64)
+    mask = np.asfortranarray(ig.mask_or != 0).astype(np.uint8, order="F")
+    img = torch.empty((ig.nz, ig.ny, ig.nx), dtype=torch.float64, device=dev)
+    geom = geom_of(cg, ig)
+    s = torch.cuda.current_stream()
+    kms = []
+
+    def step():
+        ms = C.c_float(0)
+        _lib.check(lib.fdkhip_cbct_back_mat_direct(
+            C.byref(geom), C.c_void_p(sta.data_ptr()), 0, cg.na, beta.ctypes.data_as(C.c_void_p),
+            None, 1, 0.0, mask.ctypes.data_as(C.c_void_p), 1.0, C.c_void_p(img.data_ptr()),
+            C.c_void_p(s.cuda_stream), C.byref(ms)))
+        kms.append(ms.value)
+
+    ms = time_steps(step, args.steps, args.warmup)
+    kernel_ms = float(np.mean(kms[args.warmup:]))
+    U = ig.nx * ig.ny * ig.nz * cg.na
+    nmask = int(mask.sum())
+    U_mask = nmask * ig.nz * cg.na                        # cbct_back_mat visits masked voxels
+    det = "arc" if cg.dfs == 0 else "flat"
+    achieved = MAT_FLOP_PER_UPDATE[det] * U_mask / (kernel_ms * 1e-3) / 1e12
+    log(f"mat: {ms:.3f} ms/step, kernel {kernel_ms:.3f} ms, {U / ms * 1e3:.3e} updates/s")
+    result = {
+        "metric": f"voxel-view updates/s (GUPS), {args.workload}, cbct_back_mat "
+                  "(use_mex=0, FP64 MATLAB backprojector)",
+        "value": U / (ms * 1e-3), "unit": "voxel-view updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": f"synthetic: {args.workload} FDK-filtered phantom views, float32 [ns nt na]",
+        "config": {"workload": f"{args.workload} through cbct_back_mat: {det} "
+                               f"{cg.ns}x{cg.nt} det, {cg.na} views -> "
+                               f"{ig.nx}x{ig.ny}x{ig.nz}, every voxel x every view",
+                   "nx": ig.nx, "ny": ig.ny, "nz": ig.nz, "na": cg.na},
+        "kernel_ms": kernel_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": None,
+                     "note": f"{MAT_FLOP_PER_UPDATE[det]} FP64 flop x {U_mask} masked "
+                             f"voxel-view updates / {kernel_ms:.3f} ms mat_kernel (HIP "
+                             "events on its stream)"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        g = oracle_geom(cg, ig)
+        proj_sta = sta.cpu().numpy().transpose(2, 1, 0)   # (ns, nt, na) view
+        result["cpu_baseline"] = mat_cpu_rate(g, proj_sta, beta, ig.mask_or)
+    return result
+
+
 def slab_times(args, geom, beta, mask, ig, tsa, head, cfg, axis):
     """A G-way split of the volume for G GPUs, shown on this GPU: part g is
     the same scan with nz/G slices (axis "z": z-slabs) or ny/G image rows
@@ -466,8 +566,9 @@ def main():
     ap.add_argument("--workload", default="clinical_arc",
                     choices=["clinical_arc", "plain_flat", "large_flat", "cpu_small",
                              "uneven_scans"])
-    ap.add_argument("--path", choices=list(PATHS), default=None,
-                    help="storage/math path of the headline (default per workload)")
+    ap.add_argument("--path", choices=list(PATHS) + ["mat"], default=None,
+                    help="storage/math path of the headline (default per workload); "
+                         "mat: cbct_back_mat, the FP64 use_mex=0 route")
     ap.add_argument("--nscan", type=int, default=32, help="uneven_scans: number of scans")
     ap.add_argument("--slabs", type=int, default=0, metavar="G",
                     help="also time the G z-slabs a G-GPU split would give each GPU")
@@ -487,8 +588,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU z-slab sharding is described in DESIGN.md, not built)",
-              file=sys.stderr)
+              "(a G-GPU split into column strips is timed part by part on one GPU with "
+              "--slabs G)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -502,7 +603,9 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 4,
                           "bytes_written": a.numel() * 4, "repeats": 3}), flush=True)
         return
-    if args.workload == "uneven_scans":
+    if args.path == "mat":
+        result = run_mat(args, dev)
+    elif args.workload == "uneven_scans":
         result = run_batch(args, dev)
     else:
         result = run_single(args, dev)

@@ -128,6 +128,38 @@ int fdkhip_fdk_st_back(
 	float *image);
 
 /*
+ * cbct_back_mat (fbp/cbct_back.m:142-270), the use_mex = 0 route of cbct_back:
+ * the reference's MATLAB backprojector, in double precision, with its own
+ * edge rules -- a sample position outside detector columns [1, ns] adds 0,
+ * detector rows are clamped to [1, nt] (replication, always on) -- and its
+ * weights (mag^2 for flat and parallel, dsd^2 / (r_loop^2 + y_betas^2) for
+ * the arc, r_loop = x_beta - offset_source).  proj_sta is [ns nt na] (s
+ * fastest), float (proj_fp64 = 0) or double (1), as MATLAB's proj(:,:,ia)
+ * enters double arithmetic; views 1:ia_skip:na with angles beta[ia] and
+ * source heights source_zs[ia] (NULL = 0; the helix of cbct_back.m:152-154
+ * and 219).  image_xyz [nx ny nz] double = scale * embed(sum, mask) (scale is
+ * 0.5 * deg2rad(|orbit|) / (na / ia_skip), or 1 without scale_dang,
+ * cbct_back.m:264-266).  mask may be NULL.  Unlike MATLAB, a NaN sample
+ * position adds 0 instead of failing on the index.  Synchronous.
+ */
+int fdkhip_cbct_back_mat(const fdkhip_geom *g, const void *proj_sta, int32_t proj_fp64,
+	int32_t na, const double *beta, const double *source_zs, int32_t ia_skip,
+	double offset_source, const uint8_t *mask, double scale, double *image_xyz);
+
+/*
+ * fdkhip_cbct_back_mat with proj_dev (all na views, [ns nt na]) and image_dev
+ * ([nx ny nz] double) in device memory; beta, source_zs and mask stay host
+ * pointers.  Runs on `stream` and returns when the kernel has finished;
+ * kernel_ms (may be NULL) receives the kernel's time from HIP events recorded
+ * on `stream`.  No counterpart in the reference (device-resident form of
+ * cbct_back_mat for callers that keep views on the GPU).
+ */
+int fdkhip_cbct_back_mat_direct(const fdkhip_geom *g, const void *proj_dev,
+	int32_t proj_fp64, int32_t na, const double *beta, const double *source_zs,
+	int32_t ia_skip, double offset_source, const uint8_t *mask, double scale,
+	double *image_dev, void *stream, float *kernel_ms);
+
+/*
  * Device-resident 'fdk,ts,back': a plan holds the per-view tables, the
  * device mask and the launch configuration for one geometry and one view
  * list, so repeated backprojections of new projections cost one launch

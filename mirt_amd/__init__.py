@@ -7,10 +7,11 @@ host side around it: ``ct_geom`` / ``image_geom`` conventions,
 steps of ``feldkamp``.
 """
 from .geom import CTGeom, ImageGeom, ct_geom, image_geom
-from .cbct_back import (FdkBatch, FdkMexError, FdkPlan, angle_scale, cbct_back, fdk_mex,
-                        geom_of, quantize_views)
+from .cbct_back import (FdkBatch, FdkMexError, FdkPlan, angle_scale, angle_scale64,
+                        cbct_back, cbct_back_mat, fdk_mex, geom_of, quantize_views)
 from ._lib import FdkhipError
 
 __all__ = ["CTGeom", "ImageGeom", "ct_geom", "image_geom", "FdkMexError",
-           "FdkBatch", "FdkPlan", "FdkhipError", "angle_scale", "cbct_back", "fdk_mex",
+           "FdkBatch", "FdkPlan", "FdkhipError", "angle_scale", "angle_scale64", "cbct_back",
+           "cbct_back_mat", "fdk_mex",
            "geom_of", "quantize_views"]

@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 
 import numpy as np
 
@@ -152,9 +153,16 @@ def _views(cg: CTGeom, na: int):
     return deg2rad(np.asarray(cg.orbit_start, dtype=np.float64) + cg.orbit * k / na)
 
 
+def angle_scale64(cg: CTGeom, ia_skip: int = 1) -> float:
+    """The "\\der angle" scale 0.5 * deg2rad(|orbit|) / (na/ia_skip) in double
+    (cbct_back.m:133-135 and 264-266)."""
+    return 0.5 * float(deg2rad(abs(cg.orbit))) / (cg.na / ia_skip)
+
+
 def angle_scale(cg: CTGeom, ia_skip: int = 1) -> np.float32:
-    """cbct_back.m:133-135: (0.5 * deg2rad(|orbit|) / (na/ia_skip)), in single."""
-    return np.float32(0.5 * float(deg2rad(abs(cg.orbit))) / (cg.na / ia_skip))
+    """cbct_back.m:133-135: the scale as it multiplies the single image of
+    the mex route, i.e. narrowed to single."""
+    return np.float32(angle_scale64(cg, ia_skip))
 
 
 def cbct_back(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: int = 1,
@@ -166,8 +174,10 @@ def cbct_back(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: int = 1
     backprojection ``[nx ny nz]`` as a float32 numpy array.  ``use_mex`` 1
     (default) calls ``back_call`` once with all views, 2 once per view and
     sums, 3 uses the ``'fdk,st,back'`` order; ``back_call`` defaults to
-    :func:`fdk_mex` (the GPU gateway).  ``use_mex=0`` (the reference's MATLAB
-    backprojector ``cbct_back_mat``) is not provided by this build.
+    :func:`fdk_mex` (the GPU gateway).  ``use_mex=0`` is the reference's MATLAB
+    backprojector ``cbct_back_mat`` (cbct_back.m:142-270): double precision,
+    its own edge rules, helical ``source_zs`` accepted; it returns a float64
+    ``[nx ny nz]`` array and runs on the GPU through ``fdkhip_cbct_back_mat``.
     """
     if use_mex is None:
         use_mex = 1
@@ -177,9 +187,8 @@ def cbct_back(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: int = 1
     if back_call is None:
         back_call = fdk_mex
     if not use_mex:
-        raise NotImplementedError(
-            "use_mex=0 (cbct_back_mat, the MATLAB backprojector) is not provided; "
-            "use_mex=1 computes the same backprojection on the GPU")
+        return cbct_back_mat(proj, cg, ig, ia_skip=ia_skip, offset_source=offset_source,
+                             scale_dang=scale_dang, extrapolate_t=extrapolate_t)
     if cg.pitch != 0 or np.any(cg.source_zs != 0):
         raise ValueError("helix not yet supported")
     del offset_source   # accepted as in cbct_back.m; the mex path ignores it
@@ -255,6 +264,45 @@ def _cbct_back_device(proj, cg, ig, betas, ia_skip, extrapolate_t, scale_dang):
     _lib.check(lib.fdkhip_cbct_back(C.byref(geom), _ptr(proj), int(cg.na), _ptr(betas),
                                     int(ia_skip), _ptr(mask), int(extrapolate_t), scale,
                                     _ptr(img)))
+    return img
+
+
+def cbct_back_mat(proj, cg: CTGeom, ig: ImageGeom, *, ia_skip: int = 1,
+                  offset_source: float = 0.0, scale_dang: bool = True,
+                  extrapolate_t=0):
+    """``cbct_back_mat`` (fbp/cbct_back.m:142-270) on the GPU.
+
+    ``proj`` is ``[ns nt na]`` (or ``[ns nt]`` when na = 1); float32 views stay
+    float32 on the way in, anything else enters as float64 (MATLAB's
+    ``proj1(1:ns,:) = proj(:,:,ia)`` converts to double).  Views 1:ia_skip:na
+    at the angles of cbct_back.m:157 and the heights ``cg.source_zs``; returns
+    the float64 ``[nx ny nz]`` image (F-ordered), zero outside ``ig.mask_or``.
+    """
+    if not extrapolate_t:
+        warnings.warn("mat version always extrapolates in t by replication", stacklevel=2)
+    zs = np.ascontiguousarray(np.broadcast_to(np.asarray(cg.source_zs, np.float64),
+                                              (cg.na,)))
+    if np.any(zs != 0):
+        warnings.warn("helix not yet tested", stacklevel=2)
+    ia_skip = int(ia_skip)
+    if ia_skip < 1:
+        raise ValueError("ia_skip must be >= 1")
+    ns, nt, na = cg.ns, cg.nt, cg.na
+    proj = np.asarray(proj)
+    if proj.ndim == 2:
+        proj = proj[:, :, None]
+    if proj.shape != (ns, nt, na):
+        raise ValueError(f"proj shape {proj.shape} != [ns nt na] = {(ns, nt, na)}")
+    fp64 = proj.dtype != np.float32
+    proj = np.asfortranarray(proj, dtype=np.float64 if fp64 else np.float32)
+    betas = np.ascontiguousarray(_views(cg, na), dtype=np.float64)
+    mask = np.asfortranarray(ig.mask_or != 0).astype(np.uint8, order="F")
+    img = np.empty((ig.nx, ig.ny, ig.nz), dtype=np.float64, order="F")
+    scale = angle_scale64(cg, ia_skip) if scale_dang else 1.0
+    lib = _lib.load()
+    _lib.check(lib.fdkhip_cbct_back_mat(C.byref(geom_of(cg, ig)), _ptr(proj), int(fp64), na,
+                                        _ptr(betas), _ptr(zs), ia_skip, float(offset_source),
+                                        _ptr(mask), float(scale), _ptr(img)))
     return img
 
 
@@ -546,5 +594,5 @@ def quantize_views(proj, fmt="f16", *, stream=None):
     return q, scale
 
 
-__all__ = ["fdk_mex", "cbct_back", "FdkPlan", "FdkBatch", "FdkMexError", "FdkhipError",
-           "quantize_views", "geom_of", "angle_scale"]
+__all__ = ["fdk_mex", "cbct_back", "cbct_back_mat", "FdkPlan", "FdkBatch", "FdkMexError",
+           "FdkhipError", "quantize_views", "geom_of", "angle_scale", "angle_scale64"]

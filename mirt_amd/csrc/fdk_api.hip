@@ -806,6 +806,186 @@ extern "C" int fdkhip_fdk_st_back(
 }
 
 // ---------------------------------------------------------------------------
+// cbct_back_mat (fbp/cbct_back.m:142-270): the MATLAB backprojector
+
+// Argument checks of both mat entry points; fills the kernel arguments other
+// than the device pointers.  MATLAB itself fails (an undefined sprime, or a
+// row index 0) where this returns FDKHIP_EARG for dfs and nt.
+static int mat_args(const fdkhip_geom *g, int na, const double *beta, int ia_skip,
+	double offset_source, double scale, MatArgs &a)
+{
+	int rc = check_geom(g);
+	if (rc)
+		return rc;
+	if (na < 0)
+		return fail(FDKHIP_EARG, "na must be >= 0 (na=%d)", na);
+	if (ia_skip < 1)
+		return fail(FDKHIP_EARG, "ia_skip must be >= 1 (ia_skip=%d)", ia_skip);
+	if (na > 0 && !beta)
+		return fail(FDKHIP_EARG, "beta is NULL");
+	const bool par = isinf(g->dsd) || isinf(g->dso);
+	if (!par && !isinf(g->dfs) && g->dfs != 0)
+		return fail(FDKHIP_EARG, "cbct_back_mat: dfs must be 0 (arc) or inf (flat), got %g",
+			g->dfs);
+	const long long nvox = (long long) g->nx * g->ny * g->nz;
+	if (nvox > 0 && na > 0 && (g->ns < 1 || g->nt < 2))
+		return fail(FDKHIP_EARG, "cbct_back_mat needs ns >= 1 and nt >= 2 (ns=%d nt=%d)",
+			g->ns, g->nt);
+	memset(&a, 0, sizeof a);
+	a.nx = g->nx; a.ny = g->ny; a.nz = g->nz;
+	a.ns = g->ns; a.nt = g->nt;
+	a.nv = (na + ia_skip - 1) / ia_skip;           // views 1:ia_skip:na
+	a.nzc = (g->nz + MAT_ZC - 1) / MAT_ZC;
+	a.kind = par ? KIND_PAR : (g->dfs == 0 ? KIND_ARC : KIND_FLAT);
+	a.dx = g->dx; a.dy = g->dy; a.dz = g->dz;
+	// cbct_back.m:160-162, 185-186
+	a.wx = (g->nx - 1) / 2. + g->offset_x;
+	a.wy = (g->ny - 1) / 2. + g->offset_y;
+	a.wz = (g->nz - 1) / 2. + g->offset_z;
+	a.ws = (g->ns + 1) / 2. + g->offset_s;
+	a.wt = (g->nt + 1) / 2. + g->offset_t;
+	a.dso = g->dso; a.dsd = g->dsd; a.ds = g->ds; a.dt = g->dt;
+	a.offset_source = offset_source;
+	a.scale = scale;
+	return FDKHIP_OK;
+}
+
+// [nv] {cos, sin, source_zs, 0} of views 0, ia_skip, 2 ia_skip, ...
+static std::vector<double4> mat_views(const double *beta, const double *source_zs, int nv,
+	int ia_skip)
+{
+	std::vector<double4> t((size_t) nv);
+	for (int i = 0; i < nv; ++i) {
+		const size_t ia = (size_t) i * ia_skip;
+		t[i].x = cos(beta[ia]);
+		t[i].y = sin(beta[ia]);
+		t[i].z = source_zs ? source_zs[ia] : 0.0;
+		t[i].w = 0.0;
+	}
+	return t;
+}
+
+// Upload the view table and the mask, launch, wait.  proj and image are
+// device pointers; kernel_ms (optional) receives the kernel's time from HIP
+// events recorded on `s` around the launch.
+static int mat_run(MatArgs &a, const double *beta, const double *source_zs, int ia_skip,
+	const uint8_t *mask_host, bool fp64, hipStream_t s, float *kernel_ms)
+{
+	const size_t nxy = (size_t) a.nx * a.ny;
+	std::vector<double4> views = mat_views(beta, source_zs, a.nv, ia_skip);
+	DevBuf dviews, dmask;
+	hipError_t e = hipSuccess;
+	if (a.nv > 0)
+		e = hipMalloc(&dviews.p, views.size() * sizeof(double4));
+	if (e == hipSuccess && mask_host && nxy > 0)
+		e = hipMalloc(&dmask.p, nxy);
+	if (e != hipSuccess)
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	if (a.nv > 0)
+		HIP_TRY(hipMemcpyAsync(dviews.p, views.data(), views.size() * sizeof(double4),
+			hipMemcpyHostToDevice, s));
+	if (dmask.p)
+		HIP_TRY(hipMemcpyAsync(dmask.p, mask_host, nxy, hipMemcpyHostToDevice, s));
+	a.views = (const double4 *) dviews.p;
+	a.mask = (const uint8_t *) dmask.p;
+	hipEvent_t ev[2] = {nullptr, nullptr};
+	if (kernel_ms) {
+		HIP_TRY(hipEventCreate(&ev[0]));
+		e = hipEventCreate(&ev[1]);
+		if (e != hipSuccess) {
+			(void) hipEventDestroy(ev[0]);
+			return fail(FDKHIP_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+		}
+		e = hipEventRecord(ev[0], s);
+	}
+	if (e == hipSuccess)
+		e = launch_mat(a, fp64, s);
+	if (e == hipSuccess && kernel_ms)
+		e = hipEventRecord(ev[1], s);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(s);
+	if (e == hipSuccess && kernel_ms)
+		e = hipEventElapsedTime(kernel_ms, ev[0], ev[1]);
+	if (kernel_ms) {
+		(void) hipEventDestroy(ev[0]);
+		(void) hipEventDestroy(ev[1]);
+	}
+	if (e != hipSuccess)
+		return fail(FDKHIP_EHIP, "cbct_back_mat launch: %s", hipGetErrorString(e));
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_cbct_back_mat(const fdkhip_geom *g, const void *proj_sta,
+	int32_t proj_fp64, int32_t na, const double *beta, const double *source_zs,
+	int32_t ia_skip, double offset_source, const uint8_t *mask, double scale,
+	double *image_xyz)
+{
+	MatArgs a;
+	int rc = mat_args(g, na, beta, ia_skip, offset_source, scale, a);
+	if (rc)
+		return rc;
+	const size_t nvox = (size_t) g->nx * g->ny * g->nz;
+	if (nvox == 0)
+		return FDKHIP_OK;
+	if (!image_xyz)
+		return fail(FDKHIP_EARG, "image is NULL");
+	const size_t view = (size_t) g->ns * g->nt;
+	const size_t esz = proj_fp64 ? sizeof(double) : sizeof(float);
+	if (a.nv > 0 && view > 0 && !proj_sta)
+		return fail(FDKHIP_EARG, "proj is NULL");
+	DevBuf dproj, dimg;
+	hipError_t e = hipMalloc(&dimg.p, nvox * sizeof(double));
+	if (e == hipSuccess && a.nv > 0 && view > 0)
+		e = hipMalloc(&dproj.p, view * a.nv * esz);
+	if (e != hipSuccess)
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	if (dproj.p)   // views 0, ia_skip, 2 ia_skip, ... in one strided copy
+		HIP_TRY(hipMemcpy2D(dproj.p, view * esz, proj_sta, view * ia_skip * esz, view * esz,
+			a.nv, hipMemcpyHostToDevice));
+	a.proj = dproj.p;
+	a.view_stride = (long) view;
+	a.image = (double *) dimg.p;
+	if (a.nv == 0)   // img = zeros: no view adds anything (cbct_back.m:189)
+		HIP_TRY(hipMemset(dimg.p, 0, nvox * sizeof(double)));
+	else if ((rc = mat_run(a, beta, source_zs, ia_skip, mask, proj_fp64 != 0, nullptr,
+			nullptr)))
+		return rc;
+	HIP_TRY(hipMemcpy(image_xyz, dimg.p, nvox * sizeof(double), hipMemcpyDeviceToHost));
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_cbct_back_mat_direct(const fdkhip_geom *g, const void *proj_dev,
+	int32_t proj_fp64, int32_t na, const double *beta, const double *source_zs,
+	int32_t ia_skip, double offset_source, const uint8_t *mask, double scale,
+	double *image_dev, void *stream, float *kernel_ms)
+{
+	MatArgs a;
+	int rc = mat_args(g, na, beta, ia_skip, offset_source, scale, a);
+	if (rc)
+		return rc;
+	if (kernel_ms)
+		*kernel_ms = 0.f;
+	const size_t nvox = (size_t) g->nx * g->ny * g->nz;
+	if (nvox == 0)
+		return FDKHIP_OK;
+	if (!image_dev)
+		return fail(FDKHIP_EARG, "image is NULL");
+	const size_t view = (size_t) g->ns * g->nt;
+	if (a.nv > 0 && view > 0 && !proj_dev)
+		return fail(FDKHIP_EARG, "proj is NULL");
+	hipStream_t s = (hipStream_t) stream;
+	a.proj = proj_dev;
+	a.view_stride = (long) (view * ia_skip);
+	a.image = image_dev;
+	if (a.nv == 0) {
+		HIP_TRY(hipMemsetAsync(image_dev, 0, nvox * sizeof(double), s));
+		HIP_TRY(hipStreamSynchronize(s));
+		return FDKHIP_OK;
+	}
+	return mat_run(a, beta, source_zs, ia_skip, mask, proj_fp64 != 0, s, kernel_ms);
+}
+
+// ---------------------------------------------------------------------------
 // autotuning: launch shape and views per LDS stage, chosen by timing
 
 namespace {

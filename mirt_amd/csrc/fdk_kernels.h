@@ -55,6 +55,20 @@ struct StArgs {
 	float *image;            // [nz][ny][nx]
 };
 
+// cbct_back_mat (fbp/cbct_back.m:142-270): the MATLAB backprojector, all in
+// double as MATLAB holds it.  ws and wt are 1-based (cbct_back.m:185-186).
+constexpr int MAT_ZC = 16;   // slices per lane
+struct MatArgs {
+	int nx, ny, nz, ns, nt, nv, nzc, kind;   // nv views used, nzc = ceil(nz / MAT_ZC)
+	double dx, dy, dz, wx, wy, wz, ws, wt;
+	double dso, dsd, ds, dt, offset_source, scale;
+	const double4 *views;    // [nv] {cos beta, sin beta, source_zs, 0}
+	const void *proj;        // views of [nt][ns] (s fastest), float or double
+	long view_stride;        // elements from one used view to the next
+	const uint8_t *mask;     // [ny][nx] or NULL
+	double *image;           // [nz][ny][nx]
+};
+
 size_t back_lds_bytes(int nw, int vstage, int slot, int fmt);
 bool back_config_supported(int nw, int zl);
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s);
@@ -74,5 +88,6 @@ hipError_t launch_zero_i32(int *p, int n, hipStream_t s);
 hipError_t launch_st(const StArgs &a, hipStream_t s);
 hipError_t launch_quantize(const float *proj, int ns, int nt, int na, int fmt,
 	void *out, float *scale, int *maxbits, hipStream_t s);
+hipError_t launch_mat(const MatArgs &a, bool proj_fp64, hipStream_t s);
 
 } // namespace fdk

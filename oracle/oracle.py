@@ -12,7 +12,22 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
   ``mex/src/fdk/fdk-orig`` (as shipped), driven by ``ref_driver.c`` the way
   ``fdk_mex_back`` (``mex/src/fdk/fdk-orig/fdk,mex.c:22-171``) drives them.
 
-Array conventions follow the reference mex call
+and one numpy function of its own:
+
+* :func:`back_mat_f64` -- restatement of ``cbct_back_mat``
+  (``fbp/cbct_back.m:142-270``), the reference's MATLAB backprojector, step
+  by step and vectorised over the masked pixels as the MATLAB code is, in
+  double precision with its edge rules.  Pinned in ``tests/test_oracle.py``
+  against the reference C (views with zero edge columns and replicated edge
+  rows, where the two backprojectors coincide -- the comparison the
+  reference's own ``cbct_back_test`` makes, ``cbct_back.m:335-364``), against
+  the reference C's one-bin fall-off band at the detector's s edges (which
+  ``cbct_back_mat`` must zero), and against closed forms for linear views on
+  the parallel geometry (the 1-based ``ws``/``wt``, the t clamp, the hard s
+  cut).  ``offset_source`` and ``source_zs`` are pinned through shifts of the
+  image offsets that the reference C computes.
+
+Array conventions of the compiled functions follow the reference mex call
 ``fdk_mex('fdk,ts,back', nxyz, dxyz, cxyz, mask, dso, dsd, dfs, [ds dt],
 [offset_s offset_t], proj, beta, nthread)``: ``proj`` is ``[nt ns na]`` with t
 fastest (a C-order numpy array of shape ``(na, ns, nt)``), the result is
@@ -22,6 +37,7 @@ fastest (a C-order numpy array of shape ``(na, ns, nt)``), the result is
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 import time
@@ -212,3 +228,83 @@ def rel_l2(a, b) -> float:
     b = np.asarray(b, np.float64)
     den = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))
+
+
+def back_mat_f64(g: OrcGeom, proj_sta, beta, *, source_zs=None, ia_skip=1,
+                 offset_source=0.0, mask=None, scale=1.0):
+    """numpy restatement of cbct_back_mat (fbp/cbct_back.m:142-270).
+
+    ``proj_sta`` has shape ``(ns, nt, na)`` and ``mask`` ``(nx, ny)`` (MATLAB
+    index order, any memory order); returns the float64 ``(nx, ny, nz)`` image
+    ``scale * embed(img2, mask)``.  Every step is the MATLAB expression's own
+    double operation in its order (x.^2 as x*x); cos/sin of each angle come
+    from libm (``math.cos``), as the product's host computes them.  One
+    deviation, shared with the product: a NaN sample position adds 0 where
+    MATLAB would fail on the index.
+    """
+    nx, ny, nz, ns, nt = g.nx, g.ny, g.nz, g.ns, g.nt
+    proj = np.asarray(proj_sta, dtype=np.float64)
+    if proj.ndim == 2:
+        proj = proj[:, :, None]
+    na = proj.shape[2]
+    if proj.shape[:2] != (ns, nt):
+        raise ValueError("proj must be (ns, nt, na)")
+    beta = np.asarray(beta, dtype=np.float64).reshape(-1)
+    zs = np.zeros(na) if source_zs is None else \
+        np.broadcast_to(np.asarray(source_zs, np.float64), (na,))
+    par = math.isinf(g.dsd) or math.isinf(g.dso)
+    flat = par or math.isinf(g.dfs)                  # cbct_back.m:211-216
+    if not flat and g.dfs != 0:
+        raise ValueError("dfs must be 0 (arc) or inf (flat)")
+    m = np.ones((nx, ny), bool) if mask is None else (np.asarray(mask) != 0)
+    # cbct_back.m:160-165
+    wx = (nx - 1) / 2 + g.offset_x
+    wy = (ny - 1) / 2 + g.offset_y
+    wz = (nz - 1) / 2 + g.offset_z
+    xc, yc = np.meshgrid((np.arange(nx) - wx) * g.dx, (np.arange(ny) - wy) * g.dy,
+                         indexing="ij")
+    zc = (np.arange(nz) - wz) * g.dz
+    xc = xc[m]                                       # cbct_back.m:182-183
+    yc = yc[m]
+    ws = (ns + 1) / 2 + g.offset_s                   # cbct_back.m:185-186
+    wt = (nt + 1) / 2 + g.offset_t
+    proj1 = np.zeros((ns + 1, nt))                   # cbct_back.m:190-191
+    img2 = np.zeros((xc.size, nz))
+    for ia in range(0, na, int(ia_skip)):           # cbct_back.m:201
+        cb, sb = math.cos(beta[ia]), math.sin(beta[ia])
+        x_beta = xc * cb + yc * sb                   # cbct_back.m:204-205
+        y_betas = g.dso - (-xc * sb + yc * cb)
+        mag = np.ones_like(y_betas) if par else g.dsd / y_betas
+        if flat:
+            sprime = mag * x_beta
+        else:
+            r_loop = x_beta - offset_source
+            sprime = g.dsd * np.arctan2(r_loop, y_betas)
+        tprime = mag[:, None] * (zc[None, :] - zs[ia])   # cbct_back.m:219
+        bs = sprime / g.ds + ws
+        bt = tprime / g.dt + wt
+        bs = np.where((bs >= 1) & (bs <= ns), bs, ns + 1.0)
+        bt = np.fmax(bt, 1.0)                        # max/min ignore NaN, as MATLAB's
+        bt = np.fmin(bt, float(nt))
+        is_ = np.floor(bs)
+        it = np.floor(bt)
+        is_[is_ == ns + 1] = ns
+        it[it == nt] = nt - 1
+        wr = (bs - is_)[:, None]
+        wl = 1 - wr
+        wu = bt - it
+        wd = 1 - wu
+        proj1[:ns, :] = proj[:, :, ia]
+        i0 = (is_.astype(np.int64) - 1)[:, None]     # 0-based left column
+        t0 = it.astype(np.int64) - 1                 # 0-based lower row
+        p1 = wl * proj1[i0, t0] + wr * proj1[i0 + 1, t0]
+        p2 = wl * proj1[i0, t0 + 1] + wr * proj1[i0 + 1, t0 + 1]
+        p0 = wd * p1 + wu * p2
+        if flat:
+            p0 = p0 * (mag * mag)[:, None]
+        else:
+            p0 = p0 * (g.dsd * g.dsd) / (r_loop * r_loop + y_betas * y_betas)[:, None]
+        img2 = img2 + p0
+    img = np.zeros((nx, ny, nz))
+    img[m] = img2                                    # embed (cbct_back.m:261)
+    return scale * img

@@ -116,6 +116,52 @@ def test_cbct_back_entry_argument_errors_before_device():
         == _lib.FDKHIP_EARG
 
 
+def test_cbct_back_mat_argument_errors_before_device():
+    """cbct_back_mat's own failures (an undefined sprime for a dfs that is
+    neither 0 nor inf, a row index 0 when nt = 1) and bad counts are refused
+    before any device call, by both entry points."""
+    import ctypes as C
+    from mirt_amd import _lib
+    from mirt_amd._lib import Geom
+    lib = _lib.load()
+    img = np.zeros(8, np.float64)
+    beta = np.zeros(2)
+    proj = np.zeros(64, np.float32)
+    P = lambda a: C.c_void_p(a.ctypes.data)
+    g = lambda **kw: Geom(*[kw.get(k, v) for k, v in zip(
+        [f[0] for f in Geom._fields_],
+        (2, 2, 2, 1.0, -1.0, 1.0, 0, 0, 0, 541.0, 949.0, 0.0, 8, 4, 1.0, 1.0, 0.0, 0.5))])
+    host = lambda gg, na=2, skip=1, beta=beta: lib.fdkhip_cbct_back_mat(
+        C.byref(gg), P(proj), 0, na, None if beta is None else P(beta), None, skip, 0.0,
+        None, 1.0, P(img))
+    direct = lambda gg, na=2, skip=1, beta=beta: lib.fdkhip_cbct_back_mat_direct(
+        C.byref(gg), P(proj), 0, na, None if beta is None else P(beta), None, skip, 0.0,
+        None, 1.0, P(img), None, None)
+    for call in (host, direct):
+        assert call(g(dfs=500.0)) == _lib.FDKHIP_EARG
+        assert "dfs must be 0 (arc) or inf (flat)" in lib.fdkhip_last_error().decode()
+        assert call(g(nt=1)) == _lib.FDKHIP_EARG
+        assert "nt >= 2" in lib.fdkhip_last_error().decode()
+        assert call(g(), skip=0) == _lib.FDKHIP_EARG
+        assert call(g(), na=-1) == _lib.FDKHIP_EARG
+        assert call(g(), beta=None) == _lib.FDKHIP_EARG
+        # a parallel geometry takes any dfs (cbct_back.m:211: par before arc/flat)
+        assert call(g(dfs=500.0, dso=np.inf, dsd=np.inf, nz=0)) == 0
+    assert lib.fdkhip_cbct_back_mat(None, P(proj), 0, 2, P(beta), None, 1, 0.0, None, 1.0,
+                                    P(img)) == _lib.FDKHIP_EARG
+
+
+def test_cbct_back_mat_python_checks():
+    from mirt_amd import cbct_back
+    from mirt_amd.geom import ct_geom, image_geom
+    cg = ct_geom("fan", ns=8, nt=4, na=3, ds=1.0, dt=1.0, dsd=949.0, dso=541.0, dfs=0.0)
+    ig = image_geom(nx=4, ny=4, nz=2, dx=1.0, dz=1.0)
+    with pytest.raises(ValueError, match="proj shape"):
+        cbct_back(np.zeros((8, 4, 2), np.float32), cg, ig, use_mex=0)
+    with pytest.raises(ValueError, match="ia_skip"):
+        cbct_back(np.zeros((8, 4, 3), np.float32), cg, ig, use_mex=0, ia_skip=0)
+
+
 def test_quantize_argument_errors():
     from mirt_amd import _lib
     lib = _lib.load()
@@ -253,16 +299,17 @@ def test_cbct_back_ia_skip_and_extrapolate(small_scan):
     assert np.array_equal(p[3:-3], proj.transpose(1, 0, 2))
 
 
-def test_cbct_back_rejects_helix_and_mat(small_scan):
+def test_cbct_back_rejects_helix_for_mex_routes(small_scan):
+    """cbct_back.m:46-49 refuses a helix for use_mex >= 1 only; use_mex = 0
+    (cbct_back_mat) takes source_zs (GPU test in test_gpu_mat.py)."""
     from mirt_amd import cbct_back
     from mirt_amd.geom import ct_geom
     cg, ig, proj = small_scan
     helix = ct_geom("fan", ns=40, nt=16, na=12, ds=1.5, dt=1.6, dsd=500.0, dso=300.0,
                     pitch=1.0)
-    with pytest.raises(ValueError, match="helix not yet supported"):
-        cbct_back(proj, helix, ig, back_call=oracle_back_call)
-    with pytest.raises(NotImplementedError):
-        cbct_back(proj, cg, ig, use_mex=0)
+    for use_mex in (1, 2, 3):
+        with pytest.raises(ValueError, match="helix not yet supported"):
+            cbct_back(proj, helix, ig, use_mex=use_mex, back_call=oracle_back_call)
     from mirt_amd.geom import ct_geom as cgf
     cg2 = cgf("fan", ns=40, nt=16, na=12, ds=1.5, dt=1.6, dsd=500.0, dso=300.0,
               offset_t=0.5)
