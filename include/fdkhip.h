@@ -128,6 +128,42 @@ int fdkhip_fdk_st_back(
 	float *image);
 
 /*
+ * The 19-operand 'fdk,ts,back' of the helical mex gateway
+ * (mex/src/fdk-helix/mex/fdk,mex.c:22-176, 227-236) on host memory: the
+ * operands of fdkhip_fdk_ts_back plus source_zs [na] (source heights),
+ * na1_half (views per half turn: each view's slice window reaches the heights
+ * of views ia -/+ na1_half when pitch != 0, fdk,ts,t.c:78-85), cone_par (1:
+ * views rebinned to cone-parallel geometry), w3d (the exponent of the w3d
+ * view weights; cone_par only), source_dz_per_rad (narrowed to float, as the
+ * gateway does) and pitch (single).  Computes fdk_ts_back_t of
+ * mex/src/fdk-helix/mex/fdk,ts,t.c:107-167 with every branch of fdk_ts_back1
+ * (fdk,ts.c:55-251), bit for bit.  w3d table entries past the reference's
+ * allocation read 0.  Synchronous.
+ */
+int fdkhip_fdk_ts_back_helix(
+	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
+	const uint8_t *mask,
+	double dso, double dsd, double dfs,
+	const double dst[2], const double ost[2],
+	const float *proj, int32_t ns, int32_t nt, int32_t na,
+	const double *beta, const double *source_zs, int32_t na1_half,
+	int32_t cone_par, int32_t w3d, double source_dz_per_rad, float pitch,
+	int32_t nthread, float *image);
+
+/*
+ * fdkhip_fdk_ts_back_helix with proj_dev ([nt ns na] FP32) and image_dev
+ * ([nz nx ny] FP32) in device memory and the geometry as fdkhip_geom; beta,
+ * source_zs and mask stay host pointers.  Runs on `stream` and returns when
+ * the kernel has finished; kernel_ms (may be NULL) receives its time from HIP
+ * events on `stream`.  No counterpart in the reference (device-resident
+ * form for callers that keep views on the GPU).
+ */
+int fdkhip_fdk_ts_back_helix_direct(const fdkhip_geom *g, const float *proj_dev,
+	int32_t na, const double *beta, const double *source_zs, int32_t na1_half,
+	int32_t cone_par, int32_t w3d, double source_dz_per_rad, float pitch,
+	const uint8_t *mask, float *image_dev, void *stream, float *kernel_ms);
+
+/*
  * cbct_back_mat (fbp/cbct_back.m:142-270), the use_mex = 0 route of cbct_back:
  * the reference's MATLAB backprojector, in double precision, with its own
  * edge rules -- a sample position outside detector columns [1, ns] adds 0,

@@ -8,10 +8,12 @@ steps of ``feldkamp``.
 """
 from .geom import CTGeom, ImageGeom, ct_geom, image_geom
 from .cbct_back import (FdkBatch, FdkMexError, FdkPlan, angle_scale, angle_scale64,
-                        cbct_back, cbct_back_mat, fdk_mex, geom_of, quantize_views)
+                        cbct_back, cbct_back_mat, fdk_mex, fdk_mex_helix, geom_of,
+                        quantize_views)
+from .cbct_back_helix import cbct_back_helix
 from ._lib import FdkhipError
 
 __all__ = ["CTGeom", "ImageGeom", "ct_geom", "image_geom", "FdkMexError",
            "FdkBatch", "FdkPlan", "FdkhipError", "angle_scale", "angle_scale64", "cbct_back",
-           "cbct_back_mat", "fdk_mex",
+           "cbct_back_mat", "cbct_back_helix", "fdk_mex", "fdk_mex_helix",
            "geom_of", "quantize_views"]

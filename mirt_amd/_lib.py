@@ -60,6 +60,15 @@ _SIGS = {
                                      C.c_int32, _P]),
     "fdkhip_cbct_back": (C.c_int, [C.POINTER(Geom), _P, C.c_int32, _P, C.c_int32, _P,
                                    C.c_int32, C.c_double, _P]),
+    "fdkhip_fdk_ts_back_helix": (C.c_int, [_P, _P, _P, _P, C.c_double, C.c_double,
+                                           C.c_double, _P, _P, _P, C.c_int32, C.c_int32,
+                                           C.c_int32, _P, _P, C.c_int32, C.c_int32,
+                                           C.c_int32, C.c_double, C.c_float, C.c_int32,
+                                           _P]),
+    "fdkhip_fdk_ts_back_helix_direct": (C.c_int, [C.POINTER(Geom), _P, C.c_int32, _P, _P,
+                                                  C.c_int32, C.c_int32, C.c_int32,
+                                                  C.c_double, C.c_float, _P, _P, _P,
+                                                  C.POINTER(C.c_float)]),
     "fdkhip_cbct_back_mat": (C.c_int, [C.POINTER(Geom), _P, C.c_int32, C.c_int32, _P, _P,
                                        C.c_int32, C.c_double, _P, C.c_double, _P]),
     "fdkhip_cbct_back_mat_direct": (C.c_int, [C.POINTER(Geom), _P, C.c_int32, C.c_int32, _P,

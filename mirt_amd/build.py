@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libfdkhip.so")
 
-SOURCES = ["fdk_kernels.hip", "mat_kernel.hip", "fdk_api.hip"]
+SOURCES = ["fdk_kernels.hip", "mat_kernel.hip", "helix_kernel.hip", "fdk_api.hip"]
 ARCH = os.environ.get("MIRT_AMD_ARCH", "gfx950")
 
 

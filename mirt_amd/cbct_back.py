@@ -56,25 +56,9 @@ def _scalar_double(x, name):
     raise FdkMexError(f"{name} must be a double scalar")
 
 
-def fdk_mex(cmd, nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj, beta,
-            nthread, *extra):
-    """Mirror of the mex gateway ``fdk_mex`` (fdk,mex.c:175-214, 22-171).
-
-    ``cmd`` is ``'fdk,ts,back'`` (proj ``[nt ns na]``, result ``[nz nx ny]``)
-    or ``'fdk,st,back'`` (proj ``[ns nt na]``, result ``[nx ny nz]``).
-    ``nxyz`` int32[3], ``dxyz``/``cxyz`` float64[3], ``mask`` uint8 ``[nx ny]``,
-    ``dst``/``ost`` float64[2], ``proj`` float32 2-D or 3-D, ``beta`` float64
-    (scalar for 2-D proj), ``nthread`` int32 scalar.  Returns a Fortran-order
-    float32 array.
-    """
-    if not isinstance(cmd, str):
-        raise FdkMexError("usage error. see above")
-    if extra or cmd not in ("fdk,ts,back", "fdk,st,back"):
-        if cmd not in ("fdk,ts,back", "fdk,st,back"):
-            raise FdkMexError(f"command '{cmd}' unknown")
-        raise FdkMexError("usage error. see above")   # nrhs != 13
-    is_st = cmd.startswith("fdk,st")
-
+def _gateway_operands(nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj):
+    """Checks of fdk_mex_back (fdk,mex.c:39-90) on the operands both gateways
+    share, in the reference's order and with its messages."""
     if not _is_int32(nxyz):
         raise FdkMexError("mxIsInt32")
     if nxyz.size != 3:
@@ -107,21 +91,59 @@ def fdk_mex(cmd, nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj, beta,
         raise FdkMexError("mx_ost must have 2 elements")
     if not (isinstance(proj, np.ndarray) and proj.dtype == np.float32):
         raise FdkMexError("mxIsRealSingle")
+    return (nx, ny, nz), dso, dsd, dfs
+
+
+def _per_view(x, proj, na, name):
+    """beta / source_zs: a double scalar for a 2-D proj, else [na] doubles."""
     if proj.ndim == 2:
-        na = 1
-        beta = np.asarray([_scalar_double(beta, "beta")], dtype=np.float64)
-    elif proj.ndim == 3:
-        na = proj.shape[2]
-        if not _is_real_double(beta):
-            raise FdkMexError("mxIsRealDouble")
-        if beta.size != na:
-            raise FdkMexError(f"beta length={beta.size} vs na=size(proj,3)={na}")
-    else:
-        raise FdkMexError("mx_proj must be 2d or 3d")
-    if not (isinstance(nthread, np.ndarray) and nthread.dtype == np.int32
-            and nthread.size == 1):
+        return np.asarray([_scalar_double(x, name)], dtype=np.float64)
+    if not _is_real_double(x):
+        raise FdkMexError("mxIsRealDouble")
+    if name == "beta" and x.size != na:
+        raise FdkMexError(f"beta length={x.size} vs na=size(proj,3)={na}")
+    return x
+
+
+def _scalar_int32(x):
+    if not (isinstance(x, np.ndarray) and x.dtype == np.int32 and x.size == 1):
         raise FdkMexError("mxIsScalarInt32")
-    nthread = int(nthread.reshape(-1)[0])
+    return int(x.reshape(-1)[0])
+
+
+def _c_operands(nxyz, dxyz, cxyz, mask, dst, ost, proj, beta):
+    return (np.ascontiguousarray(nxyz.reshape(-1), dtype=np.int32),
+            np.ascontiguousarray(dxyz.reshape(-1)), np.ascontiguousarray(cxyz.reshape(-1)),
+            np.asfortranarray(mask), np.ascontiguousarray(dst.reshape(-1)),
+            np.ascontiguousarray(ost.reshape(-1)), np.asfortranarray(proj),
+            np.ascontiguousarray(np.asarray(beta, np.float64).reshape(-1)))
+
+
+def fdk_mex(cmd, nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj, beta,
+            nthread, *extra):
+    """Mirror of the mex gateway ``fdk_mex`` (fdk,mex.c:175-214, 22-171).
+
+    ``cmd`` is ``'fdk,ts,back'`` (proj ``[nt ns na]``, result ``[nz nx ny]``)
+    or ``'fdk,st,back'`` (proj ``[ns nt na]``, result ``[nx ny nz]``).
+    ``nxyz`` int32[3], ``dxyz``/``cxyz`` float64[3], ``mask`` uint8 ``[nx ny]``,
+    ``dst``/``ost`` float64[2], ``proj`` float32 2-D or 3-D, ``beta`` float64
+    (scalar for 2-D proj), ``nthread`` int32 scalar.  Returns a Fortran-order
+    float32 array.
+    """
+    if not isinstance(cmd, str):
+        raise FdkMexError("usage error. see above")
+    if extra or cmd not in ("fdk,ts,back", "fdk,st,back"):
+        if cmd not in ("fdk,ts,back", "fdk,st,back"):
+            raise FdkMexError(f"command '{cmd}' unknown")
+        raise FdkMexError("usage error. see above")   # nrhs != 13
+    is_st = cmd.startswith("fdk,st")
+    (nx, ny, nz), dso, dsd, dfs = _gateway_operands(nxyz, dxyz, cxyz, mask, dso, dsd, dfs,
+                                                    dst, ost, proj)
+    if proj.ndim not in (2, 3):
+        raise FdkMexError("mx_proj must be 2d or 3d")
+    na = 1 if proj.ndim == 2 else proj.shape[2]
+    beta = _per_view(beta, proj, na, "beta")
+    nthread = _scalar_int32(nthread)
 
     if is_st:
         ns, nt = proj.shape[0], proj.shape[1]
@@ -131,18 +153,64 @@ def fdk_mex(cmd, nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj, beta,
         out = np.zeros((nz, nx, ny), dtype=np.float32, order="F")
 
     lib = _lib.load()
-    projf = np.asfortranarray(proj)
-    maskf = np.asfortranarray(mask)
-    nxyz_c = np.ascontiguousarray(nxyz.reshape(-1), dtype=np.int32)
-    dxyz_c = np.ascontiguousarray(dxyz.reshape(-1))
-    cxyz_c = np.ascontiguousarray(cxyz.reshape(-1))
-    dst_c = np.ascontiguousarray(dst.reshape(-1))
-    ost_c = np.ascontiguousarray(ost.reshape(-1))
-    beta_c = np.ascontiguousarray(np.asarray(beta, np.float64).reshape(-1))
+    nxyz_c, dxyz_c, cxyz_c, maskf, dst_c, ost_c, projf, beta_c = _c_operands(
+        nxyz, dxyz, cxyz, mask, dst, ost, proj, beta)
     fn = lib.fdkhip_fdk_st_back if is_st else lib.fdkhip_fdk_ts_back
     rc = fn(_ptr(nxyz_c), _ptr(dxyz_c), _ptr(cxyz_c), _ptr(maskf), dso, dsd, dfs,
             _ptr(dst_c), _ptr(ost_c), _ptr(projf), ns, nt, na, _ptr(beta_c),
             nthread, _ptr(out))
+    _lib.check(rc)
+    return out
+
+
+def fdk_mex_helix(cmd, nxyz, dxyz, cxyz, mask, dso, dsd, dfs, dst, ost, proj, beta,
+                  source_zs, na1_half, cone_par, w3d, source_dz_per_rad, pitch, nthread,
+                  *extra):
+    """Mirror of the helical mex gateway (mex/src/fdk-helix/mex/fdk,mex.c:22-236):
+    ``fdk_mex('fdk,ts,back', ..., proj, beta, source_zs, na1_half, cone_par,
+    w3d, source_dz_per_rad, pitch, nthread)`` -- 19 operands.  ``source_zs``
+    float64 like ``beta``; ``na1_half``, ``cone_par``, ``w3d`` int32 scalars,
+    ``source_dz_per_rad`` float64, ``pitch`` float32 (single).  Returns the
+    ``[nz nx ny]`` float32 image (Fortran order).  The ``'fdk,st,back'``
+    order of this gateway ignores the helical operands (fdk,mex.c:150-155);
+    use :func:`fdk_mex` for it.
+    """
+    if not isinstance(cmd, str):
+        raise FdkMexError("usage error. see above")
+    if cmd not in ("fdk,ts,back", "fdk,st,back"):
+        raise FdkMexError(f"command '{cmd}' unknown")
+    if extra:
+        raise FdkMexError("usage error. see above")   # nrhs != 19
+    if cmd == "fdk,st,back":
+        raise FdkMexError("'fdk,st,back' does not handle helical geometry; use fdk_mex")
+    (nx, ny, nz), dso, dsd, dfs = _gateway_operands(nxyz, dxyz, cxyz, mask, dso, dsd, dfs,
+                                                    dst, ost, proj)
+    if proj.ndim not in (2, 3):
+        raise FdkMexError("mx_proj must be 2d or 3d")
+    na = 1 if proj.ndim == 2 else proj.shape[2]
+    beta = _per_view(beta, proj, na, "beta")
+    source_zs = _per_view(source_zs, proj, na, "source_zs")
+    if source_zs.size < na:
+        raise FdkMexError(f"source_zs length={source_zs.size} vs na={na}")
+    nthread = _scalar_int32(nthread)
+    na1_half = _scalar_int32(na1_half)
+    cone_par = _scalar_int32(cone_par)
+    w3d = _scalar_int32(w3d)
+    if not _is_real_double(source_dz_per_rad):
+        raise FdkMexError("mxIsRealDouble")
+    if not (isinstance(pitch, np.ndarray) and pitch.dtype == np.float32):
+        raise FdkMexError("mxIsRealSingle")
+    nt, ns = proj.shape[0], proj.shape[1]
+    out = np.zeros((nz, nx, ny), dtype=np.float32, order="F")
+    lib = _lib.load()
+    nxyz_c, dxyz_c, cxyz_c, maskf, dst_c, ost_c, projf, beta_c = _c_operands(
+        nxyz, dxyz, cxyz, mask, dst, ost, proj, beta)
+    zs_c = np.ascontiguousarray(source_zs.reshape(-1)[:na], dtype=np.float64)
+    rc = lib.fdkhip_fdk_ts_back_helix(
+        _ptr(nxyz_c), _ptr(dxyz_c), _ptr(cxyz_c), _ptr(maskf), dso, dsd, dfs,
+        _ptr(dst_c), _ptr(ost_c), _ptr(projf), ns, nt, na, _ptr(beta_c), _ptr(zs_c),
+        na1_half, cone_par, w3d, float(np.asarray(source_dz_per_rad).reshape(-1)[0]),
+        float(pitch.reshape(-1)[0]), nthread, _ptr(out))
     _lib.check(rc)
     return out
 
@@ -594,5 +662,5 @@ def quantize_views(proj, fmt="f16", *, stream=None):
     return q, scale
 
 
-__all__ = ["fdk_mex", "cbct_back", "cbct_back_mat", "FdkPlan", "FdkBatch", "FdkMexError",
+__all__ = ["fdk_mex", "fdk_mex_helix", "cbct_back", "cbct_back_mat", "FdkPlan", "FdkBatch", "FdkMexError",
            "FdkhipError", "quantize_views", "geom_of", "angle_scale", "angle_scale64"]

@@ -806,6 +806,238 @@ extern "C" int fdkhip_fdk_st_back(
 }
 
 // ---------------------------------------------------------------------------
+// helical / cone-parallel 'fdk,ts,back' (mex/src/fdk-helix/mex)
+
+// Host tables of fdk_ts_back_t / fdk_ts_back_init (fdk,ts,t.c:39-141): per
+// view trig, height and window; the w3d table with a zero tail; per slice
+// chunk the range of views whose window reaches it.
+struct HelixHost {
+	std::vector<HelixView> views;
+	std::vector<int2> chunks;
+	std::vector<float> tab;
+};
+
+static int helix_setup(const fdkhip_geom *g, int na, const double *beta,
+	const double *source_zs, int na1_half, int cone_par, int w3d, double dz_per_rad,
+	float pitch, HelixArgs &a, HelixHost &h)
+{
+	int rc = check_geom(g);
+	if (rc)
+		return rc;
+	if (na < 0)
+		return fail(FDKHIP_EARG, "na must be >= 0 (na=%d)", na);
+	if (na > 0 && (!beta || !source_zs))
+		return fail(FDKHIP_EARG, "beta or source_zs is NULL");
+	if (na1_half < 0 || w3d < 0)
+		return fail(FDKHIP_EARG, "na1_half and w3d must be >= 0");
+	memset(&a, 0, sizeof a);
+	a.g = narrow_geom(g, na);
+	const GeomF &f = a.g;
+	if (na > 0 && f.dz / f.dt < 0)   // fdk,ts.c:153-154
+		return fail(FDKHIP_EGEOM, "todo: dz/dt < 0 not done");
+	a.cone_par = cone_par != 0;
+	a.w3d = w3d;
+	a.pitch = pitch;
+	a.dz_per_rad = (double) (float) dz_per_rad;   // fdk,mex.c:128: a float
+	a.nt2 = g->nt / 2;
+	a.nzc = (g->nz + HX_ZC - 1) / HX_ZC;
+	h.views.resize((size_t) std::max(na, 1));
+	const float wz = f.wz, dz = f.dz;
+	for (int ia = 0; ia < na; ++ia) {
+		HelixView &v = h.views[ia];
+		memset(&v, 0, sizeof v);
+		const float b = (float) beta[ia];
+		v.cosb = cosf(b);
+		v.sinb = sinf(b);
+		v.zs = (float) source_zs[ia];
+		v.izb0 = 0;
+		v.izb1 = g->nz;
+		if (pitch != 0.f) {   // fdk,ts,t.c:80-85
+			const int a0 = std::max(ia - na1_half, 0);
+			const int a1 = std::min(ia + na1_half, na - 1);
+			const float c0 = ceilf((float) (wz + source_zs[a0] / dz));
+			const float c1 = ceilf((float) (wz + source_zs[a1] / dz));
+			v.izb0 = (int) (c0 > 0 ? c0 : 0);
+			v.izb1 = (int) (c1 < g->nz ? c1 : g->nz);
+		}
+	}
+	if (na > 0) {
+		a.zs_min = (float) source_zs[0];       // fdk,ts,t.c:70-71
+		a.zs_max = (float) source_zs[na - 1];
+	}
+	// w3d table (fdk,ts,t.c:134-141 -> fdk,w3d,pow,tab.c:9-31, whose `dt`
+	// argument receives wz) and a zero tail up to the largest index the
+	// lookup can form: nt/2 + (nt/2) * trunc(max |iz - wz|)
+	const int nt2 = a.nt2;
+	const int rows = (int) ceilf((float) (g->nz / 2));
+	const int ntab = pitch != 0.f ? nt2 : nt2 * rows;
+	const float span = std::max(fabsf(wz), fabsf((float) (g->nz - 1) - wz));
+	const long tail = (long) nt2 * ((long) span + 2) + 2;
+	if (a.cone_par && w3d && ntab + tail > (1l << 28))
+		return fail(FDKHIP_EUNSUPPORTED, "w3d table too large");
+	h.tab.assign(a.cone_par && w3d ? (size_t) (ntab + tail) : 1, 0.f);
+	if (a.cone_par && w3d) {
+		const float dsd = f.dsd;
+		if (pitch != 0.f) {
+			for (int it = 0; it < nt2; ++it)
+				h.tab[it] = (float) pow(it * wz / dsd, w3d * pitch);
+		} else {
+			size_t k = 0;
+			for (int iz = 0; iz < rows; ++iz)
+				for (int it = 0; it < nt2; ++it)
+					h.tab[k++] = (float) pow(it * wz / dsd, w3d * iz * dz);
+		}
+	}
+	a.tab_len = (int) h.tab.size();
+	// views whose window [izb0, izb1) meets each chunk of HX_ZC slices
+	h.chunks.assign((size_t) std::max(a.nzc, 1), make_int2(0, 0));
+	for (int c = 0; c < a.nzc; ++c) {
+		const int z0 = c * HX_ZC, z1 = std::min(z0 + HX_ZC, g->nz);
+		int va = -1, vb = -1;
+		for (int ia = 0; ia < na; ++ia)
+			if (h.views[ia].izb0 < z1 && h.views[ia].izb1 > z0) {
+				if (va < 0)
+					va = ia;
+				vb = ia + 1;
+			}
+		if (va >= 0)
+			h.chunks[c] = make_int2(va, vb);
+	}
+	return FDKHIP_OK;
+}
+
+// Upload the tables, launch on `s`, wait; kernel_ms (optional) from events.
+static int helix_run(HelixArgs &a, const HelixHost &h, const uint8_t *mask_host,
+	hipStream_t s, float *kernel_ms)
+{
+	const size_t nxy = (size_t) a.g.nx * a.g.ny;
+	DevBuf dviews, dchunks, dtab, dmask;
+	hipError_t e = hipMalloc(&dviews.p, h.views.size() * sizeof(HelixView));
+	if (e == hipSuccess)
+		e = hipMalloc(&dchunks.p, h.chunks.size() * sizeof(int2));
+	if (e == hipSuccess)
+		e = hipMalloc(&dtab.p, h.tab.size() * sizeof(float));
+	if (e == hipSuccess && mask_host && nxy > 0)
+		e = hipMalloc(&dmask.p, nxy);
+	if (e != hipSuccess)
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	HIP_TRY(hipMemcpyAsync(dviews.p, h.views.data(), h.views.size() * sizeof(HelixView),
+		hipMemcpyHostToDevice, s));
+	HIP_TRY(hipMemcpyAsync(dchunks.p, h.chunks.data(), h.chunks.size() * sizeof(int2),
+		hipMemcpyHostToDevice, s));
+	HIP_TRY(hipMemcpyAsync(dtab.p, h.tab.data(), h.tab.size() * sizeof(float),
+		hipMemcpyHostToDevice, s));
+	if (dmask.p)
+		HIP_TRY(hipMemcpyAsync(dmask.p, mask_host, nxy, hipMemcpyHostToDevice, s));
+	a.views = (const HelixView *) dviews.p;
+	a.chunk_views = (const int2 *) dchunks.p;
+	a.tab = (const float *) dtab.p;
+	a.mask = (const uint8_t *) dmask.p;
+	hipEvent_t ev[2] = {nullptr, nullptr};
+	if (kernel_ms) {
+		HIP_TRY(hipEventCreate(&ev[0]));
+		e = hipEventCreate(&ev[1]);
+		if (e != hipSuccess) {
+			(void) hipEventDestroy(ev[0]);
+			return fail(FDKHIP_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+		}
+		e = hipEventRecord(ev[0], s);
+	}
+	if (e == hipSuccess)
+		e = launch_helix(a, s);
+	if (e == hipSuccess && kernel_ms)
+		e = hipEventRecord(ev[1], s);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(s);
+	if (e == hipSuccess && kernel_ms)
+		e = hipEventElapsedTime(kernel_ms, ev[0], ev[1]);
+	if (kernel_ms) {
+		(void) hipEventDestroy(ev[0]);
+		(void) hipEventDestroy(ev[1]);
+	}
+	if (e != hipSuccess)
+		return fail(FDKHIP_EHIP, "helical backprojection launch: %s", hipGetErrorString(e));
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_fdk_ts_back_helix(
+	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
+	const uint8_t *mask, double dso, double dsd, double dfs,
+	const double dst[2], const double ost[2],
+	const float *proj, int32_t ns, int32_t nt, int32_t na,
+	const double *beta, const double *source_zs, int32_t na1_half,
+	int32_t cone_par, int32_t w3d, double source_dz_per_rad, float pitch,
+	int32_t nthread, float *image)
+{
+	int rc = check_mex_args(nxyz, dxyz, cxyz, dst, ost, ns, nt, na, beta, nthread, proj, image);
+	if (rc)
+		return rc;
+	if (na > 0 && !source_zs)
+		return fail(FDKHIP_EARG, "source_zs length=0 vs na=%d", na);
+	const fdkhip_geom g = mex_geom(nxyz, dxyz, cxyz, dso, dsd, dfs, dst, ost, ns, nt);
+	HelixArgs a;
+	HelixHost h;
+	if ((rc = helix_setup(&g, na, beta, source_zs, na1_half, cone_par, w3d,
+			source_dz_per_rad, pitch, a, h)))
+		return rc;
+	const size_t nvox = (size_t) g.nx * g.ny * g.nz;
+	if (nvox == 0)
+		return FDKHIP_OK;
+	const size_t nproj = (size_t) ns * nt * na;
+	DevBuf dproj, dimg;
+	hipError_t e = hipMalloc(&dimg.p, nvox * sizeof(float));
+	if (e == hipSuccess && nproj > 0)
+		e = hipMalloc(&dproj.p, nproj * sizeof(float));
+	if (e != hipSuccess)
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	if (nproj > 0)
+		HIP_TRY(hipMemcpy(dproj.p, proj, nproj * sizeof(float), hipMemcpyHostToDevice));
+	a.proj = (const float *) dproj.p;
+	a.image = (float *) dimg.p;
+	if (nproj == 0) {
+		HIP_TRY(launch_fill(a.image, (long) g.nx * g.ny, g.nz, 0, 0, nullptr));
+		HIP_TRY(hipDeviceSynchronize());
+	} else if ((rc = helix_run(a, h, mask, nullptr, nullptr))) {
+		return rc;
+	}
+	HIP_TRY(hipMemcpy(image, dimg.p, nvox * sizeof(float), hipMemcpyDeviceToHost));
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_fdk_ts_back_helix_direct(const fdkhip_geom *g, const float *proj_dev,
+	int32_t na, const double *beta, const double *source_zs, int32_t na1_half,
+	int32_t cone_par, int32_t w3d, double source_dz_per_rad, float pitch,
+	const uint8_t *mask, float *image_dev, void *stream, float *kernel_ms)
+{
+	if (!g)
+		return fail(FDKHIP_EARG, "geometry is NULL");
+	HelixArgs a;
+	HelixHost h;
+	int rc = helix_setup(g, na, beta, source_zs, na1_half, cone_par, w3d, source_dz_per_rad,
+		pitch, a, h);
+	if (rc)
+		return rc;
+	if (kernel_ms)
+		*kernel_ms = 0.f;
+	const size_t nvox = (size_t) g->nx * g->ny * g->nz;
+	if (nvox == 0)
+		return FDKHIP_OK;
+	if (!image_dev)
+		return fail(FDKHIP_EARG, "image is NULL");
+	if (na > 0 && (size_t) g->ns * g->nt > 0 && !proj_dev)
+		return fail(FDKHIP_EARG, "proj is NULL");
+	hipStream_t s = (hipStream_t) stream;
+	a.proj = proj_dev;
+	a.image = image_dev;
+	if (na == 0 || (size_t) g->ns * g->nt == 0) {
+		HIP_TRY(launch_fill(image_dev, (long) g->nx * g->ny, g->nz, 0, 0, s));
+		HIP_TRY(hipStreamSynchronize(s));
+		return FDKHIP_OK;
+	}
+	return helix_run(a, h, mask, s, kernel_ms);
+}
+
+// ---------------------------------------------------------------------------
 // cbct_back_mat (fbp/cbct_back.m:142-270): the MATLAB backprojector
 
 // Argument checks of both mat entry points; fills the kernel arguments other

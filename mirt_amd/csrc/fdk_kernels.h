@@ -55,6 +55,30 @@ struct StArgs {
 	float *image;            // [nz][ny][nx]
 };
 
+// helical / cone-parallel 'fdk,ts,back' of the fdk-helix kernel
+// (mex/src/fdk-helix/mex/fdk,ts.c:55-251): per-view tables as
+// fdk_ts_back_init (fdk,ts,t.c:39-100) computes them
+struct HelixView {
+	float cosb, sinb;        // cosf/sinf of the float-narrowed beta
+	float zs;                // source_zs narrowed to float
+	int izb0, izb1;          // iz_min_beta, iz_max_beta (fdk,ts,t.c:78-85)
+	int pad[3];
+};
+constexpr int HX_ZC = 16;    // slices per lane
+struct HelixArgs {
+	GeomF g;
+	const float *proj;       // [na][ns][nt] (t fastest)
+	const HelixView *views;  // [na]
+	const int2 *chunk_views; // [nzc] views whose window reaches the slice chunk
+	const float *tab;        // w3d table (fdk,w3d,pow,tab.c) and a zero tail
+	int tab_len;             // readable entries of tab
+	int nt2, cone_par, w3d, nzc;
+	float pitch, zs_min, zs_max;
+	double dz_per_rad;       // float-narrowed, as the gateway narrows it
+	const uint8_t *mask;     // [ny][nx] or null
+	float *image;            // [ny][nx][nz]
+};
+
 // cbct_back_mat (fbp/cbct_back.m:142-270): the MATLAB backprojector, all in
 // double as MATLAB holds it.  ws and wt are 1-based (cbct_back.m:185-186).
 constexpr int MAT_ZC = 16;   // slices per lane
@@ -89,5 +113,6 @@ hipError_t launch_st(const StArgs &a, hipStream_t s);
 hipError_t launch_quantize(const float *proj, int ns, int nt, int na, int fmt,
 	void *out, float *scale, int *maxbits, hipStream_t s);
 hipError_t launch_mat(const MatArgs &a, bool proj_fp64, hipStream_t s);
+hipError_t launch_helix(const HelixArgs &a, hipStream_t s);
 
 } // namespace fdk

@@ -18,6 +18,18 @@
  *   orc_back_ts_f64   the same algorithm carried out entirely in double
  *                     precision (double beta, sin/cos/atan2/floor in double);
  *                     the error reference for every precision claim.
+ *   orc_back_ts_helix_f32
+ *                     fdk_ts_back_t / fdk_ts_back1 of mex/src/fdk-helix/mex
+ *                     with every branch: per-view source heights, the view
+ *                     window iz_min_beta/iz_max_beta (fdk,ts,t.c:76-85),
+ *                     cone-parallel magnification and heights
+ *                     (fdk,ts.c:115-158), the w3d weights of
+ *                     fdk,w3d,pow,tab.c:9-56 with the table filled as
+ *                     fdk,ts,t.c:136-141 fills it (its `dt` argument receives
+ *                     wz).  Table entries past the reference's allocation
+ *                     read 0 here (the reference reads past its calloc'ed
+ *                     table there; glibc's zeroed slack gives 0 when the
+ *                     table size is a multiple of 16 bytes).
  *   orc_back_st_f32   fdk_st_back1 in mex/src/fdk/fdk-orig/fdk,st.c:38-141
  *                     ([ns nt] view order, no zero padding, edge samples
  *                     skipped, vertical weights as in that file).
@@ -27,9 +39,11 @@
  * (fdk,ts.c:90-98: w* computed in double then narrowed, beta narrowed to
  * float before sinf/cosf).
  *
- * Pinning (tests/test_oracle.py): the f32 restatement is compared with the
- * reference C compiled from its own sources (oracle/_ref) on seeded flat, arc
- * and parallel geometries and on the golden fixtures in tests/golden; the f64
+ * Pinning (tests/test_oracle.py, tests/test_oracle_helix.py): the f32
+ * restatements are compared with the reference C compiled from its own
+ * sources (oracle/_ref) on seeded flat, arc and parallel geometries (helical,
+ * cone-parallel and w3d cases for the helix one) and on the golden fixtures
+ * in tests/golden; the f64
  * restatement against closed forms (constant views on the parallel geometry,
  * linearity, masked columns, impulse footprints) and against the f32 one.
  */
@@ -273,6 +287,195 @@ int orc_back_ts_f64(const orc_geom *g, const float *proj, int32_t na,
 		}
 	}
 	return ok;
+}
+
+/* ------------------------------------------------------------------ */
+/* helical / cone-parallel path of fdk-helix, FP32                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+	int na1_half, cone_par, w3d, ntab, nt2;
+	float pitch, zs_min, zs_max, wz;
+	double dz_per_rad;       /* float-narrowed by the gateway (fdk,mex.c:128) */
+	float *tab;              /* ntab entries, then a zero tail */
+} orc_hx;
+
+/* fdk_w3d_pow_tab (fdk,w3d,pow,tab.c:33-56) */
+static float orc_w3d(const orc_hx *h, float t_bin, float t_bin_c, float wt, int iz)
+{
+	const int id = (int) (h->pitch ? 0.f : fabsf((float) iz - h->wz));
+	const float half = (float) h->nt2;
+	const float a = fabsf(t_bin - wt), a_c = fabsf(t_bin_c - wt);
+	const int indx = (int) (floorf(a < half ? a : half) + ceilf(half) * id);
+	const int indx_c = (int) (floorf(a_c < half ? a_c : half) + ceilf(half) * id);
+	const float scale = h->tab[indx], scale_c = h->tab[indx_c];
+	const float eps = 0.001f;
+	float r = scale_c / (scale_c + scale);
+	r = (r > eps) ? r : eps;
+	r = (r < 1 - eps) ? r : 1 - eps;
+	return 2 * r;
+}
+
+/* One column, all views: fdk,ts.c:112-245 (fdk-helix) for a single (ix,iy). */
+static void orc_column_helix(const orc_f32 *f, const orc_hx *h, int ix, int iy,
+	const float *proj, int na, const float *sinb, const float *cosb,
+	const float *zs, const int *izb0, const int *izb1, float *col)
+{
+	const int nt = f->nt, ns = f->ns;
+	const float yy = f->dy * (iy - f->wy);
+	const float xx = f->dx * (ix - f->wx);
+	const int is_par = f->kind == ORC_PAR, is_arc = f->kind == ORC_ARC;
+	for (int ia = 0; ia < na; ++ia) {
+		const float *view = proj + (long) ia * ns * nt; /* [nt ns] */
+		const float xbeta = xx * cosb[ia] + yy * sinb[ia];
+		const float ybetas = f->dso - (-xx * sinb[ia] + yy * cosb[ia]);
+		const float mag = h->cone_par
+			? (float) (f->dsd / (pow(f->dso * f->dso - xbeta * xbeta, 0.5)
+				- (f->dso - ybetas)))
+			: (is_par ? 1.f : f->dsd / ybetas);
+		const float ss = h->cone_par ? xbeta
+			: (is_arc ? (float) ((double) f->dsd * atan2((double) xbeta, (double) ybetas))
+				: mag * xbeta);
+		const float ss_bin = ss / f->ds + f->ws;
+		const int is = (int) floorf(ss_bin);
+		if (is < -1 || is >= ns)
+			continue;
+		const float wr = ss_bin - is;
+		const float wl = 1.f - wr;
+		const float t_inc = mag * f->dz / f->dt;
+		const float wzb0 = f->wz + zs[ia] / f->dz;
+		const float wzbeta = h->cone_par
+			? (float) (wzb0 + asin(-xbeta / f->dso) * h->dz_per_rad / f->dz)
+			: wzb0;
+		int iz_min = (int) (floorf(wzbeta - (f->wt + 1) / t_inc) + 1);
+		int iz_max = (int) (ceilf(wzbeta + (nt - f->wt) / t_inc) - 1 + 1);
+		iz_min = iz_min > 0 ? iz_min : 0;
+		iz_min = iz_min > izb0[ia] ? iz_min : izb0[ia];
+		iz_max = iz_max < f->nz ? iz_max : f->nz;
+		iz_max = iz_max < izb1[ia] ? iz_max : izb1[ia];
+		float t_bin = t_inc * (iz_min - wzbeta) + f->wt;
+		if (t_bin == -1) {
+			++iz_min;
+			t_bin += t_inc;
+		}
+		/* conjugate ray (fdk,ts.c:169-175) */
+		const float xbeta_c = -xbeta;
+		const float ybetas_c = 2 * f->dso - ybetas;
+		const float mag_c = (float) (f->dsd / (pow(f->dso * f->dso - xbeta_c * xbeta_c, 0.5)
+			- (f->dso - ybetas_c)));
+		const float t_inc_c = mag_c * f->dz / f->dt;
+		for (int iz = iz_min; iz < iz_max; ++iz) {
+			const int it = (int) floorf(t_bin);
+			const float wu = t_bin - it;
+			const float wd = 1.f - wu;
+			float a[2][2]; /* view at (is+ds, it+dt), 0 outside */
+			for (int a_s = 0; a_s < 2; ++a_s)
+				for (int a_t = 0; a_t < 2; ++a_t) {
+					const int s = is + a_s, t = it + a_t;
+					a[a_s][a_t] = (s >= 0 && s < ns && t >= 0 && t < nt)
+						? view[t + (long) s * nt] : 0.f;
+				}
+			float p0 = 0.f;
+			if (it >= -1 && it < nt) {
+				const float p1 = wl * a[0][0] + wr * a[1][0];
+				const float p2 = wl * a[0][1] + wr * a[1][1];
+				p0 = wd * p1 + wu * p2;
+			}
+			float ww = 1.f;
+			if (h->cone_par && h->w3d) {
+				const float zs_c = (float) (h->pitch
+					? ((t_bin >= f->wt) ? (zs[ia] + M_PI * h->dz_per_rad)
+						: (zs[ia] - M_PI * h->dz_per_rad))
+					: 0);
+				const float wzbeta_c = (float) (f->wz + zs_c / f->dz
+					+ asin(-xbeta_c / f->dso) * h->dz_per_rad / f->dz);
+				const float t_bin_c = t_inc_c * (iz - wzbeta_c) + f->wt;
+				if (zs_c >= h->zs_min && zs_c <= h->zs_max)
+					ww = orc_w3d(h, t_bin, t_bin_c, f->wt, iz);
+			}
+			const float w2 = h->cone_par ? ww : (is_arc
+				? (f->dsd * f->dsd) / (ybetas * ybetas + xbeta * xbeta)
+				: mag * mag);
+			col[iz] += w2 * p0;
+			t_bin += t_inc;
+		}
+	}
+}
+
+/* fdk_ts_back_t (fdk,ts,t.c:107-167, fdk-helix) with all of its arguments:
+ * proj [nt ns na], beta and source_zs [na] (double), image [nz nx ny].
+ * source_dz_per_rad is narrowed to float as the gateway does.  Returns 0 on
+ * the reference's dz/dt < 0 failure (checked up front here), 1 otherwise. */
+int orc_back_ts_helix_f32(const orc_geom *g, const float *proj, int32_t na,
+	const double *beta, const double *source_zs, int32_t na1_half, int32_t cone_par,
+	int32_t w3d, double source_dz_per_rad, float pitch, const uint8_t *mask,
+	float *image)
+{
+	orc_f32 f;
+	orc_f32_init(&f, g);
+	const long nxy = (long) g->nx * g->ny;
+	memset(image, 0, sizeof(float) * nxy * g->nz);
+	if (na <= 0)
+		return 1;
+	if (f.dz / f.dt < 0)
+		return 0;
+	orc_hx h;
+	h.na1_half = na1_half;
+	h.cone_par = cone_par;
+	h.w3d = w3d;
+	h.pitch = pitch;
+	h.dz_per_rad = (float) source_dz_per_rad;
+	h.zs_min = (float) source_zs[0];
+	h.zs_max = (float) source_zs[na - 1];
+	h.wz = (float) ((g->nz - 1) / 2. + (float) g->offset_z);
+	h.nt2 = g->nt / 2;
+	const int rows = (int) ceilf((float) (g->nz / 2));
+	h.ntab = pitch ? (int) ceilf((float) h.nt2) : (int) (ceilf((float) h.nt2) * rows);
+	/* largest index orc_w3d can form: nt/2 + (nt/2) * trunc(max |iz - wz|) */
+	const float span = fabsf(h.wz) > fabsf((float) (g->nz - 1) - h.wz)
+		? fabsf(h.wz) : fabsf((float) (g->nz - 1) - h.wz);
+	const long tail = (long) h.nt2 * ((long) span + 2) + 2;
+	h.tab = (float *) calloc((size_t) (h.ntab + tail), sizeof(float));
+	/* fdk_w3d_pow_tab_init (fdk,w3d,pow,tab.c:9-31) called with dt := wz */
+	const float dtw = h.wz, dsd = f.dsd;
+	if (pitch) {
+		for (int it = 0; it < ceilf((float) h.nt2); it++)
+			h.tab[it] = (float) pow(it * dtw / dsd, w3d * pitch);
+	} else {
+		float *t = h.tab;
+		for (int iz = 0; iz < rows; iz++)
+			for (int it = 0; it < ceilf((float) h.nt2); it++, t++)
+				*t = (float) pow(it * dtw / dsd, w3d * iz * f.dz);
+	}
+	float *sinb = (float *) malloc(sizeof(float) * na);
+	float *cosb = (float *) malloc(sizeof(float) * na);
+	float *zs = (float *) malloc(sizeof(float) * na);
+	int *izb0 = (int *) malloc(sizeof(int) * na);
+	int *izb1 = (int *) malloc(sizeof(int) * na);
+	for (int ia = 0; ia < na; ++ia) {
+		const float b = (float) beta[ia];
+		sinb[ia] = sinf(b);
+		cosb[ia] = cosf(b);
+		zs[ia] = (float) source_zs[ia];
+		izb0[ia] = 0;
+		izb1[ia] = g->nz;
+		if (pitch) { /* fdk,ts,t.c:80-85 */
+			const int a0 = ia - na1_half > 0 ? ia - na1_half : 0;
+			const int a1 = ia + na1_half < na - 1 ? ia + na1_half : na - 1;
+			const float c0 = ceilf((float) (h.wz + source_zs[a0] / f.dz));
+			const float c1 = ceilf((float) (h.wz + source_zs[a1] / f.dz));
+			izb0[ia] = (int) (c0 > 0 ? c0 : 0);
+			izb1[ia] = (int) (c1 < g->nz ? c1 : g->nz);
+		}
+	}
+	#pragma omp parallel for schedule(dynamic, 16)
+	for (long j = 0; j < nxy; ++j) {
+		if (mask && !mask[j]) continue;
+		orc_column_helix(&f, &h, (int) (j % g->nx), (int) (j / g->nx), proj, na,
+			sinb, cosb, zs, izb0, izb1, image + j * g->nz);
+	}
+	free(sinb); free(cosb); free(zs); free(izb0); free(izb1); free(h.tab);
+	return 1;
 }
 
 /* ------------------------------------------------------------------ */

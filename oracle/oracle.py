@@ -6,7 +6,8 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 
 * ``oracle/lib/libfdk_oracle.so`` -- this directory's clean-room restatement
   (``fdk_oracle.c``): FP32 with the reference's operation order, FP64 error
-  reference, and the ``[ns nt]``-order variant;
+  reference, the ``[ns nt]``-order variant, and the helical / cone-parallel
+  path of the fdk-helix kernel (``back_ts_helix_f32``);
 * ``oracle/_ref/libfdkref_{fixed,shipped}.so`` -- the reference C compiled from
   ``mex/src/fdk-helix/mex`` (debugged vertical weights, axial path) and
   ``mex/src/fdk/fdk-orig`` (as shipped), driven by ``ref_driver.c`` the way
@@ -156,6 +157,58 @@ def back_ts_f64(g: OrcGeom, proj, beta, mask=None, *, cols=None):
                              _ptr(out, C.c_double))
     if not ok:
         raise RuntimeError("todo: dz/dt < 0 not done")
+    return out
+
+
+def _helix_args(na, source_zs):
+    zs = np.ascontiguousarray(np.broadcast_to(np.asarray(source_zs, np.float64), (na,)))
+    return zs
+
+
+def back_ts_helix_f32(g: OrcGeom, proj, beta, source_zs, mask=None, *, na1_half=0,
+                      cone_par=0, w3d=0, source_dz_per_rad=0.0, pitch=0.0):
+    """FP32 restatement of the fdk-helix fdk_ts_back_t with its helical
+    arguments (fdk,ts.c:55-251, fdk,ts,t.c:39-167, fdk,w3d,pow,tab.c).
+    Returns [nz nx ny]."""
+    lib = _load(LIB_ORACLE)
+    proj, na, beta, mask = _prep(g, proj, beta, mask)
+    zs = _helix_args(na, source_zs)
+    out = np.empty((g.ny, g.nx, g.nz), np.float32)
+    ok = lib.orc_back_ts_helix_f32(C.byref(g), _ptr(proj, C.c_float), C.c_int32(na),
+                                   _ptr(beta, C.c_double), _ptr(zs, C.c_double),
+                                   C.c_int32(na1_half), C.c_int32(cone_par), C.c_int32(w3d),
+                                   C.c_double(source_dz_per_rad), C.c_float(pitch),
+                                   _ptr(mask, C.c_uint8), _ptr(out, C.c_float))
+    if not ok:
+        raise RuntimeError("todo: dz/dt < 0 not done")
+    return out
+
+
+def ref_back_ts_helix(g: OrcGeom, proj, beta, source_zs, mask=None, *, na1_half=0,
+                      cone_par=0, w3d=0, source_dz_per_rad=0.0, pitch=0.0, nthread=1):
+    """The reference C (oracle/_ref, fdk-helix build) 'fdk,ts,back' with its
+    19 operands; returns [nz nx ny]."""
+    lib = _load(LIB_REF["fixed"])
+    proj, na, beta, mask = _prep(g, proj, beta, mask)
+    zs = _helix_args(na, source_zs)
+    if mask is None:
+        mask = np.ones(g.nx * g.ny, np.uint8)
+    out = np.empty((g.ny, g.nx, g.nz), np.float32)
+    nxyz = np.array([g.nx, g.ny, g.nz], np.int32)
+    dxyz = np.array([g.dx, g.dy, g.dz], np.float64)
+    cxyz = np.array([g.offset_x, g.offset_y, g.offset_z], np.float64)
+    dst = np.array([g.ds, g.dt], np.float64)
+    ost = np.array([g.offset_s, g.offset_t], np.float64)
+    ok = lib.ref_fdk_ts_back_helix(
+        _ptr(out, C.c_float), _ptr(nxyz, C.c_int32), _ptr(dxyz, C.c_double),
+        _ptr(cxyz, C.c_double), _ptr(mask, C.c_uint8), C.c_double(g.dso),
+        C.c_double(g.dsd), C.c_double(g.dfs), _ptr(dst, C.c_double), _ptr(ost, C.c_double),
+        _ptr(proj, C.c_float), C.c_int32(g.ns), C.c_int32(g.nt), C.c_int32(na),
+        _ptr(beta, C.c_double), _ptr(zs, C.c_double), C.c_int32(na1_half),
+        C.c_int32(cone_par), C.c_int32(w3d), C.c_double(source_dz_per_rad),
+        C.c_float(pitch), C.c_int32(nthread))
+    if not ok:
+        raise RuntimeError("reference fdk_ts_back_t failed")
     return out
 
 

@@ -14,7 +14,8 @@
  *   - Built with -DREF_HELIX it drives the debugged kernel of
  *     mex/src/fdk-helix/mex/fdk,ts,t.c:107-167 on its axial path
  *     (source_zs = 0, na1_half = 0, cone_par = 0, w3d = 0,
- *     source_dz_per_rad = 0, pitch = 0) -- the "fixed" reference.
+ *     source_dz_per_rad = 0, pitch = 0) -- the "fixed" reference -- and,
+ *     through ref_fdk_ts_back_helix, with all of its helical arguments.
  *   - Without it, it drives mex/src/fdk/fdk-orig/fdk,ts,t.c:72-110, the
  *     "as-shipped" reference with the swapped vertical weights.
  *
@@ -75,6 +76,48 @@ int ref_fdk_ts_back(
 	free(ig->mask2);
 	return ok;
 }
+
+#ifdef REF_HELIX
+/* The 19-operand 'fdk,ts,back' of the fdk-helix gateway
+ * (mex/src/fdk-helix/mex/fdk,mex.c:22-176, 227-236): as ref_fdk_ts_back plus
+ * source_zs [na], na1_half, cone_par, w3d, source_dz_per_rad (narrowed to
+ * float by the gateway, fdk,mex.c:128) and pitch (single). */
+int ref_fdk_ts_back_helix(
+	float *image,
+	const int32_t *nxyz, const double *dxyz, const double *cxyz,
+	const uint8_t *mask,
+	double dso, double dsd, double dfs,
+	const double *dst, const double *ost,
+	const float *proj, int32_t ns, int32_t nt, int32_t na,
+	const double *beta, const double *source_zs, int32_t na1_half,
+	int32_t cone_par, int32_t w3d, double source_dz_per_rad, float pitch,
+	int32_t nthread)
+{
+	cbct_ig ig_, *ig = &ig_;
+	cbct_cg cg_, *cg = &cg_;
+	const int nx = nxyz[0], ny = nxyz[1], nz = nxyz[2];
+	const float dz_per_rad = (float) source_dz_per_rad;
+	int ok;
+
+	ig->mask2 = (byte *) calloc((size_t) nx * ny, 1);
+	if (!ig->mask2) return 0;
+	if (!cbct_mask_init(ig->mask2, (cbyte *) mask, nx, ny, nthread, 0)) {
+		free(ig->mask2);
+		return 0;
+	}
+	ig->nx = nx; ig->ny = ny; ig->nz = nz;
+	ig->dx = dxyz[0]; ig->dy = dxyz[1]; ig->dz = dxyz[2];
+	ig->offset_x = cxyz[0]; ig->offset_y = cxyz[1]; ig->offset_z = cxyz[2];
+	cg->dso = dso; cg->dsd = dsd; cg->dfs = dfs;
+	cg->ns = ns; cg->nt = nt;
+	cg->ds = dst[0]; cg->dt = dst[1];
+	cg->offset_s = ost[0]; cg->offset_t = ost[1];
+	ok = fdk_ts_back_t(image, ig, cg, na, proj, beta, source_zs, na1_half,
+		cone_par, w3d, dz_per_rad, pitch, nthread, 0);
+	free(ig->mask2);
+	return ok;
+}
+#endif
 
 /* Mirrors the 'fdk,st,back' branch of fdk_mex_back (fdk,mex.c:93-100,139-156):
  * proj is [ns nt na] (s fastest), image is [nx ny nz] and is zeroed first

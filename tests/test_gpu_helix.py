@@ -1,0 +1,141 @@
+"""GPU: the helical / cone-parallel 'fdk,ts,back' (fdkhip_fdk_ts_back_helix)
+against the reference C and its restatement.
+
+Bar: bit for bit -- the kernel does the reference's float operations in its
+order (doubles where it uses double: atan2, asin, the square root of its
+pow(., 0.5)):
+* the fixtures the reference C wrote (tests/golden/helix);
+* seeded native helical flat/arc cases, cone-parallel ones with and without
+  w3d weights, helical and axial, masks and offsets, against the
+  restatement (and the reference C when oracle/_ref is built);
+* a larger helical scan (96 x 96 x 64, 3 turns of 120 views);
+* the device-resident entry equals the host one; cbct_back_helix's default
+  (GPU) route equals the same host steps with the restatement as back_call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from test_oracle_helix import HELIX_GRID, _seeded, load_helix
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HELIX_GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "helix", "*.npz")))
+
+
+def gpu_helix(g, proj, beta, zs, mask, *, na1_half, cone_par, w3d, source_dz_per_rad,
+              pitch):
+    """fdkhip_fdk_ts_back_helix through the gateway mirror; C-order arrays of
+    the oracle's conventions in and out ((na, ns, nt) views, (ny, nx, nz))."""
+    from mirt_amd import fdk_mex_helix
+    na = proj.shape[0]
+    ptsa = np.asfortranarray(np.asarray(proj, np.float32).transpose(2, 1, 0))   # [nt ns na]
+    m = np.ones((g.nx, g.ny), np.uint8) if mask is None else \
+        np.asfortranarray(np.asarray(mask, np.uint8).reshape(g.ny, g.nx).T)
+    img = fdk_mex_helix(
+        "fdk,ts,back", np.array([g.nx, g.ny, g.nz], np.int32),
+        np.array([g.dx, g.dy, g.dz]), np.array([g.offset_x, g.offset_y, g.offset_z]), m,
+        g.dso, g.dsd, g.dfs, np.array([g.ds, g.dt]), np.array([g.offset_s, g.offset_t]),
+        ptsa, np.ascontiguousarray(beta, np.float64), np.ascontiguousarray(zs, np.float64),
+        np.array([na1_half], np.int32), np.array([cone_par], np.int32),
+        np.array([w3d], np.int32), np.array([source_dz_per_rad]),
+        np.array([pitch], np.float32), np.array([1], np.int32))
+    assert img.shape == (g.nz, g.nx, g.ny) and na == beta.size
+    return np.ascontiguousarray(img.transpose(2, 1, 0))                         # (ny, nx, nz)
+
+
+@pytest.mark.parametrize("path", HELIX_GOLDEN, ids=[os.path.basename(p) for p in HELIX_GOLDEN])
+def test_helix_gpu_matches_golden(device, path):
+    g, d, ops = load_helix(path)
+    print(f"helix golden {os.path.basename(path)}", flush=True)
+    got = gpu_helix(g, d["proj"], d["beta"], d["source_zs"], d["mask"], **ops)
+    assert np.array_equal(got, d["out_fixed"]), float(np.abs(got - d["out_fixed"]).max())
+
+
+@pytest.mark.parametrize("dfs,pitch,cone_par,w3d", HELIX_GRID)
+def test_helix_gpu_matches_restatement(device, dfs, pitch, cone_par, w3d):
+    g, proj, beta, zs, mask, ops = _seeded(dfs, pitch=pitch, cone_par=cone_par, w3d=w3d,
+                                           seed=int(10 * pitch) + 3 * cone_par + w3d)
+    print(f"helix dfs={dfs} pitch={pitch} cone_par={cone_par} w3d={w3d}", flush=True)
+    got = gpu_helix(g, proj, beta, zs, mask, **ops)
+    want = O.back_ts_helix_f32(g, proj, beta, zs, mask, **ops)
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+    if O.have_reference():
+        assert np.array_equal(got, O.ref_back_ts_helix(g, proj, beta, zs, mask, nthread=4,
+                                                       **ops))
+
+
+def test_helix_gpu_parallel_beam(device):
+    g, proj, beta, zs, mask, ops = _seeded(0.0, pitch=1.0, cone_par=0, w3d=0, seed=5,
+                                           dso=np.inf, dsd=np.inf)
+    got = gpu_helix(g, proj, beta, zs, mask, **ops)
+    assert np.array_equal(got, O.back_ts_helix_f32(g, proj, beta, zs, mask, **ops))
+
+
+def test_helix_gpu_larger_scan(device):
+    """96 x 96 x 64 at 1 mm, 3 turns of 120 views, pitch 1, arc detector:
+    several slice chunks per column and view windows that cut them."""
+    rng = np.random.default_rng(77)
+    g = O.geom((96, 96, 64), (1.0, -1.0, 1.0), (0.0, 0.0, 0.0), 541.0, 949.075, 0.0,
+               128, 32, (1.3, 1.2), (1.25, 0.0))
+    na360, na = 120, 360
+    zfov = 541.0 / 949.075 * 32 * 1.2
+    dz_view = zfov / na360
+    zs = (np.arange(na) - na / 2) * dz_view
+    beta = 2 * np.pi * np.arange(na) / na360
+    proj = rng.standard_normal((na, 128, 32)).astype(np.float32)
+    ops = dict(na1_half=na360 // 2, cone_par=0, w3d=0,
+               source_dz_per_rad=dz_view / (2 * np.pi / na360), pitch=1.0)
+    print("helix 96x96x64, 360 views", flush=True)
+    got = gpu_helix(g, proj, beta, zs, None, **ops)
+    want = O.back_ts_helix_f32(g, proj, beta, zs, None, **ops)
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+    assert (np.abs(want) > 0).mean() > 0.5
+
+
+def test_helix_direct_equals_host(device):
+    import torch
+    from mirt_amd import _lib
+    from mirt_amd._lib import Geom
+    lib = _lib.load()
+    g, proj, beta, zs, mask, ops = _seeded(np.inf, pitch=1.0, cone_par=1, w3d=1, seed=3)
+    want = gpu_helix(g, proj, beta, zs, mask, **ops)
+    pt = torch.from_numpy(np.ascontiguousarray(proj)).to(device)
+    img = torch.empty((g.ny, g.nx, g.nz), dtype=torch.float32, device=device)
+    gg = Geom(*[getattr(g, f[0]) for f in Geom._fields_])
+    b = np.ascontiguousarray(beta, np.float64)
+    z = np.ascontiguousarray(zs, np.float64)
+    m = np.ascontiguousarray(mask, np.uint8)                           # (ny, nx) = [nx ny]
+    ms = C.c_float(-1)
+    s = torch.cuda.current_stream()
+    P = lambda a: C.c_void_p(a.ctypes.data)
+    _lib.check(lib.fdkhip_fdk_ts_back_helix_direct(
+        C.byref(gg), C.c_void_p(pt.data_ptr()), proj.shape[0], P(b), P(z), ops["na1_half"],
+        ops["cone_par"], ops["w3d"], ops["source_dz_per_rad"], ops["pitch"], P(m),
+        C.c_void_p(img.data_ptr()), C.c_void_p(s.cuda_stream), C.byref(ms)))
+    assert np.array_equal(img.cpu().numpy(), want)
+    assert ms.value > 0
+
+
+def test_cbct_back_helix_gpu_route(device):
+    from mirt_amd import cbct_back_helix
+    from mirt_amd.geom import ct_geom, image_geom
+    from test_helix_host import helix_back_call
+    cg = ct_geom("fan", ns=48, nt=16, na=96, ds=1.5, dt=1.6, offset_s=0.25, dsd=500.0,
+                 dso=300.0, dfs=np.inf, orbit=720, pitch=0.9, source_z0="center")
+    ig = image_geom(nx=20, ny=18, nz=24, dx=2.0, dz=1.0, mask="all-but-edge")
+    proj = np.asfortranarray(np.random.default_rng(9).standard_normal(cg.dim)
+                             .astype(np.float32))
+    for kw in (dict(ia_skip=1), dict(ia_skip=2, cone_par=1, w3d=1)):
+        with pytest.warns(UserWarning):
+            got = cbct_back_helix(proj, cg, ig, **kw)
+        with pytest.warns(UserWarning):
+            want = cbct_back_helix(proj, cg, ig, back_call=helix_back_call, **kw)
+        assert np.array_equal(got, want)
