@@ -417,6 +417,162 @@ def run_mat(args, dev):
     return result
 
 
+def helix_update_count(cg, ig, beta, zs, izb, dev, z_range=None):
+    """Voxel-view updates of the reference's helical loop (fdk,ts.c:112-245:
+    columns whose s bin is in [-1, ns), slices [iz_min, iz_max) inside the
+    view window), evaluated in float32 with torch on the device -- the count
+    behind the updates/s and the roofline of the helical runs (float rounding
+    of the slice bounds may move it by a few 1e-6)."""
+    import torch
+    f = dict(device=dev, dtype=torch.float32)
+    nx, ny, nz = ig.nx, ig.ny, ig.nz
+    wx = (nx - 1) / 2 + ig.offset_x
+    wy = (ny - 1) / 2 + ig.offset_y
+    wz = (nz - 1) / 2 + ig.offset_z
+    ws = (cg.ns - 1) / 2 + cg.offset_s
+    wt = (cg.nt - 1) / 2 + cg.offset_t
+    xx = (ig.dx * (torch.arange(nx, **f) - wx))[None, None, :]
+    yy = (ig.dy * (torch.arange(ny, **f) - wy))[None, :, None]
+    m = torch.as_tensor(np.ascontiguousarray(np.asarray(ig.mask_or).T), device=dev)
+    lo_z, hi_z = z_range or (0, nz)
+    total = 0
+    for v0 in range(0, len(beta), 32):
+        b = torch.as_tensor(beta[v0:v0 + 32], **f)[:, None, None]
+        z = torch.as_tensor(zs[v0:v0 + 32], **f)[:, None, None]
+        i0 = torch.as_tensor(izb[0][v0:v0 + 32], device=dev)[:, None, None]
+        i1 = torch.as_tensor(izb[1][v0:v0 + 32], device=dev)[:, None, None]
+        cb, sb = torch.cos(b), torch.sin(b)
+        xb = xx * cb + yy * sb
+        yb = cg.dso - (-xx * sb + yy * cb)
+        mag = cg.dsd / yb
+        ss = cg.dsd * torch.atan2(xb, yb) if cg.dfs == 0 else mag * xb
+        isb = torch.floor(ss / cg.ds + ws)
+        ok = (isb >= -1) & (isb < cg.ns) & (m[None] != 0)
+        t_inc = mag * ig.dz / cg.dt
+        wzb = wz + z / ig.dz
+        izmin = torch.floor(wzb - (wt + 1) / t_inc) + 1
+        izmax = torch.ceil(wzb + (cg.nt - wt) / t_inc)
+        izmin = torch.maximum(torch.maximum(izmin, i0.to(izmin.dtype)),
+                              torch.tensor(float(lo_z), **f))
+        izmax = torch.minimum(torch.minimum(izmax, i1.to(izmax.dtype)),
+                              torch.tensor(float(hi_z), **f))
+        total += int(torch.where(ok, (izmax - izmin).clamp(min=0), 0).sum().item())
+    return total
+
+
+def helix_windows(cg, ig, na1_half):
+    """iz_min_beta / iz_max_beta of every view (fdk,ts,t.c:78-85)."""
+    zs = np.asarray(cg.source_zs, np.float64)
+    na = len(zs)
+    wz = np.float32((ig.nz - 1) / 2 + np.float32(ig.offset_z))
+    dz = np.float32(ig.dz)
+    a0 = np.maximum(np.arange(na) - na1_half, 0)
+    a1 = np.minimum(np.arange(na) + na1_half, na - 1)
+    c0 = np.ceil((wz + zs[a0] / dz).astype(np.float32))
+    c1 = np.ceil((wz + zs[a1] / dz).astype(np.float32))
+    return np.maximum(c0, 0).astype(np.int64), np.minimum(c1, ig.nz).astype(np.int64)
+
+
+def run_helix(args, dev):
+    """The helical 'fdk,ts,back' of the fdk-helix kernel on a helical scan
+    (clinical_helix: 3 turns of the clinical_arc detector): views [nt ns na]
+    FP32 and the [nz nx ny] image resident on the device
+    (fdkhip_fdk_ts_back_helix_direct), native helical geometry."""
+    import ctypes as C
+    import torch
+    from mirt_amd import _lib, geom_of
+    from mirt_amd.cbct_back import _views
+    from mirt_amd.cbct_back_helix import helix_operands
+    from mirt_amd.workloads import WORKLOADS, filtered_views
+
+    lib = _lib.load()
+    name = args.workload if args.workload != "clinical_arc" else "clinical_helix"
+    scan = WORKLOADS[name]()
+    cg, ig = scan.cg, scan.ig
+    t0 = time.perf_counter()
+    tsa = filtered_views(scan, dev)                       # (na, ns, nt)
+    torch.cuda.synchronize()
+    log(f"inputs ready in {time.perf_counter() - t0:.1f} s")
+    beta = np.ascontiguousarray(_views(cg, cg.na), np.float64)
+    zs = np.ascontiguousarray(cg.source_zs, np.float64)
+    na1_half, dpr = helix_operands(cg, 1)
+    mask = np.ascontiguousarray(np.asarray(ig.mask_or).T).astype(np.uint8)
+    img = torch.empty((ig.ny, ig.nx, ig.nz), dtype=torch.float32, device=dev)
+    geom = geom_of(cg, ig)
+    s = torch.cuda.current_stream()
+    kms = []
+
+    def step():
+        ms = C.c_float(0)
+        _lib.check(lib.fdkhip_fdk_ts_back_helix_direct(
+            C.byref(geom), C.c_void_p(tsa.data_ptr()), cg.na,
+            beta.ctypes.data_as(C.c_void_p), zs.ctypes.data_as(C.c_void_p), na1_half, 0, 0,
+            dpr, float(cg.pitch), mask.ctypes.data_as(C.c_void_p), C.c_void_p(img.data_ptr()),
+            C.c_void_p(s.cuda_stream), C.byref(ms)))
+        kms.append(ms.value)
+
+    ms = time_steps(step, args.steps, args.warmup)
+    kernel_ms = float(np.mean(kms[args.warmup:]))
+    izb = helix_windows(cg, ig, na1_half)
+    U = helix_update_count(cg, ig, beta, zs, izb, dev)
+    U_nom = ig.nx * ig.ny * ig.nz * cg.na
+    achieved = FLOP_PER_UPDATE * U / (kernel_ms * 1e-3) / 1e12
+    log(f"helix: {ms:.3f} ms/step, kernel {kernel_ms:.3f} ms, {U:.4e} updates "
+        f"({U / ms * 1e3:.3e}/s)")
+    result = {
+        "metric": f"voxel-view updates/s (in-window updates), {name}, helical fdk,ts,back",
+        "value": U / (ms * 1e-3), "unit": "voxel-view updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: {len(scan.ellipsoids)} seeded ellipsoids, analytic helical arc "
+                f"cone-beam line integrals, Poisson I0={scan.i0:g}, FDK weight + ramp filter",
+        "config": {"workload": f"{name}: arc {cg.ns}x{cg.nt} det, {cg.na} views over "
+                               f"{cg.orbit:g} deg, pitch {cg.pitch:g} -> "
+                               f"{ig.nx}x{ig.ny}x{ig.nz} ({ig.dx:g}/{ig.dz:g} mm)",
+                   "nx": ig.nx, "ny": ig.ny, "nz": ig.nz, "na": cg.na, "na1_half": na1_half},
+        "updates_in_window": U, "updates_nominal": U_nom,
+        "gups_nominal": U_nom / (ms * 1e-3) / 1e9,
+        "kernel_ms": kernel_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": None,
+                     "note": f"{FLOP_PER_UPDATE} FP32 flop x {U} in-window updates / "
+                             f"{kernel_ms:.3f} ms helix_kernel (HIP events on its stream)"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        if O.have_reference("fixed"):
+            nthread = max(1, min(16, host_cores()))
+            tsa_host = tsa.cpu().numpy()
+            g = oracle_geom(cg, ig)
+
+            def slab(nzs):
+                z0 = (ig.nz - nzs) // 2      # central slices, same z coordinates
+                off = ig.offset_z + (ig.nz - 1) / 2 - z0 - (nzs - 1) / 2
+                gs = O.geom((ig.nx, ig.ny, nzs), (ig.dx, ig.dy, ig.dz),
+                            (ig.offset_x, ig.offset_y, off), cg.dso, cg.dsd, cg.dfs,
+                            cg.ns, cg.nt, (cg.ds, cg.dt), (cg.offset_s, cg.offset_t))
+                t = time.perf_counter()
+                O.ref_back_ts_helix(gs, tsa_host, beta, zs, mask, na1_half=na1_half,
+                                    source_dz_per_rad=dpr, pitch=float(cg.pitch),
+                                    nthread=nthread)
+                return time.perf_counter() - t, z0
+
+            tp, _ = slab(8)                  # per-view set-up included: conservative
+            nzs = int(max(8, min(ig.nz, 8 * 15.0 / max(tp, 1e-3))))
+            t, z0 = slab(nzs)
+            u = helix_update_count(cg, ig, beta, zs, izb, dev, z_range=(z0, z0 + nzs))
+            result["cpu_baseline"] = {
+                "value": u / t, "unit": "voxel-view updates/s (in-window)",
+                "cores": nthread, "kind": "reference",
+                "sample": f"reference C (fdk-helix fdk_ts_back_t, gcc -O3, {nthread} "
+                          f"pthreads), all {cg.na} views, central {nzs} of {ig.nz} slices "
+                          f"({u:.3e} in-window updates) in {t:.2f} s; host CPU {cpu_model()}",
+                "seconds": t}
+    return result
+
+
 def slab_times(args, geom, beta, mask, ig, tsa, head, cfg, axis):
     """A G-way split of the volume for G GPUs, shown on this GPU: part g is
     the same scan with nz/G slices (axis "z": z-slabs) or ny/G image rows
@@ -565,10 +721,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="clinical_arc",
                     choices=["clinical_arc", "plain_flat", "large_flat", "cpu_small",
-                             "uneven_scans"])
-    ap.add_argument("--path", choices=list(PATHS) + ["mat"], default=None,
+                             "uneven_scans", "clinical_helix"])
+    ap.add_argument("--path", choices=list(PATHS) + ["mat", "helix"], default=None,
                     help="storage/math path of the headline (default per workload); "
-                         "mat: cbct_back_mat, the FP64 use_mex=0 route")
+                         "mat: cbct_back_mat, the FP64 use_mex=0 route; helix: the "
+                         "helical fdk,ts,back (workload clinical_helix)")
     ap.add_argument("--nscan", type=int, default=32, help="uneven_scans: number of scans")
     ap.add_argument("--slabs", type=int, default=0, metavar="G",
                     help="also time the G z-slabs a G-GPU split would give each GPU")
@@ -605,6 +762,8 @@ def main():
         return
     if args.path == "mat":
         result = run_mat(args, dev)
+    elif args.path == "helix" or args.workload == "clinical_helix":
+        result = run_helix(args, dev)
     elif args.workload == "uneven_scans":
         result = run_batch(args, dev)
     else:

@@ -64,7 +64,13 @@ struct HelixView {
 	int izb0, izb1;          // iz_min_beta, iz_max_beta (fdk,ts,t.c:78-85)
 	int pad[3];
 };
-constexpr int HX_ZC = 16;    // slices per lane
+#ifndef HX_ZC_SLICES
+#define HX_ZC_SLICES 32
+#endif
+constexpr int HX_ZC = HX_ZC_SLICES;   // slices per lane (a multiple of 8)
+#ifndef HX_ACC_LDS
+#define HX_ACC_LDS 1                  // accumulators in LDS (else registers)
+#endif
 struct HelixArgs {
 	GeomF g;
 	const float *proj;       // [na][ns][nt] (t fastest)

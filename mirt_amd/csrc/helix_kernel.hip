@@ -16,7 +16,8 @@
 //
 // Launch: one wave per 8 x 8 tile of columns and HX_ZC slices, lane =
 // column, HX_ZC accumulators in registers over the views of the chunk's
-// window (chunk_views, from the host).  Samples are read straight from the
+// window (chunk_views, from the host); groups of 8 slices no lane of the wave
+// reaches are skipped per view.  Samples are read straight from the
 // [na][ns][nt] views with the reference's zero padding as bounds checks.
 // Deviations, as in the axial kernel: samples whose row is outside
 // [-1, nt-1] (which the reference reads from beyond its padded view buffer)
@@ -70,6 +71,41 @@ __device__ __forceinline__ float hx_sample(const float *view, int ns, int nt, in
 	return (s >= 0 && s < ns && t >= 0 && t < nt) ? view[(long) s * nt + t] : 0.f;
 }
 
+// One slice update of fdk,ts.c:177-236 at slice iz into accumulator ACC:
+// bilinear sample (the view's zero padding as bounds checks), the cone-
+// parallel w3d weight of the conjugate ray, ACC += w2 * p0, t_bin += t_inc.
+#define HX_UPDATE(ACC)                                                           \
+	do {                                                                         \
+		const int it = hx_to_int(floorf(t_bin));                                 \
+		const float wu = t_bin - (float) it;                                     \
+		const float wd = 1.f - wu;                                               \
+		float p0 = 0.f;                                                          \
+		if (it >= -1 && it < g.nt) {                                             \
+			const float a00 = hx_sample(view, g.ns, g.nt, is, it);               \
+			const float a10 = hx_sample(view, g.ns, g.nt, is + 1, it);           \
+			const float a01 = hx_sample(view, g.ns, g.nt, is, it + 1);           \
+			const float a11 = hx_sample(view, g.ns, g.nt, is + 1, it + 1);       \
+			const float p1 = wl * a00 + wr * a10;                                \
+			const float p2 = wl * a01 + wr * a11;                                \
+			p0 = wd * p1 + wu * p2;                                              \
+		}                                                                        \
+		float w2 = w2f;                                                          \
+		if (a.cone_par) {                                                        \
+			float ww = 1.f;                                                      \
+			if (weigh) {                                                         \
+				const bool up = t_bin >= g.wt;                                   \
+				const float zs_c = up ? zsc_hi : zsc_lo;                         \
+				const float wzbeta_c = up ? wzc_hi : wzc_lo;                     \
+				const float t_bin_c = t_inc_c * ((float) iz - wzbeta_c) + g.wt;  \
+				if (zs_c >= a.zs_min && zs_c <= a.zs_max)                        \
+					ww = hx_w3d(a, t_bin, t_bin_c, iz);                          \
+			}                                                                    \
+			w2 = ww;                                                             \
+		}                                                                        \
+		(ACC) += w2 * p0;                                                        \
+		t_bin += t_inc;                                                          \
+	} while (0)
+
 __global__ void __launch_bounds__(64) helix_kernel(const HelixArgs a)
 {
 	const GeomF &g = a.g;
@@ -83,10 +119,19 @@ __global__ void __launch_bounds__(64) helix_kernel(const HelixArgs a)
 	const long col = ix + (long) iy * g.nx;
 	const int z0 = bz * HX_ZC;
 	const int nk = min(HX_ZC, g.nz - z0);
+#if HX_ACC_LDS
+	// the lane's accumulators, slice-major so that lanes at different slices
+	// still hit different banks; each lane touches only its own entries
+	__shared__ float sacc[HX_ZC * 64];
+	float *acc = sacc + threadIdx.x;
+	for (int k = 0; k < HX_ZC; ++k)
+		acc[k * 64] = 0.f;
+#else
 	float acc[HX_ZC];
 #pragma unroll
 	for (int k = 0; k < HX_ZC; ++k)
 		acc[k] = 0.f;
+#endif
 
 	if (!a.mask || a.mask[col]) {
 		const float yy = g.dy * ((float) iy - g.wy);
@@ -140,6 +185,8 @@ __global__ void __launch_bounds__(64) helix_kernel(const HelixArgs a)
 				continue;
 			for (int iz = iz_min; iz < lo; ++iz)   // the running sum up to this chunk
 				t_bin += t_inc;
+			// from here t_bin is the value at slice lo and advances once per
+			// slice of [lo, hi), in slice order
 			// conjugate ray (fdk,ts.c:169-175) and its heights, for w3d
 			float t_inc_c = 0.f, wzc_hi = 0.f, wzc_lo = 0.f, zsc_hi = 0.f, zsc_lo = 0.f;
 			const bool weigh = a.cone_par && a.w3d;
@@ -162,48 +209,37 @@ __global__ void __launch_bounds__(64) helix_kernel(const HelixArgs a)
 			const float w2f = (g.kind == KIND_ARC)
 				? (g.dsd * g.dsd) / (ybetas * ybetas + xbeta * xbeta)
 				: mag * mag;
+#if HX_ACC_LDS
+			for (int iz = lo; iz < hi; ++iz) {
+				HX_UPDATE(acc[(iz - z0) * 64]);
+			}
+#else
 #pragma unroll
-			for (int k = 0; k < HX_ZC; ++k) {
-				const int iz = z0 + k;
-				if (iz >= lo && iz < hi) {
-					// fdk,ts.c:177-236
-					const int it = hx_to_int(floorf(t_bin));
-					const float wu = t_bin - (float) it;
-					const float wd = 1.f - wu;
-					float p0 = 0.f;
-					if (it >= -1 && it < g.nt) {
-						const float a00 = hx_sample(view, g.ns, g.nt, is, it);
-						const float a10 = hx_sample(view, g.ns, g.nt, is + 1, it);
-						const float a01 = hx_sample(view, g.ns, g.nt, is, it + 1);
-						const float a11 = hx_sample(view, g.ns, g.nt, is + 1, it + 1);
-						const float p1 = wl * a00 + wr * a10;
-						const float p2 = wl * a01 + wr * a11;
-						p0 = wd * p1 + wu * p2;
+			for (int kb = 0; kb < HX_ZC; kb += 8) {
+				// groups of 8 slices that no lane of the wave adds to are skipped
+				if (!__any(lo < z0 + kb + 8 && hi > z0 + kb))
+					continue;
+#pragma unroll
+				for (int k = kb; k < kb + 8; ++k) {
+					const int iz = z0 + k;
+					if (iz >= lo && iz < hi) {
+						HX_UPDATE(acc[k]);
 					}
-					float w2 = w2f;
-					if (a.cone_par) {
-						float ww = 1.f;
-						if (weigh) {
-							const bool up = t_bin >= g.wt;
-							const float zs_c = up ? zsc_hi : zsc_lo;
-							const float wzbeta_c = up ? wzc_hi : wzc_lo;
-							const float t_bin_c = t_inc_c * ((float) iz - wzbeta_c) + g.wt;
-							if (zs_c >= a.zs_min && zs_c <= a.zs_max)
-								ww = hx_w3d(a, t_bin, t_bin_c, iz);
-						}
-						w2 = ww;
-					}
-					acc[k] += w2 * p0;
-					t_bin += t_inc;
 				}
 			}
+#endif
 		}
 	}
 	float *out = a.image + col * g.nz + z0;
+#if HX_ACC_LDS
+	for (int k = 0; k < nk; ++k)
+		out[k] = acc[k * 64];
+#else
 #pragma unroll
 	for (int k = 0; k < HX_ZC; ++k)
 		if (k < nk)
 			out[k] = acc[k];
+#endif
 }
 
 } // namespace

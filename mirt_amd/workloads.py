@@ -17,6 +17,10 @@ filter).
 * ``large_flat``    flat 1024 x 768 at 0.4 mm, 1440 views, 1024 x 1024 x 512 at
                      0.25 mm, 40 ellipsoids plus small high-contrast inserts,
                      Poisson I0 = 1e5;
+* ``clinical_helix`` the ``clinical_arc`` detector on a helix: 3 turns of
+                     984 views at pitch 1 (source_z0 centred), 512 x 512 x 192
+                     at 0.7 / 0.625 mm (the 120 mm the source travels),
+                     30 ellipsoids, Poisson I0 = 2e5;
 * ``uneven_scans``  seeded scans of uneven size: volume edge in
                      {128, 192, 256, 384} with nz = edge/2, views in [180, 720],
                      flat detector columns in [256, 768] and rows in
@@ -42,10 +46,11 @@ class Scan:
 
 
 def _scan(name, seed, *, ns, nt, na, ds, dt, dfs, nxyz, dx, dz, n_ell, i0,
-          offset_s=0.0, inserts=0):
+          offset_s=0.0, inserts=0, turns=1, pitch=0.0):
     from .phantom import random_ellipsoids
+    helix = dict(pitch=pitch, source_z0="center") if pitch else {}
     cg = ct_geom("fan", ns=ns, nt=nt, na=na, ds=ds, dt=dt, offset_s=offset_s,
-                 dsd=949.0, dso=541.0, dfs=dfs, orbit=360)
+                 dsd=949.0, dso=541.0, dfs=dfs, orbit=360 * turns, **helix)
     ig = image_geom(nx=nxyz[0], ny=nxyz[1], nz=nxyz[2], dx=dx, dz=dz)
     ell = random_ellipsoids(seed, n_ell, ig.nx * ig.dx, ig.nz * ig.dz, inserts=inserts)
     return Scan(name, cg, ig, ell, i0, seed)
@@ -65,6 +70,12 @@ def clinical_arc(seed: int = 2000) -> Scan:
     return _scan("clinical_arc", seed, ns=888, nt=64, na=984, ds=1.0239, dt=1.0964,
                  dfs=0.0, offset_s=1.25, nxyz=(512, 512, 256), dx=0.7, dz=0.625,
                  n_ell=30, i0=2e5)
+
+
+def clinical_helix(seed: int = 2500) -> Scan:
+    return _scan("clinical_helix", seed, ns=888, nt=64, na=3 * 984, ds=1.0239, dt=1.0964,
+                 dfs=0.0, offset_s=1.25, nxyz=(512, 512, 192), dx=0.7, dz=0.625,
+                 n_ell=30, i0=2e5, turns=3, pitch=1.0)
 
 
 def large_flat(seed: int = 3000) -> Scan:
@@ -96,7 +107,8 @@ def uneven_scans(n: int = 32, seed: int = 4000) -> list[Scan]:
 
 
 WORKLOADS = {"cpu_small": cpu_small, "plain_flat": plain_flat,
-             "clinical_arc": clinical_arc, "large_flat": large_flat}
+             "clinical_arc": clinical_arc, "large_flat": large_flat,
+             "clinical_helix": clinical_helix}
 
 
 def filtered_views(scan: Scan, device):

@@ -471,6 +471,17 @@ def test_workloads_match_the_benchmark_configurations():
         assert np.all((mu >= 0.005) & (mu <= 0.02)), name
         assert np.all(sc.ellipsoids[nell:, 8] == 0.05), name
     assert WORKLOADS["clinical_arc"]().cg.offset_s == 1.25
+    hx = WORKLOADS["clinical_helix"]()
+    cg, ig = hx.cg, hx.ig
+    assert (cg.ns, cg.nt, cg.na, cg.ds, cg.dt, cg.offset_s) == (888, 64, 2952, 1.0239,
+                                                                 1.0964, 1.25)
+    assert cg.dfs == 0 and cg.orbit == 1080 and cg.pitch == 1.0
+    assert (ig.nx, ig.ny, ig.nz, ig.dx, ig.dz) == (512, 512, 192, 0.7, 0.625)
+    # the source travels 3 turns x (pitch * zfov) = the volume's 120 mm, centred
+    zs = cg.source_zs
+    assert zs[-1] - zs[0] == pytest.approx(3 * cg.zfov * (1 - 1 / cg.na), rel=1e-12)
+    assert abs(zs[0] + zs[-1]) < cg.source_dz_per_view + 1e-12
+    assert ig.nz * ig.dz == pytest.approx(3 * cg.zfov, rel=1e-4)
     scans = uneven_scans(32)
     assert len(scans) == 32
     for sc in scans:
