@@ -417,6 +417,11 @@ def run_mat(args, dev):
     return result
 
 
+# launch shape of the helical kernel as built (slices per lane, LDS footprint
+# floats); the committed profile's traffic is used only for this shape
+HELIX_LAUNCH = "slices=24 footprint=768"
+
+
 def helix_update_count(cg, ig, beta, zs, izb, dev, z_range=None):
     """Voxel-view updates of the reference's helical loop (fdk,ts.c:112-245:
     columns whose s bin is in [-1, ns), slices [iz_min, iz_max) inside the
@@ -515,6 +520,13 @@ def run_helix(args, dev):
     kernel_ms = float(np.mean(kms[args.warmup:]))
     izb = helix_windows(cg, ig, na1_half)
     U = helix_update_count(cg, ig, beta, zs, izb, dev)
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"helix_kernel_{name}_f32.md")
+    if os.path.exists(prof):
+        text = open(prof).read()
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+        if m and re.search(r"^launch: " + re.escape(HELIX_LAUNCH) + "$", text, re.M):
+            traffic = float(m.group(1))
     U_nom = ig.nx * ig.ny * ig.nz * cg.na
     achieved = FLOP_PER_UPDATE * U / (kernel_ms * 1e-3) / 1e12
     log(f"helix: {ms:.3f} ms/step, kernel {kernel_ms:.3f} ms, {U:.4e} updates "
@@ -535,7 +547,7 @@ def run_helix(args, dev):
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None,
+                     "traffic": traffic,
                      "note": f"{FLOP_PER_UPDATE} FP32 flop x {U} in-window updates / "
                              f"{kernel_ms:.3f} ms helix_kernel (HIP events on its stream)"},
         "cpu_baseline": None,

@@ -65,12 +65,9 @@ struct HelixView {
 	int pad[3];
 };
 #ifndef HX_ZC_SLICES
-#define HX_ZC_SLICES 32
+#define HX_ZC_SLICES 24
 #endif
-constexpr int HX_ZC = HX_ZC_SLICES;   // slices per lane (a multiple of 8)
-#ifndef HX_ACC_LDS
-#define HX_ACC_LDS 1                  // accumulators in LDS (else registers)
-#endif
+constexpr int HX_ZC = HX_ZC_SLICES;   // slices per lane
 struct HelixArgs {
 	GeomF g;
 	const float *proj;       // [na][ns][nt] (t fastest)
