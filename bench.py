@@ -389,6 +389,11 @@ def run_mat(args, dev):
     U_mask = nmask * ig.nz * cg.na                        # cbct_back_mat visits masked voxels
     det = "arc" if cg.dfs == 0 else "flat"
     achieved = MAT_FLOP_PER_UPDATE[det] * U_mask / (kernel_ms * 1e-3) / 1e12
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"mat_kernel_{args.workload}_f64.md")
+    if os.path.exists(prof):
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(prof).read(), re.M)
+        traffic = float(m.group(1)) if m else None
     log(f"mat: {ms:.3f} ms/step, kernel {kernel_ms:.3f} ms, {U / ms * 1e3:.3e} updates/s")
     result = {
         "metric": f"voxel-view updates/s (GUPS), {args.workload}, cbct_back_mat "
@@ -404,7 +409,7 @@ def run_mat(args, dev):
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None,
+                     "traffic": traffic,
                      "note": f"{MAT_FLOP_PER_UPDATE[det]} FP64 flop x {U_mask} masked "
                              f"voxel-view updates / {kernel_ms:.3f} ms mat_kernel (HIP "
                              "events on its stream)"},
