@@ -52,6 +52,23 @@ __device__ __forceinline__ int to_int_x86(float f)
 	return (f >= -2147483648.0f && f < 2147483648.0f) ? (int) f : (int) 0x80000000u;
 }
 
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+// (lo, hi) rounded to FP16 (nearest even) in one 32-bit word, lo in bits 0-15
+__device__ __forceinline__ uint32_t pack_f16x2(float lo, float hi)
+{
+	return (uint32_t) __builtin_bit_cast(unsigned short, (_Float16) lo)
+		| ((uint32_t) __builtin_bit_cast(unsigned short, (_Float16) hi) << 16);
+}
+
+// wl*lo + wr*hi of an FP16 column pair and FP16 horizontal weights packed as
+// (wl, wr), accumulated in FP32 by one v_dot2c_f32_f16
+__device__ __forceinline__ float dot_pair_f16(uint32_t pair, uint32_t wpk)
+{
+	return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, pair),
+		__builtin_bit_cast(half2v, wpk), 0.f, false);
+}
+
 // FP32 sample idx of the zero-padded copy of FP32 views
 __device__ __forceinline__ float load_sample(const void *padded, long idx)
 {
@@ -341,7 +358,9 @@ extern __shared__ float fdk_lds[];
 //   slots [vstage][slot]        staged footprints
 //   P     [vstage][64][8]       per (view, column) record, two 16-byte halves
 //                               read with one ds_read_b128 each:
-//           P0 = {cb, wl, wr, w2 | u_first}   cb: LDS word of the lane's left
+//           P0 = {cb, wl, wr, w2 | u_first}   (FP16: wl = the FP16 pair (wl, wr),
+//                                             wr = w2 * view scale)
+//                                             cb: LDS word of the lane's left
 //                                             column at row 1 (walk) / padded
 //                                             row 0 (DIRECT); for views
 //                                             gathered from HBM: is
@@ -415,7 +434,12 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 		const int cb = act ? vv * a.slot + (c.is + 1 - s_lo) * tw + (DIRECT ? 0 : 1) - t_lo : 0;
 		float wl = act ? c.wl : 0.f, wr = act ? c.wr : 0.f;
 		const float w2 = act ? c.w2 : 0.f;
-		if (FAST) {
+		if constexpr (FMT == FMT_F16) {
+			// FP16 horizontal weights, packed; fan-beam weight times view scale
+			const float w2s = w2 * a.scale[v];
+			wl = __uint_as_float(pack_f16x2(wl, wr));
+			wr = w2s;
+		} else if (FAST) {
 			const float sc = (FMT != FMT_F32) ? a.scale[v] : 1.f;
 			wl = w2 * wl * sc;
 			wr = w2 * wr * sc;
@@ -538,29 +562,38 @@ __device__ __forceinline__ float pair_hi(uint32_t w)
 		return __uint_as_float(w & 0xffff0000u);
 }
 
-// The same update for the FP16/BF16 paths in u = t_bin + 1 (> 0): the
-// padded row is the truncation of u, the vertical weight its fraction; c1
-// addresses padded row 0 of the lane's staged column pair (is, is + 1); w2
-// and the view scale are folded into wl/wr.
+// FP16 slice update: the horizontal weights (wl, wr) are FP16 in wpk, the
+// fan-beam weight times the view scale is w2s.  The second row's dot product
+// accumulates onto -q1, giving q2 - q1 for the vertical interpolation
+// q1 + wu*(q2 - q1) in one FMA.
+__device__ __forceinline__ float slice_update_f16(float acc, float u, int c1, uint32_t wpk,
+	float w2s)
+{
+	const int iu = (int) u;
+	const float wu = __builtin_amdgcn_fractf(u);
+	const uint32_t w0 = __float_as_uint(fdk_lds[c1 + iu]);
+	const uint32_t w1 = __float_as_uint(fdk_lds[c1 + iu + 1]);
+	const float q1 = dot_pair_f16(w0, wpk);
+	const float d = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, w1),
+		__builtin_bit_cast(half2v, wpk), -q1, false);
+	return fmaf(w2s, fmaf(wu, d, q1), acc);
+}
+
+// The same update for the BF16 path in u = t_bin + 1 (> 0): the padded row
+// is the truncation of u, the vertical weight its fraction; c1 addresses
+// padded row 0 of the lane's staged column pair (is, is + 1); w2 and the
+// view scale are folded into wl/wr (FP32: BF16 weights would be too coarse).
 template <int FMT>
 __device__ __forceinline__ float slice_update_u(float acc, float u, int c1, float wl, float wr)
 {
+	static_assert(FMT == FMT_BF16, "FP16 views take slice_update_f16");
 	const int iu = (int) u;
 	const float wu = __builtin_amdgcn_fractf(u);
 	const float wd = 1.f - wu;
 	const uint32_t w0 = __float_as_uint(fdk_lds[c1 + iu]);
 	const uint32_t w1 = __float_as_uint(fdk_lds[c1 + iu + 1]);
-	float q1, q2;
-	if constexpr (FMT == FMT_F16) {
-		// fmaf(w, x, 0) keeps the FP16 -> FP32 conversion inside v_fma_mix;
-		// the two products are independent so that their mixed-precision
-		// FMAs overlap instead of forming a chain
-		q1 = fmaf(wl, pair_lo<FMT>(w0), 0.f) + fmaf(wr, pair_hi<FMT>(w0), 0.f);
-		q2 = fmaf(wl, pair_lo<FMT>(w1), 0.f) + fmaf(wr, pair_hi<FMT>(w1), 0.f);
-	} else {
-		q1 = fmaf(wr, pair_hi<FMT>(w0), wl * pair_lo<FMT>(w0));
-		q2 = fmaf(wr, pair_hi<FMT>(w1), wl * pair_lo<FMT>(w1));
-	}
+	const float q1 = fmaf(wr, pair_hi<FMT>(w0), wl * pair_lo<FMT>(w0));
+	const float q2 = fmaf(wr, pair_hi<FMT>(w1), wl * pair_lo<FMT>(w1));
 	acc = fmaf(wd, q1, acc);
 	return fmaf(wu, q2, acc);
 }
@@ -616,8 +649,13 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 				// every lane covers the whole chunk: straight-line slices
 				if constexpr (DIRECT) {
 #pragma unroll
-					for (int j = 0; j < ZL; ++j)
-						acc[j] = slice_update_u<FMT>(acc[j], fmaf((float) j, ti, t0), c1, wl, wr);
+					for (int j = 0; j < ZL; ++j) {
+						const float u = fmaf((float) j, ti, t0);
+						if constexpr (FMT == FMT_F16)
+							acc[j] = slice_update_f16(acc[j], u, c1, __float_as_uint(wl), wr);
+						else
+							acc[j] = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
+					}
 				} else {
 					float t = t0;
 #pragma unroll
@@ -643,7 +681,11 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 						for (int j = g0; j < g0 + 8; ++j) {
 							const bool r = (unsigned) (j - jlo) < nj;
 							const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), uf, ul);
-							const float up = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
+							float up;
+							if constexpr (FMT == FMT_F16)
+								up = slice_update_f16(acc[j], u, c1, __float_as_uint(wl), wr);
+							else
+								up = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
 							acc[j] = r ? up : acc[j];
 						}
 					}
@@ -682,10 +724,18 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 						const int iu = (int) u;
 						const float wu = __builtin_amdgcn_fractf(u);
 						const uint32_t w0 = pc[iu], w1 = pc[iu + 1];
-						const float q1 = fmaf(wr, pair_hi<FMT>(w0), fmaf(wl, pair_lo<FMT>(w0), 0.f));
-						const float q2 = fmaf(wr, pair_hi<FMT>(w1), fmaf(wl, pair_lo<FMT>(w1), 0.f));
-						acc[j] = fmaf(1.f - wu, q1, acc[j]);
-						acc[j] = fmaf(wu, q2, acc[j]);
+						if constexpr (FMT == FMT_F16) {
+							const float bu = wr * wu;
+							acc[j] = fmaf(wr - bu, dot_pair_f16(w0, __float_as_uint(wl)), acc[j]);
+							acc[j] = fmaf(bu, dot_pair_f16(w1, __float_as_uint(wl)), acc[j]);
+						} else {
+							const float q1 = fmaf(wr, pair_hi<FMT>(w0),
+								fmaf(wl, pair_lo<FMT>(w0), 0.f));
+							const float q2 = fmaf(wr, pair_hi<FMT>(w1),
+								fmaf(wl, pair_lo<FMT>(w1), 0.f));
+							acc[j] = fmaf(1.f - wu, q1, acc[j]);
+							acc[j] = fmaf(wu, q2, acc[j]);
+						}
 					}
 				}
 			} else {
