@@ -88,6 +88,16 @@ def profiled_traffic(workload, fmt, cfg, kernel="back_kernel"):
     return float(m.group(1))
 
 
+def profiled_valu_issue(name):
+    """VALU issue utilisation (fraction of the CU's wave64 VALU slots used)
+    stated in the committed rocprofv3 summary profiles/<name>.md, or None."""
+    path = os.path.join(ROOT, "profiles", f"{name}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"(\d+) % VALU issue utilisation", open(path).read())
+    return int(m.group(1)) / 100 if m else None
+
+
 def prepare(tsa, fmt):
     from mirt_amd import quantize_views
     if fmt == "f32":
@@ -236,6 +246,8 @@ def run_single(args, dev):
     kms = kernel[head]["back"]
     achieved = FLOP_PER_UPDATE * U_eff / (kms * 1e-3) / 1e12
     traffic = profiled_traffic(args.workload, head, cfg)
+    valu_issue = (profiled_valu_issue(f"back_kernel_{args.workload}_{head}")
+                  if traffic is not None else None)
     det = "arc" if cg.dfs == 0 else "flat"
     result = {
         "metric": HEADLINE_METRIC if args.workload == "clinical_arc"
@@ -269,11 +281,13 @@ def run_single(args, dev):
         "kernel_ms": kernel,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": traffic,
+                     "traffic": traffic, "valu_issue": valu_issue,
                      "note": f"{FLOP_PER_UPDATE} FP32 flop x {U_eff} in-support updates per "
                              f"launch / {kms:.3f} ms back_kernel (HIP events on its "
                              "stream); traffic = HBM bytes per launch from the "
-                             "rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/"},
+                             "rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/, "
+                             "valu_issue = SQ_INSTS_VALU / (CUs x GRBM_GUI_ACTIVE cycles) "
+                             "from the same summary"},
     }
     if args.slabs and args.slabs > 1:
         result["slabs"] = slab_times(args, geom, beta, mask, ig, tsa, head, cfg, "z")
@@ -553,6 +567,8 @@ def run_helix(args, dev):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic,
+                     "valu_issue": (profiled_valu_issue(f"helix_kernel_{name}_f32")
+                                    if traffic is not None else None),
                      "note": f"{FLOP_PER_UPDATE} FP32 flop x {U} in-window updates / "
                              f"{kernel_ms:.3f} ms helix_kernel (HIP events on its stream)"},
         "cpu_baseline": None,
@@ -691,6 +707,7 @@ def run_batch(args, dev):
 
     kms = prof["back"]
     achieved = FLOP_PER_UPDATE * U_eff / (kms * 1e-3) / 1e12
+    btraffic = profiled_traffic("uneven_scans", head, batch.config, "batch_kernel")
     result = {
         "metric": "voxel-view updates/s (GUPS), uneven_scans batch",
         "value": U / (ms * 1e-3),
@@ -709,11 +726,12 @@ def run_batch(args, dev):
         "kernel_ms": prof,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": profiled_traffic("uneven_scans", head, batch.config,
-                                                 "batch_kernel"),
+                     "traffic": btraffic,
+                     "valu_issue": (profiled_valu_issue(f"batch_kernel_uneven_scans_{head}")
+                                    if btraffic is not None else None),
                      "note": f"{FLOP_PER_UPDATE} FP32 flop x {U_eff} in-support updates / "
-                             f"{kms:.3f} ms batch_kernel (HIP events); traffic from the "
-                             "rocprofv3 summary in profiles/"},
+                             f"{kms:.3f} ms batch_kernel (HIP events); traffic and VALU issue "
+                             "from the rocprofv3 summary in profiles/"},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline and hosts:
