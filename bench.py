@@ -684,6 +684,9 @@ def run_batch(args, dev):
         tuned = batch.autotune(projs, scales, outs, reps=2)
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned:.3f} ms, "
             f"config {batch.config}")
+    launch = dict(batch.config)
+    launch["half_height_scans"] = sum(z < launch["zper"] for z in batch.scan_zper)
+    log(f"per-scan slices per lane: {batch.scan_zper}")
     ms = time_steps(lambda: batch(projs, scales, outs), args.steps, args.warmup)
     prof = batch.profile(projs, scales, outs, reps=max(2, args.steps))
     U = sum(sc.ig.nx * sc.ig.ny * sc.ig.nz * sc.cg.na for sc in scans)
@@ -720,7 +723,7 @@ def run_batch(args, dev):
         "config": {"workload": f"uneven_scans: {len(scans)} scans in one persistent launch "
                                "(edge {128,192,256,384}, nz=edge/2, 180-720 views, flat "
                                "256-768 x 128-512 det)",
-                   "storage": head, "accumulate": "f32", "launch": batch.config},
+                   "storage": head, "accumulate": "f32", "launch": launch},
         "updates_nominal": U, "updates_effective": U_eff,
         "sequential_plans_ms_per_step": ms_seq,
         "kernel_ms": prof,

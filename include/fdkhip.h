@@ -296,12 +296,26 @@ int fdkhip_batch_execute_direct(fdkhip_batch *batch, const void *const *proj,
 int fdkhip_batch_config(const fdkhip_batch *batch, int32_t cfg[6]);
 
 /* Launch shape for the whole batch (a pair the build supports) and an upper
- * bound on views per LDS stage (0 = largest that fits). */
+ * bound on views per LDS stage (0 = largest that fits).  Each scan's slices
+ * per lane are then chosen anew: zper, or zper/2 (zper >= 16) for a scan
+ * whose footprint over waves*zper slices would not fit the LDS slot. */
 int fdkhip_batch_set_config(fdkhip_batch *batch, int32_t waves, int32_t zper,
 	int32_t vstage);
 
+/* Per-scan slices per lane under the current launch shape: zper[i] is the
+ * shape's zper or, where that is >= 16, half of it; NULL restores the
+ * automatic choice of fdkhip_batch_set_config.  Results do not depend on it
+ * (EXACT: same bits); only the z-group height of scan i's work items does.
+ * No reference counterpart (a launch parameter of the batch object). */
+int fdkhip_batch_set_zper(fdkhip_batch *batch, const int32_t *zper);
+
+/* zper[0..nscan) = each scan's slices per lane in the current configuration. */
+int fdkhip_batch_scan_zper(const fdkhip_batch *batch, int32_t *zper);
+
 /* As fdkhip_plan_autotune, over the batch's launch shapes and stage depths
- * (the persistent kernel over the first 128 views of every scan). */
+ * (the persistent kernel over the first 128 views of every scan), each shape
+ * with the automatic per-scan slices per lane and, where those differ from
+ * the shape's, with the shape's for every scan. */
 int fdkhip_batch_autotune(fdkhip_batch *batch, const void *const *proj,
 	const float *const *scale, float *const *image, void *stream, int32_t reps,
 	float *best_ms);

@@ -572,6 +572,21 @@ class FdkBatch:
     def set_config(self, waves, zper, vstage=0):
         _lib.check(self.lib.fdkhip_batch_set_config(self.handle, waves, zper, vstage))
 
+    @property
+    def scan_zper(self):
+        """Each scan's slices per lane (the launch's ``zper`` or half of it)."""
+        z = (C.c_int32 * self.nscan)()
+        _lib.check(self.lib.fdkhip_batch_scan_zper(self.handle, z))
+        return list(z)
+
+    def set_zper(self, zper=None):
+        """Fix each scan's slices per lane (``None``: the automatic choice,
+        half height for scans whose footprint overflows the LDS slot)."""
+        if zper is not None and len(zper) != self.nscan:
+            raise ValueError(f"expected {self.nscan} per-scan values")
+        arr = None if zper is None else (C.c_int32 * self.nscan)(*[int(z) for z in zper])
+        _lib.check(self.lib.fdkhip_batch_set_zper(self.handle, arr))
+
     def count_updates(self) -> int:
         n = C.c_int64()
         _lib.check(self.lib.fdkhip_batch_count_updates(self.handle, C.byref(n)))

@@ -225,6 +225,7 @@ static BackArgs make_args(const fdkhip_plan *p, const void *proj, const float *s
 	a.ngroups = p->ngroups;
 	a.vstage = p->vstage;
 	a.slot = p->slot;
+	a.zper = p->zl;
 	a.count = p->d_count;
 	a.maxarea = p->d_maxarea;
 	a.fov_any = p->d_any;
@@ -1400,6 +1401,10 @@ struct fdkhip_batch {
 	int nscan = 0, fmt = FMT_F32, math = FDKHIP_MATH_FAST;
 	std::vector<fdkhip_plan *> plans;
 	int nw = 8, zl = 16, vstage = 8, slot = 64, nwg = 256;
+	int vstage_req = 0;   // requested views per stage (0: largest that fits)
+	// per-scan slices per consumer wave: chosen by batch_configure
+	// (zper_req empty) or fixed by fdkhip_batch_set_zper
+	std::vector<int> zper_req;
 	long long nitems = 0;
 	std::vector<long long> item_begin;
 	std::vector<BackArgs> host_args;
@@ -1440,29 +1445,76 @@ static int batch_free(fdkhip_batch *b)
 	return FDKHIP_OK;
 }
 
-// common launch shape for every scan, LDS layout sized for the largest one
+// Half the slices per consumer wave, the batch kernel's second height for a
+// launch of zl slices per wave (zl itself where no half height is built).
+static int batch_half_zper(int zl)
+{
+	return zl >= 16 ? zl / 2 : zl;
+}
+
+// Launch shape and LDS layout for the batch, and each scan's z-group height.
+// Without zper_req, every scan starts at zl slices per wave; the LDS layout
+// is chosen for the largest footprint, and a scan whose footprint at zl
+// exceeds that layout's slot moves to zl/2 (its z-groups half as tall, its
+// footprints smaller), until no scan at zl overflows.  The layout is then
+// chosen again for the heights kept.
 static int batch_configure(fdkhip_batch *b, int nw, int zl, int vstage_req)
 {
 	if (!back_config_supported(nw, zl))
 		return fail(FDKHIP_EUNSUPPORTED, "configuration waves=%d zper=%d not built", nw, zl);
-	int maxarea = 64;
-	for (auto *p : b->plans) {
+	const int zh = batch_half_zper(zl);
+	if (!b->zper_req.empty())
+		for (int i = 0; i < b->nscan; ++i)
+			if (b->zper_req[i] != zl && b->zper_req[i] != zh)
+				return fail(FDKHIP_EARG, "scan %d: zper %d is neither %d nor %d", i,
+					b->zper_req[i], zl, zh);
+	const int n = b->nscan;
+	std::vector<int> area_hi(n, 0), area_lo(n, 0), zper(n, zl);
+	for (int i = 0; i < n; ++i) {
+		fdkhip_plan *p = b->plans[i];
 		int rc = configure(p, nw, zl, 0, 0);
 		if (rc)
 			return rc;
-		maxarea = std::max(maxarea, p->maxarea);
+		area_hi[i] = p->maxarea;
+		area_lo[i] = p->maxarea;
+		if (zh != zl) {
+			rc = configure(p, nw, zh, 0, 0);
+			if (rc)
+				return rc;
+			area_lo[i] = p->maxarea;
+		}
+		if (!b->zper_req.empty())
+			zper[i] = b->zper_req[i];
 	}
 	const size_t lds_max = b->plans.empty() ? 65536 : b->plans[0]->lds_max;
 	const size_t lds_limit = (nw >= 8) ? lds_max : lds_max / 2;
 	int vstage = 0, slot = 0;
-	if (!choose_layout(nw, b->fmt, maxarea, vstage_req, 0, lds_limit, vstage, slot))
-		return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
+	for (;;) {
+		int maxarea = 64;
+		for (int i = 0; i < n; ++i)
+			maxarea = std::max(maxarea, zper[i] == zl ? area_hi[i] : area_lo[i]);
+		if (!choose_layout(nw, b->fmt, maxarea, vstage_req, 0, lds_limit, vstage, slot))
+			return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
+		bool moved = false;
+		for (int i = 0; b->zper_req.empty() && zh != zl && i < n; ++i)
+			if (zper[i] == zl && area_hi[i] > slot) {
+				zper[i] = zh;
+				moved = true;
+			}
+		if (!moved)
+			break;
+	}
 	b->nw = nw;
 	b->zl = zl;
+	b->vstage_req = vstage_req;
 	b->vstage = vstage;
 	b->slot = slot;
 	b->item_begin.assign(1, 0);
-	for (auto *p : b->plans) {
+	for (int i = 0; i < n; ++i) {
+		fdkhip_plan *p = b->plans[i];
+		int rc = configure(p, nw, zper[i], 0, 0);   // survey cached above
+		if (rc)
+			return rc;
 		p->vstage = vstage;
 		p->slot = slot;
 		const long long items = (long long) p->tiles_x * p->tiles_y
@@ -1550,7 +1602,34 @@ extern "C" int fdkhip_batch_set_config(fdkhip_batch *b, int32_t waves, int32_t z
 		return fail(FDKHIP_EARG, "batch is NULL");
 	if (vstage < 0 || vstage > 64)
 		return fail(FDKHIP_EARG, "views per stage out of range (%d)", vstage);
+	b->zper_req.clear();
 	return batch_configure(b, waves, zper, vstage);
+}
+
+extern "C" int fdkhip_batch_set_zper(fdkhip_batch *b, const int32_t *zper)
+{
+	if (!b)
+		return fail(FDKHIP_EARG, "batch is NULL");
+	const std::vector<int> keep = b->zper_req;
+	if (zper)
+		b->zper_req.assign(zper, zper + b->nscan);
+	else
+		b->zper_req.clear();
+	int rc = batch_configure(b, b->nw, b->zl, b->vstage_req);
+	if (rc) {
+		b->zper_req = keep;
+		(void) batch_configure(b, b->nw, b->zl, b->vstage_req);
+	}
+	return rc;
+}
+
+extern "C" int fdkhip_batch_scan_zper(const fdkhip_batch *b, int32_t *zper)
+{
+	if (!b || !zper)
+		return fail(FDKHIP_EARG, "NULL argument");
+	for (int i = 0; i < b->nscan; ++i)
+		zper[i] = b->plans[i]->zl;
+	return FDKHIP_OK;
 }
 
 extern "C" int fdkhip_batch_count_updates(fdkhip_batch *b, int64_t *n)
@@ -1702,6 +1781,7 @@ extern "C" int fdkhip_batch_autotune(fdkhip_batch *b, const void *const *proj,
 		return fail(FDKHIP_EARG, "reps must be >= 1");
 	hipStream_t s = (hipStream_t) stream;
 	int bn = b->nw, bz = b->zl, bv = b->vstage;
+	std::vector<int> breq = b->zper_req;
 	float best_t = 3.4e38f;
 	// zero fills and padded views once; then the persistent kernel alone over
 	// the first kTuneViews views of every scan
@@ -1710,31 +1790,46 @@ extern "C" int fdkhip_batch_autotune(fdkhip_batch *b, const void *const *proj,
 		rc = batch_launch(b, proj, scale, image, s);
 	if (rc)
 		return rc;
+	// per shape: the chosen per-scan heights, then (where that choice moved
+	// some scan to the half height) every scan at the shape's own height
 	for (const auto &sh : kShapes) {
-		int last_vstage = -1;
-		for (int vs : kStages) {
-			if (batch_configure(b, sh[0], sh[1], vs) != FDKHIP_OK)
-				continue;
-			if (b->vstage == last_vstage)
-				continue;
-			last_vstage = b->vstage;
-			rc = batch_upload_args(b, proj, scale, image, s, kTuneViews);
-			if (rc)
-				return rc;
-			float t = 0.f;
-			rc = time_launches([&](hipStream_t st) {
-				return batch_kernel_launch(b, st);
-			}, s, reps, &t);
-			if (rc)
-				return rc;
-			if (t < best_t) {
-				best_t = t;
-				bn = sh[0];
-				bz = sh[1];
-				bv = b->vstage;
+		bool moved = false;
+		for (int pass = 0; pass < 2; ++pass) {
+			if (pass == 1 && !moved)
+				break;
+			if (pass == 0)
+				b->zper_req.clear();
+			else
+				b->zper_req.assign(b->nscan, sh[1]);
+			int last_vstage = -1;
+			for (int vs : kStages) {
+				if (batch_configure(b, sh[0], sh[1], vs) != FDKHIP_OK)
+					continue;
+				for (auto *p : b->plans)
+					moved = moved || p->zl != sh[1];
+				if (b->vstage == last_vstage)
+					continue;
+				last_vstage = b->vstage;
+				rc = batch_upload_args(b, proj, scale, image, s, kTuneViews);
+				if (rc)
+					return rc;
+				float t = 0.f;
+				rc = time_launches([&](hipStream_t st) {
+					return batch_kernel_launch(b, st);
+				}, s, reps, &t);
+				if (rc)
+					return rc;
+				if (t < best_t) {
+					best_t = t;
+					bn = sh[0];
+					bz = sh[1];
+					bv = b->vstage;
+					breq = b->zper_req;
+				}
 			}
 		}
 	}
+	b->zper_req = breq;
 	rc = batch_configure(b, bn, bz, bv);
 	if (rc)
 		return rc;

@@ -32,6 +32,7 @@ struct BackArgs {
 	int tiles_x, tiles_y, ntiles;
 	int z0, ngroups;         // z-groups cover [z0, z0 + ngroups*NW*ZL)
 	int vstage, slot;        // views per LDS stage, floats per view slot
+	int zper;                // slices per consumer wave (ZL) of this scan's z-groups
 	unsigned long long *count;   // survey: reference loop iterations
 	int *maxarea;            // survey: largest staged footprint (floats)
 	int *fov_any;            // survey: some column meets some view

@@ -853,9 +853,11 @@ back_kernel(const BackArgs a)
 // Persistent form for a batch of scans of uneven sizes: every workgroup takes
 // (scan, tile, z-group) items from a device counter until the queue is empty.
 // Item ranges are contiguous per scan (item_begin, a prefix sum), so the
-// scan of an item is found by binary search.  All scans share the launch
-// shape and the LDS layout (vstage, slot), which the host sets to the
-// largest any scan needs.
+// scan of an item is found by binary search.  All scans share the waves per
+// workgroup and the LDS layout (vstage, slot); the slices per consumer wave
+// are per scan, ZL or ZL/2 (a.zper): a steep-cone scan whose footprint over
+// NW*ZL slices overflows the shared slot takes z-groups half as tall, whose
+// footprints fit, instead of gathering from HBM.
 template <int NW, int NP, int ZL, int FMT, bool FAST>
 __global__ void __launch_bounds__((NW + NP) * 64)
 batch_kernel(const BatchArgs b)
@@ -895,7 +897,11 @@ batch_kernel(const BatchArgs b)
 		}
 		const int local = (int) (item - b.item_begin[lo]);
 		const int tile = local / a.ngroups;
-		tile_pipeline<NW, NP, ZL, FMT, FAST>(a, tile, local - tile * a.ngroups);
+		constexpr int ZH = ZL >= 16 ? ZL / 2 : ZL;
+		if (ZH != ZL && a.zper == ZH)
+			tile_pipeline<NW, NP, ZH, FMT, FAST>(a, tile, local - tile * a.ngroups);
+		else
+			tile_pipeline<NW, NP, ZL, FMT, FAST>(a, tile, local - tile * a.ngroups);
 	}
 }
 

@@ -7,6 +7,8 @@ Bars:
 * FAST FP32 / FP16 batch: equal to single-scan FAST plans of the same format
   within relative L2 1e-6 (same arithmetic, different launch);
 * graph replays with new projection data match direct launches bit for bit;
+* per-scan z-group heights (the launch's slices per lane or half of it, in
+  one launch) leave EXACT results bit for bit unchanged;
 * the autotuner leaves results unchanged (EXACT: bit for bit).
 """
 from __future__ import annotations
@@ -137,6 +139,30 @@ def test_batch_launch_shapes(device, waves, zper, vstage):
         assert np.array_equal(outs[i], O.back_ts_f32(g, proj, beta, mask)), i
 
 
+@pytest.mark.parametrize("waves,zper,vstage", [(4, 16, 0), (4, 32, 0), (8, 16, 0), (4, 16, 8)])
+def test_batch_per_scan_heights(device, waves, zper, vstage):
+    """Scans at the launch's slices per lane and at half of it in one launch:
+    EXACT results bit for bit equal to the oracle, for a fixed mix and for
+    the automatic choice (a deep stage shrinks the slot, so more scans move
+    to the half height)."""
+    import torch
+    cases = _cases()
+    b = _batch(cases)
+    b.set_config(waves, zper, vstage)
+    auto = b.scan_zper
+    print(f"batch heights waves={waves} zper={zper} vstage={vstage} auto={auto}", flush=True)
+    assert set(auto) <= {zper, zper // 2}
+    projs = [torch.from_numpy(p).to(device) for _, p, _, _ in cases]
+    want = [O.back_ts_f32(g, proj, beta, mask) for g, proj, beta, mask in cases]
+    mix = [zper if i % 2 else zper // 2 for i in range(len(cases))]
+    for heights in (mix, [zper // 2] * len(cases), None):
+        b.set_zper(heights)
+        assert b.scan_zper == (auto if heights is None else heights)
+        outs = [o.cpu().numpy() for o in b(projs)]
+        for i in range(len(cases)):
+            assert np.array_equal(outs[i], want[i]), (heights, i)
+
+
 def test_plan_autotune_keeps_results(device):
     import torch
     g, proj, beta, mask = make_case("arc_tall", seed=11)
@@ -173,6 +199,13 @@ def test_batch_argument_errors(device):
         b(projs[:1])
     with pytest.raises(FdkhipError):
         b.set_config(8, 32, 0)    # shape not built
+    b.set_config(4, 16, 0)
+    keep = b.scan_zper
+    with pytest.raises(FdkhipError):
+        b.set_zper([12, 16])      # neither the launch's height nor half of it
+    with pytest.raises(ValueError):
+        b.set_zper([16])
+    assert b.scan_zper == keep and b.config["zper"] == 16
     b16 = _batch(cases, fmt="f16", math_mode="fast")
     with pytest.raises(ValueError):
         b16([p.half() for p in projs])   # no scales
