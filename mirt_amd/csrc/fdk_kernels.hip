@@ -62,7 +62,7 @@ __device__ __forceinline__ uint32_t pack_f16x2(float lo, float hi)
 }
 
 // wl*lo + wr*hi of an FP16 column pair and FP16 horizontal weights packed as
-// (wl, wr), accumulated in FP32 by one v_dot2c_f32_f16
+// (wl, wr), accumulated in FP32 by one v_dot2_f32_f16
 __device__ __forceinline__ float dot_pair_f16(uint32_t pair, uint32_t wpk)
 {
 	return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, pair),
@@ -562,21 +562,24 @@ __device__ __forceinline__ float pair_hi(uint32_t w)
 		return __uint_as_float(w & 0xffff0000u);
 }
 
-// FP16 slice update: the horizontal weights (wl, wr) are FP16 in wpk, the
-// fan-beam weight times the view scale is w2s.  The second row's dot product
-// accumulates onto -q1, giving q2 - q1 for the vertical interpolation
-// q1 + wu*(q2 - q1) in one FMA.
+// FP16 slice update: the horizontal weights (wl, wr) are FP16 in wpk and,
+// sign-flipped, in wneg; the fan-beam weight times the view scale is w2s.
+// The first row's dot product against wneg is -q1 exactly; the second row's
+// accumulates onto it, giving q2 - q1 for the vertical interpolation
+// q1 + wu*(q2 - q1) in one FMA.  No operand modifier is asked of the dot
+// products' FP32 accumulator: in the VOP3P encoding its neg_hi bit means
+// abs, not negation, so a negated accumulator there would add -|q1|.
 __device__ __forceinline__ float slice_update_f16(float acc, float u, int c1, uint32_t wpk,
-	float w2s)
+	uint32_t wneg, float w2s)
 {
 	const int iu = (int) u;
 	const float wu = __builtin_amdgcn_fractf(u);
 	const uint32_t w0 = __float_as_uint(fdk_lds[c1 + iu]);
 	const uint32_t w1 = __float_as_uint(fdk_lds[c1 + iu + 1]);
-	const float q1 = dot_pair_f16(w0, wpk);
+	const float nq1 = dot_pair_f16(w0, wneg);
 	const float d = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, w1),
-		__builtin_bit_cast(half2v, wpk), -q1, false);
-	return fmaf(w2s, fmaf(wu, d, q1), acc);
+		__builtin_bit_cast(half2v, wpk), nq1, false);
+	return fmaf(w2s, fmaf(wu, d, -nq1), acc);
 }
 
 // The same update for the BF16 path in u = t_bin + 1 (> 0): the padded row
@@ -640,6 +643,8 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 			continue;                  // no lane of this wave meets the view
 		const int cb = __float_as_int(p0.x);
 		const float wl = p0.y, wr = p0.z, w2 = p0.w;
+		// FP16: the packed horizontal weights with both signs flipped
+		const uint32_t wneg = __float_as_uint(wl) ^ 0x80008000u;
 		const float ti = p1.x;
 		if (mode == MODE_LDS) {
 			const int tw = __builtin_amdgcn_readfirstlane(vi.x);
@@ -652,7 +657,7 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 					for (int j = 0; j < ZL; ++j) {
 						const float u = fmaf((float) j, ti, t0);
 						if constexpr (FMT == FMT_F16)
-							acc[j] = slice_update_f16(acc[j], u, c1, __float_as_uint(wl), wr);
+							acc[j] = slice_update_f16(acc[j], u, c1, __float_as_uint(wl), wneg, wr);
 						else
 							acc[j] = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
 					}
@@ -683,7 +688,7 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 							const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), uf, ul);
 							float up;
 							if constexpr (FMT == FMT_F16)
-								up = slice_update_f16(acc[j], u, c1, __float_as_uint(wl), wr);
+								up = slice_update_f16(acc[j], u, c1, __float_as_uint(wl), wneg, wr);
 							else
 								up = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
 							acc[j] = r ? up : acc[j];
