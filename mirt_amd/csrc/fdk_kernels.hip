@@ -564,11 +564,16 @@ __device__ __forceinline__ float pair_hi(uint32_t w)
 
 // FP16 slice update: the horizontal weights (wl, wr) are FP16 in wpk and,
 // sign-flipped, in wneg; the fan-beam weight times the view scale is w2s.
-// The first row's dot product against wneg is -q1 exactly; the second row's
-// accumulates onto it, giving q2 - q1 for the vertical interpolation
-// q1 + wu*(q2 - q1) in one FMA.  No operand modifier is asked of the dot
-// products' FP32 accumulator: in the VOP3P encoding its neg_hi bit means
-// abs, not negation, so a negated accumulator there would add -|q1|.
+// CHAIN: the first row's dot product against wneg is -q1 exactly; the second
+// row's accumulates onto it, giving q2 - q1 for the vertical interpolation
+// q1 + wu*(q2 - q1) in one FMA (4 VALU after the address).  No operand
+// modifier is asked of the dot products' FP32 accumulator: in the VOP3P
+// encoding its neg_hi bit means abs, not negation, so a negated accumulator
+// there would add -|q1|.  Otherwise the two rows' dot products are
+// independent and q2 - q1 is one more subtraction (5 VALU): consumers of 16
+// or fewer slices have too few slices per view to hide the chained form's
+// latency and run faster this way; 32-slice consumers run faster chained.
+template <bool CHAIN>
 __device__ __forceinline__ float slice_update_f16(float acc, float u, int c1, uint32_t wpk,
 	uint32_t wneg, float w2s)
 {
@@ -576,10 +581,16 @@ __device__ __forceinline__ float slice_update_f16(float acc, float u, int c1, ui
 	const float wu = __builtin_amdgcn_fractf(u);
 	const uint32_t w0 = __float_as_uint(fdk_lds[c1 + iu]);
 	const uint32_t w1 = __float_as_uint(fdk_lds[c1 + iu + 1]);
-	const float nq1 = dot_pair_f16(w0, wneg);
-	const float d = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, w1),
-		__builtin_bit_cast(half2v, wpk), nq1, false);
-	return fmaf(w2s, fmaf(wu, d, -nq1), acc);
+	if constexpr (CHAIN) {
+		const float nq1 = dot_pair_f16(w0, wneg);
+		const float d = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, w1),
+			__builtin_bit_cast(half2v, wpk), nq1, false);
+		return fmaf(w2s, fmaf(wu, d, -nq1), acc);
+	} else {
+		const float q1 = dot_pair_f16(w0, wpk);
+		const float q2 = dot_pair_f16(w1, wpk);
+		return fmaf(w2s, fmaf(wu, q2 - q1, q1), acc);
+	}
 }
 
 // The same update for the BF16 path in u = t_bin + 1 (> 0): the padded row
@@ -657,7 +668,7 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 					for (int j = 0; j < ZL; ++j) {
 						const float u = fmaf((float) j, ti, t0);
 						if constexpr (FMT == FMT_F16)
-							acc[j] = slice_update_f16(acc[j], u, c1, __float_as_uint(wl), wneg, wr);
+							acc[j] = slice_update_f16<(ZL >= 32)>(acc[j], u, c1, __float_as_uint(wl), wneg, wr);
 						else
 							acc[j] = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
 					}
@@ -688,7 +699,7 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 							const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), uf, ul);
 							float up;
 							if constexpr (FMT == FMT_F16)
-								up = slice_update_f16(acc[j], u, c1, __float_as_uint(wl), wneg, wr);
+								up = slice_update_f16<(ZL >= 32)>(acc[j], u, c1, __float_as_uint(wl), wneg, wr);
 							else
 								up = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
 							acc[j] = r ? up : acc[j];

@@ -183,3 +183,21 @@ def test_all_configs(device, waves, zper):
     want = reference("fixed", g, proj, beta, mask)
     img = plan(torch.from_numpy(proj).to(device)).cpu().numpy()
     assert np.array_equal(img, want)
+
+
+@pytest.mark.parametrize("waves,zper", [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32)])
+def test_all_configs_f16(device, waves, zper):
+    """FP16 storage at every built launch shape (consumers of 16 or fewer
+    slices take the independent-dot slice update, 32-slice ones the chained
+    one) within 1e-3 of the FP64 restatement, with the reference's own error
+    on the same views as the scale of comparison."""
+    import torch
+    from mirt_amd import quantize_views
+    g, proj, beta, mask = make_case("arc_tall", seed=9, data="smooth")
+    plan = _plan(g, beta, mask, "f16")
+    plan.set_config(waves, zper, 0, 0)
+    q, sc = quantize_views(torch.from_numpy(proj).to(device), "f16")
+    img = plan(q, sc).cpu().numpy()
+    err = O.rel_l2(img, O.back_ts_f64(g, proj, beta, mask))
+    print(f"f16 waves={waves} zper={zper} relL2 vs FP64 = {err:.3e}", flush=True)
+    assert err <= 1e-3, err
