@@ -693,6 +693,23 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 						// groups of 8 slices that no lane of the wave adds are skipped
 						if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
 							continue;
+						// 16-slice consumers: a group every lane adds whole runs
+						// straight-line, with the same u as the clamped slices
+						// below (8-slice consumers meet that case as a whole
+						// chunk above; in 32-slice consumers the extra code
+						// costs more registers than it saves)
+						if (ZL == 16 && __all(jlo <= g0 && jhi >= g0 + 8)) {
+#pragma unroll
+							for (int j = g0; j < g0 + 8; ++j) {
+								const float u = fmaf((float) j, ti, t0);
+								if constexpr (FMT == FMT_F16)
+									acc[j] = slice_update_f16<(ZL >= 32)>(acc[j], u, c1,
+										__float_as_uint(wl), wneg, wr);
+								else
+									acc[j] = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
+							}
+							continue;
+						}
 #pragma unroll
 						for (int j = g0; j < g0 + 8; ++j) {
 							const bool r = (unsigned) (j - jlo) < nj;
