@@ -163,6 +163,28 @@ def test_batch_per_scan_heights(device, waves, zper, vstage):
             assert np.array_equal(outs[i], want[i]), (heights, i)
 
 
+def test_batch_f16_heights_vs_fp64(device):
+    """FP16 batch with scans at 16 and at 8 slices per lane in one launch
+    (the independent-row slice update and the straight-line groups of
+    16-slice consumers): every scan within 1e-3 of the FP64 restatement."""
+    import torch
+    from mirt_amd import quantize_views
+    cases = _cases(data="smooth")
+    b = _batch(cases, fmt="f16", math_mode="fast")
+    b.set_config(4, 16, 0)
+    b.set_zper([16, 8, 16, 8, 16])
+    projs, scales = [], []
+    for _, p, _, _ in cases:
+        q, s = quantize_views(torch.from_numpy(p).to(device), "f16")
+        projs.append(q)
+        scales.append(s)
+    outs = b(projs, scales)
+    for i, (g, proj, beta, mask) in enumerate(cases):
+        err = O.rel_l2(outs[i].cpu().numpy(), O.back_ts_f64(g, proj, beta, mask))
+        print(f"batch f16 scan {i} zper={b.scan_zper[i]} relL2 vs FP64 = {err:.2e}", flush=True)
+        assert err <= 1e-3, (i, err)
+
+
 def test_plan_autotune_keeps_results(device):
     import torch
     g, proj, beta, mask = make_case("arc_tall", seed=11)
