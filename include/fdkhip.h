@@ -342,6 +342,25 @@ int fdkhip_batch_destroy(fdkhip_batch *batch);
 int fdkhip_quantize_views(const float *proj_dev, int32_t ns, int32_t nt,
 	int32_t na, int32_t fmt, void *out_dev, float *scale_dev, void *stream);
 
+/*
+ * The FDK steps before the backprojection, on the device: the cosine
+ * pre-weight of feldkamp.m:103-132 (feldkamp_weight1) and the zero-padded
+ * ramp filter along s of fdk_filter.m:1-63, in FP32.  proj is MATLAB's
+ * [ns nt na] (s fastest); ww1 is the [ns nt] weight table; toeplitz is the
+ * filter as an ns x ns matrix, toeplitz[s + ns*s'] = h(s' - s), h the
+ * filter's spatial taps (the inverse DFT of fdk_filter's response, lags
+ * -(ns-1)..ns-1: the zero padding to 2^ceil(log2(2ns-1)) makes the padded
+ * circular convolution exactly this product).  work ([ns nt na], may be
+ * proj itself to weight in place) receives proj .* ww1; tsa receives the
+ * filtered views as [nt ns na] (t fastest), the layout fdkhip_plan_execute
+ * takes.  One rocBLAS strided-batched SGEMM (a view per batch entry) does
+ * the filtering.  All pointers are device pointers; tsa must not alias proj
+ * or work.  Asynchronous on `stream`.
+ */
+int fdkhip_fdk_filter_views(const float *proj_dev, int32_t ns, int32_t nt,
+	int32_t na, const float *ww1_dev, const float *toeplitz_dev, float *work_dev,
+	float *tsa_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

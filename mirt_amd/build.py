@@ -64,7 +64,10 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(cc)))
+    # rocBLAS: the strided-batched SGEMM of fdkhip_fdk_filter_views
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + [
+        f"-L{rocm}/lib", f"-Wl,-rpath,{rocm}/lib", "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

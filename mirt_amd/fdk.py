@@ -6,11 +6,17 @@
 * :func:`fdk_filter` -- fbp/fdk_filter.m:1-63 (zero-padded FFT filtering
   along s, ``ifft(...,'symmetric')`` taken as the real inverse transform of
   the non-negative frequencies);
+* :func:`weight_filter_views` -- the weight and the filter on the device
+  through ``fdkhip_fdk_filter_views`` (a HIP weight kernel and one rocBLAS
+  strided-batched SGEMM with the filter's Toeplitz matrix), FP32, producing
+  the ``[nt ns na]`` views the plans take;
 * :func:`feldkamp` -- fbp/feldkamp.m:1-98: weight, filter, backproject.
 
-The weight and the filter are written with torch so they run where the
-projections live (the GPU for full-size scans); the backprojection is the
-HIP kernel in libfdkhip.so.
+:func:`feldkamp_weight1` and :func:`fdk_filter` are written with torch in
+any precision, on any device (FP64 on the host for the host route and for
+synthesising benchmark views); :func:`feldkamp`'s device route uses
+:func:`weight_filter_views`.  The backprojection is the HIP kernel in
+libfdkhip.so.
 """
 from __future__ import annotations
 
@@ -29,7 +35,12 @@ def _t(x, like):
 
 def feldkamp_weight1(proj, cg: CTGeom, *, w1cyl: bool = True):
     """feldkamp.m:103-132: multiply every view ``[ns nt]`` by ww1."""
-    ns, nt = proj.shape[0], proj.shape[1]
+    ww1 = weight_table(cg, proj.shape[0], proj.shape[1], w1cyl=w1cyl)
+    return proj * _t(ww1, proj)[:, :, None]
+
+
+def weight_table(cg: CTGeom, ns: int, nt: int, *, w1cyl: bool = True) -> np.ndarray:
+    """ww1 of feldkamp.m:103-132 as an FP64 ``[ns nt]`` array."""
     ds, dt, dsd, dso = cg.ds, cg.dt, cg.dsd, cg.dso
     ss = (np.arange(ns) - (ns - 1) / 2 - cg.offset_s) * ds
     tt = (np.arange(nt) - (nt - 1) / 2 - cg.offset_t) * dt
@@ -46,7 +57,7 @@ def feldkamp_weight1(proj, cg: CTGeom, *, w1cyl: bool = True):
             ww1 = (dso / dsd) * np.cos(ss / (dsd * np.sqrt(1 + (tt / dsd) ** 2)))
     else:
         raise ValueError("other configurations not implemented")
-    return proj * _t(ww1, proj)[:, :, None]
+    return ww1
 
 
 def fbp_ramp(type: str, n: int, ds: float, dsd: float = None) -> np.ndarray:
@@ -101,6 +112,67 @@ def fdk_filter(proj, cg: CTGeom, window: str = "ramp"):
     return out[:ns]
 
 
+def filter_taps(cg: CTGeom, ns: int, window: str = "ramp") -> np.ndarray:
+    """Spatial taps of fdk_filter's zero-padded filter, FP64, for the lags
+    -(ns-1)..ns-1: the inverse DFT of the response (fdk_filter.m:24-33), tap
+    for lag k at index k mod npad.  With npad = 2^ceil(log2(2ns-1)) the padded
+    circular convolution of a row of ns samples is the product with the
+    Toeplitz matrix of these taps."""
+    npadh = 2 ** math.ceil(math.log2(2 * ns - 1))
+    h = np.real(np.fft.ifft(fdk_filter_response(cg, npadh, window)))
+    return h[np.arange(-(ns - 1), ns) % npadh]
+
+
+def toeplitz_matrix(cg: CTGeom, ns: int, window: str = "ramp") -> np.ndarray:
+    """The filter as the FP64 C-order ``(ns, ns)`` array ``T[s', s] = tap(s' - s)``
+    (in memory: ``T[s + ns*s']``, the layout fdkhip_fdk_filter_views takes)."""
+    taps = filter_taps(cg, ns, window)
+    k = np.arange(ns)
+    return taps[(k[:, None] - k[None, :]) + ns - 1]
+
+
+_FILTER_TABLES = {}
+
+
+def _filter_tables(cg: CTGeom, ns, nt, window, w1cyl, device):
+    import torch
+    key = (ns, nt, cg.ds, cg.dt, cg.dsd, cg.dso, cg.dfs, cg.offset_s, cg.offset_t, window,
+           w1cyl, str(device))
+    if key not in _FILTER_TABLES:
+        ww1 = torch.as_tensor(weight_table(cg, ns, nt, w1cyl=w1cyl).T.copy(),  # [ns nt], s fastest
+                              dtype=torch.float32, device=device)
+        T = torch.as_tensor(toeplitz_matrix(cg, ns, window), dtype=torch.float32, device=device)
+        _FILTER_TABLES.clear()
+        _FILTER_TABLES[key] = (ww1, T)
+    return _FILTER_TABLES[key]
+
+
+def weight_filter_views(proj, cg: CTGeom, *, window: str = "ramp", w1cyl: bool = True,
+                        stream=None):
+    """feldkamp_weight1 + fdk_filter on the device (fdkhip_fdk_filter_views):
+    ``proj`` is a float32 device tensor ``[ns nt na]`` (MATLAB order, s
+    fastest: torch shape ``(na, nt, ns)``); returns the filtered views as a
+    float32 ``(na, ns, nt)`` tensor, i.e. ``[nt ns na]`` with t fastest, as
+    the plans take them.  FP32 throughout (the weight and the taps are
+    computed in FP64 on the host, then rounded)."""
+    import ctypes as C
+    import torch
+    from . import _lib
+    if not (isinstance(proj, torch.Tensor) and proj.is_cuda and proj.dtype == torch.float32
+            and proj.dim() == 3 and proj.is_contiguous()):
+        raise ValueError("proj must be a contiguous float32 device tensor (na, nt, ns)")
+    na, nt, ns = proj.shape
+    lib = _lib.load()
+    ww1, T = _filter_tables(cg, ns, nt, window, w1cyl, proj.device)
+    work = torch.empty_like(proj)
+    tsa = torch.empty((na, ns, nt), dtype=torch.float32, device=proj.device)
+    s = stream if stream is not None else torch.cuda.current_stream(proj.device)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    _lib.check(lib.fdkhip_fdk_filter_views(P(proj), ns, nt, na, P(ww1), P(T), P(work),
+                                           P(tsa), C.c_void_p(s.cuda_stream)))
+    return tsa
+
+
 def feldkamp(cg: CTGeom, ig: ImageGeom, proj, *, window="ramp", w1cyl=True,
              ia_skip=1, extrapolate_t=0, fmt="f32", math_mode="exact"):
     """feldkamp.m:1-98.  ``proj`` ``[ns nt na]``: a numpy array (host path,
@@ -113,10 +185,11 @@ def feldkamp(cg: CTGeom, ig: ImageGeom, proj, *, window="ramp", w1cyl=True,
         p = fdk_filter(feldkamp_weight1(p, cg, w1cyl=w1cyl), cg, window)
         return cbct_back(p.numpy().astype(np.float32), cg, ig, ia_skip=ia_skip,
                          extrapolate_t=extrapolate_t)
-    p = fdk_filter(feldkamp_weight1(proj, cg, w1cyl=w1cyl), cg, window)
     if extrapolate_t:
         raise NotImplementedError("extrapolate_t on the device path")
-    tsa = p[:, :, ::ia_skip].permute(2, 0, 1).contiguous().to(torch.float32)  # [nt ns na]
+    # [ns nt na] device views (torch shape (ns, nt, na)) -> MATLAB memory order
+    sta = proj[:, :, ::ia_skip].permute(2, 1, 0).contiguous().to(torch.float32)
+    tsa = weight_filter_views(sta, cg, window=window, w1cyl=w1cyl)        # [nt ns na]
     plan = FdkPlan.from_geoms(cg, ig, ia_skip=ia_skip, fmt=fmt, math_mode=math_mode)
     if fmt == "f32":
         img = plan(tsa)

@@ -391,6 +391,23 @@ def test_fdk_filter_matches_direct_convolution():
         assert np.allclose(got, want, atol=1e-10)
 
 
+@pytest.mark.parametrize("ns", [37, 64])
+@pytest.mark.parametrize("window", ["ramp", "hann"])
+def test_toeplitz_filter_matches_fdk_filter(ns, window):
+    """The Toeplitz matrix handed to fdkhip_fdk_filter_views (taps = inverse
+    DFT of fdk_filter's response) filters rows exactly as fdk_filter's
+    zero-padded FFT does, for arc and flat detectors."""
+    from mirt_amd.fdk import fdk_filter, toeplitz_matrix
+    from mirt_amd.geom import ct_geom
+    for dfs in (0.0, np.inf):
+        cg = ct_geom("fan", ns=ns, nt=3, na=2, ds=1.3, dt=1.0, dsd=800.0, dso=500.0, dfs=dfs)
+        p = np.random.default_rng(ns).standard_normal((ns, 3, 2))
+        want = fdk_filter(torch.as_tensor(p), cg, window).numpy()
+        T = toeplitz_matrix(cg, ns, window)                 # T[s', s]
+        got = np.einsum("ab,btv->atv", T, p)
+        assert np.allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
 def test_feldkamp_weight1_center_value():
     from mirt_amd.fdk import feldkamp_weight1
     from mirt_amd.geom import ct_geom
