@@ -199,6 +199,31 @@ def column_errors(g, tsa_host, beta, mask, images, ncol=1024, seed=7):
     return errs
 
 
+def filter_times(args, cg, dev):
+    """The FDK weight + ramp filter before the backprojection, on synthetic
+    FP32 views of the workload's size: fdkhip_fdk_filter_views (weight kernel
+    + strided-batched SGEMM with the Toeplitz matrix) against torch's FP32
+    FFT route (feldkamp_weight1 + fdk_filter + the permute to [nt ns na])."""
+    import torch
+    from mirt_amd.fdk import fdk_filter, feldkamp_weight1, weight_filter_views
+    ns, nt, na = cg.ns, cg.nt, cg.na
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    sta = torch.randn((na, nt, ns), generator=g, device=dev, dtype=torch.float32)
+    native = time_steps(lambda: weight_filter_views(sta, cg), args.steps, args.warmup)
+    p = sta.permute(2, 1, 0)                                   # (ns, nt, na) view
+    fft = time_steps(lambda: fdk_filter(feldkamp_weight1(p, cg), cg).permute(2, 0, 1)
+                     .contiguous(), args.steps, args.warmup)
+    flop = 2.0 * ns * ns * nt * na
+    tflops = flop / (native * 1e-3) / 1e12
+    log(f"fdk filter: {native:.3f} ms native (SGEMM {tflops:.1f} TFLOP/s), "
+        f"{fft:.3f} ms torch FP32 FFT")
+    return {"ms": native, "path": "fdkhip_fdk_filter_views", "torch_fft_f32_ms": fft,
+            "gemm_tflops": tflops, "fp32_matrix_peak_tflops": FP32_PEAK_TFLOPS,
+            "note": f"{ns}x{nt}x{na} FP32 views, weight + Toeplitz SGEMM "
+                    f"(2*ns^2*nt*na = {flop:.3g} flop); includes output allocation"}
+
+
 def run_single(args, dev):
     import torch
     from mirt_amd import FdkPlan, geom_of
@@ -289,6 +314,8 @@ def run_single(args, dev):
                              "valu_issue = SQ_INSTS_VALU / (CUs x GRBM_GUI_ACTIVE cycles) "
                              "from the same summary"},
     }
+    if not args.fmt_only:
+        result["fdk_filter"] = filter_times(args, cg, dev)
     if args.slabs and args.slabs > 1:
         result["slabs"] = slab_times(args, geom, beta, mask, ig, tsa, head, cfg, "z")
         result["strips"] = slab_times(args, geom, beta, mask, ig, tsa, head, cfg, "y")
