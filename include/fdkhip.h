@@ -350,12 +350,13 @@ int fdkhip_quantize_views(const float *proj_dev, int32_t ns, int32_t nt,
  * filter as an ns x ns matrix, toeplitz[s + ns*s'] = h(s' - s), h the
  * filter's spatial taps (the inverse DFT of fdk_filter's response, lags
  * -(ns-1)..ns-1: the zero padding to 2^ceil(log2(2ns-1)) makes the padded
- * circular convolution exactly this product).  work ([ns nt na], may be
- * proj itself to weight in place) receives proj .* ww1; tsa receives the
+ * circular convolution exactly this product).  work is scratch of ns*nt*na
+ * floats (may be proj itself, which is then overwritten); tsa receives the
  * filtered views as [nt ns na] (t fastest), the layout fdkhip_plan_execute
- * takes.  One rocBLAS strided-batched SGEMM (a view per batch entry) does
- * the filtering.  All pointers are device pointers; tsa must not alias proj
- * or work.  Asynchronous on `stream`.
+ * takes.  The weighted views go through one rocBLAS SGEMM (every row of
+ * every view against the Toeplitz matrix) and a tiled transpose.  All
+ * pointers are device pointers; tsa must not alias proj or work.
+ * Asynchronous on `stream`.
  */
 int fdkhip_fdk_filter_views(const float *proj_dev, int32_t ns, int32_t nt,
 	int32_t na, const float *ww1_dev, const float *toeplitz_dev, float *work_dev,

@@ -617,30 +617,36 @@ extern "C" int fdkhip_fdk_filter_views(const float *proj, int32_t ns, int32_t nt
 		return FDKHIP_OK;
 	if (!proj || !ww1 || !toeplitz || !work || !tsa)
 		return fail(FDKHIP_EARG, "NULL device pointer");
-	if ((long) ns * nt > 0x7fffffffL / 64)
-		return fail(FDKHIP_EARG, "view of %d x %d samples too large", ns, nt);
+	if ((long) ns * nt > 0x7fffffffL / 64 || (long) nt * na > 0x7fffffffL)
+		return fail(FDKHIP_EARG, "views of %d x %d x %d samples too large", ns, nt, na);
 	if (work == tsa || proj == tsa)
 		return fail(FDKHIP_EARG, "tsa must not alias proj or work");
 	hipStream_t s = (hipStream_t) stream;
 	const int nst = ns * nt;
+	// 1. tsa (as scratch, [ns nt na]) = proj .* ww1
 	dim3 grid((nst + 255) / 256, std::min(na, 1024));
-	hipLaunchKernelGGL(weight_views_kernel, grid, dim3(256), 0, s, proj, ww1, work, nst, na);
+	hipLaunchKernelGGL(weight_views_kernel, grid, dim3(256), 0, s, proj, ww1, tsa, nst, na);
 	HIP_TRY(hipGetLastError());
 	rocblas_handle h = nullptr;
 	int rc = blas_handle(&h);
 	if (rc)
 		return rc;
-	std::lock_guard<std::mutex> lock(g_blas_mutex);
-	if (rocblas_set_stream(h, s) != rocblas_status_success)
-		return fail(FDKHIP_EHIP, "rocblas_set_stream failed");
-	// per view: tsa_v (nt x ns, t fastest) = work_v^T (nt x ns) * toeplitz (ns x ns),
-	// toeplitz[s + ns*s'] = tap(s' - s)
-	const float one = 1.f, zero = 0.f;
-	const rocblas_status st = rocblas_sgemm_strided_batched(h, rocblas_operation_transpose,
-		rocblas_operation_none, nt, ns, ns, &one, work, ns, (rocblas_stride) nst,
-		toeplitz, ns, 0, &zero, tsa, nt, (rocblas_stride) nst, na);
-	if (st != rocblas_status_success)
-		return fail(FDKHIP_EHIP, "rocblas_sgemm_strided_batched: status %d", (int) st);
+	{
+		std::lock_guard<std::mutex> lock(g_blas_mutex);
+		if (rocblas_set_stream(h, s) != rocblas_status_success)
+			return fail(FDKHIP_EHIP, "rocblas_set_stream failed");
+		// 2. every row at once: work (ns x nt*na, s fastest) = Toeplitzᵀ (ns x ns) * tsa,
+		// where the toeplitz buffer read column-major is M[s, s'] = h(s' - s)
+		const float one = 1.f, zero = 0.f;
+		const rocblas_status st = rocblas_sgemm(h, rocblas_operation_transpose,
+			rocblas_operation_none, ns, nt * na, ns, &one, toeplitz, ns, tsa, ns, &zero,
+			work, ns);
+		if (st != rocblas_status_success)
+			return fail(FDKHIP_EHIP, "rocblas_sgemm: status %d", (int) st);
+	}
+	// 3. [ns nt na] -> [nt ns na]
+	HIP_TRY(launch_transpose(work, tsa, na, nt, ns, (long) nst, ns, (long) nst, nt, nt, 0, 1.f,
+		s));
 	return FDKHIP_OK;
 }
 
