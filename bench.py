@@ -202,7 +202,7 @@ def column_errors(g, tsa_host, beta, mask, images, ncol=1024, seed=7):
 def filter_times(args, cg, dev):
     """The FDK weight + ramp filter before the backprojection, on synthetic
     FP32 views of the workload's size: fdkhip_fdk_filter_views (weight kernel
-    + strided-batched SGEMM with the Toeplitz matrix) against torch's FP32
+    + SGEMM with the Toeplitz matrix + transpose) against torch's FP32
     FFT route (feldkamp_weight1 + fdk_filter + the permute to [nt ns na])."""
     import torch
     from mirt_amd.fdk import fdk_filter, feldkamp_weight1, weight_filter_views
@@ -220,7 +220,7 @@ def filter_times(args, cg, dev):
         f"{fft:.3f} ms torch FP32 FFT")
     return {"ms": native, "path": "fdkhip_fdk_filter_views", "torch_fft_f32_ms": fft,
             "gemm_tflops": tflops, "fp32_matrix_peak_tflops": FP32_PEAK_TFLOPS,
-            "note": f"{ns}x{nt}x{na} FP32 views, weight + Toeplitz SGEMM "
+            "note": f"{ns}x{nt}x{na} FP32 views, weight + Toeplitz SGEMM + transpose "
                     f"(2*ns^2*nt*na = {flop:.3g} flop); includes output allocation"}
 
 

@@ -65,7 +65,7 @@ def build(verbose: bool = False) -> str:
         objs.append(obj)
     tmp = LIB + ".tmp"
     rocm = os.path.dirname(os.path.dirname(os.path.realpath(cc)))
-    # rocBLAS: the strided-batched SGEMM of fdkhip_fdk_filter_views
+    # rocBLAS: the SGEMM of fdkhip_fdk_filter_views
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + [
         f"-L{rocm}/lib", f"-Wl,-rpath,{rocm}/lib", "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -7,9 +7,9 @@
   along s, ``ifft(...,'symmetric')`` taken as the real inverse transform of
   the non-negative frequencies);
 * :func:`weight_filter_views` -- the weight and the filter on the device
-  through ``fdkhip_fdk_filter_views`` (a HIP weight kernel and one rocBLAS
-  strided-batched SGEMM with the filter's Toeplitz matrix), FP32, producing
-  the ``[nt ns na]`` views the plans take;
+  through ``fdkhip_fdk_filter_views`` (a HIP weight kernel, one rocBLAS SGEMM
+  with the filter's Toeplitz matrix over every row, a tiled transpose), FP32,
+  producing the ``[nt ns na]`` views the plans take;
 * :func:`feldkamp` -- fbp/feldkamp.m:1-98: weight, filter, backproject.
 
 :func:`feldkamp_weight1` and :func:`fdk_filter` are written with torch in

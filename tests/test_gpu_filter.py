@@ -1,6 +1,6 @@
 """GPU: the FDK weight and ramp filter on the device (fdkhip_fdk_filter_views:
-a weight kernel and one rocBLAS strided-batched SGEMM with the filter's
-Toeplitz matrix) and feldkamp's device route.
+a weight kernel, one rocBLAS SGEMM with the filter's Toeplitz matrix over
+every row, a tiled transpose) and feldkamp's device route.
 
 Bars (FP32 arithmetic against the FP64 torch restatement of
 feldkamp_weight1 + fdk_filter, which tests/test_host.py pins to direct
