@@ -356,7 +356,9 @@ int fdkhip_quantize_views(const float *proj_dev, int32_t ns, int32_t nt,
  * takes.  The weighted views go through one rocBLAS SGEMM (every row of
  * every view against the Toeplitz matrix) and a tiled transpose.  All
  * pointers are device pointers; tsa must not alias proj or work.
- * Asynchronous on `stream`.
+ * Asynchronous on `stream`.  The library keeps one rocBLAS handle per device
+ * for the life of the process; calls from several threads are serialised
+ * around it.
  */
 int fdkhip_fdk_filter_views(const float *proj_dev, int32_t ns, int32_t nt,
 	int32_t na, const float *ww1_dev, const float *toeplitz_dev, float *work_dev,
