@@ -364,6 +364,24 @@ int fdkhip_fdk_filter_views(const float *proj_dev, int32_t ns, int32_t nt,
 	int32_t na, const float *ww1_dev, const float *toeplitz_dev, float *work_dev,
 	float *tsa_dev, void *stream);
 
+/*
+ * ellipsoid_proj (fbp/ellipsoid_proj.m:35-125): analytic line integrals of
+ * ellipsoids for a cone-beam (or parallel-beam) scan, in FP64, written to the
+ * device buffer proj_dev [ns nt na] (s fastest, MATLAB order).  s [ns] and
+ * t [nt] are the detector sample positions (ct_geom's cg.s, cg.t), beta [na]
+ * the source angles in radians (cg.ar), source_zs [na] the source heights
+ * (NULL: 0); dso = inf is parallel beam, else dfs = inf a flat and dfs = 0 an
+ * arc detector (ir_coord_cb_{flat,arc}_to_par.m; other dfs: "not done").
+ * params [nell][9] are MIRT's rows [cx cy cz rx ry rz xy_angle_deg
+ * z_angle_deg value] (z rotation "not done"), at most 512.  oversample > 1
+ * averages No x No sub-samples per sample as ellipsoid_proj.m:44-58 (uniform
+ * spacing required).  Synchronous on `stream` when it returns.
+ */
+int fdkhip_ellipsoid_proj(const double *s, int32_t ns, const double *t, int32_t nt,
+	const double *beta, const double *source_zs, int32_t na,
+	double dso, double dod, double dfs, const double *params, int32_t nell,
+	int32_t oversample, double *proj_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,6 +101,9 @@ _SIGS = {
     "fdkhip_batch_profile": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32, _P]),
     "fdkhip_batch_count_updates": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "fdkhip_batch_destroy": (C.c_int, [_P]),
+    "fdkhip_ellipsoid_proj": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, C.c_int32,
+                                        C.c_double, C.c_double, C.c_double, _P, C.c_int32,
+                                        C.c_int32, _P, _P]),
 }
 
 _lib = None

@@ -21,7 +21,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libfdkhip.so")
 
-SOURCES = ["fdk_kernels.hip", "mat_kernel.hip", "helix_kernel.hip", "fdk_api.hip"]
+SOURCES = ["fdk_kernels.hip", "mat_kernel.hip", "helix_kernel.hip", "phantom_kernel.hip",
+           "fdk_api.hip"]
 ARCH = os.environ.get("MIRT_AMD_ARCH", "gfx950")
 
 
@@ -37,11 +38,6 @@ def flags() -> list[str]:
         f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
         "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        # FP16 dot products as the VOP3P v_dot2_f32_f16 (inline 0 and
-        # negated accumulators as operand modifiers) rather than the tied
-        # v_dot2c_f32_f16, which costs a v_mov per zero accumulator and a
-        # v_xor per negation: 7.7 instead of 9.7 VALU per FP16 slice update
-        "-Xclang", "-target-feature", "-Xclang", "-dot5-insts",
         "-Wall", "-Wno-unused-function",
         f"-I{INCLUDE}", f"-I{CSRC}",
     ]

@@ -49,6 +49,12 @@ static int fail(int code, const char *fmt, ...)
 			return fail(FDKHIP_EHIP, "%s: %s", #expr, hipGetErrorString(e_));  \
 	} while (0)
 
+// the error-recording entry for the library's other source files
+int fdkhip_fail_msg(int code, const char *msg)
+{
+	return fail(code, "%s", msg);
+}
+
 extern "C" const char *fdkhip_last_error(void)
 {
 	return g_last_error.c_str();
@@ -182,7 +188,7 @@ struct fdkhip_plan {
 	unsigned long long *d_count = nullptr;
 	int *d_maxarea = nullptr;
 	int *d_any = nullptr;
-	void *d_padded = nullptr;     // zero-padded copy of the views (see launch_pad)
+	void *d_padded = nullptr;     // FP32: zero-padded copy of the views (see launch_pad)
 	// survey results per launch shape (nw, zl): maxarea, fov_any, n_updates
 	struct Survey {
 		int nw, zl, maxarea, fov_any;
@@ -198,6 +204,7 @@ struct fdkhip_plan {
 	int maxarea = 0, fov_any = 0;
 	size_t lds_limit = 65536;   // per workgroup, for the current configuration
 	size_t lds_max = 65536;     // device limit per workgroup
+	hipStream_t survey_stream = nullptr;   // stream of the survey pass (set-up)
 	// graph cache
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t gexec = nullptr;
@@ -298,15 +305,18 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 			cached = true;
 		}
 	if (!cached && p->ngroups > 0 && p->tiles_x > 0 && p->tiles_y > 0 && p->na > 0) {
-		HIP_TRY(launch_zero_u64(p->d_count, 1, nullptr));
-		HIP_TRY(launch_zero_i32(p->d_maxarea, 1, nullptr));
-		HIP_TRY(launch_zero_i32(p->d_any, 1, nullptr));
+		hipStream_t ss = p->survey_stream;
+		HIP_TRY(launch_zero_u64(p->d_count, 1, ss));
+		HIP_TRY(launch_zero_i32(p->d_maxarea, 1, ss));
+		HIP_TRY(launch_zero_i32(p->d_any, 1, ss));
 		BackArgs a = make_args(p, nullptr, nullptr, nullptr);
-		HIP_TRY(launch_survey(a, nw, zl, p->fmt != FDKHIP_F32, nullptr));
+		HIP_TRY(launch_survey(a, nw, zl, p->fmt != FDKHIP_F32, ss));
 		unsigned long long cnt = 0;
-		HIP_TRY(hipMemcpy(&cnt, p->d_count, sizeof cnt, hipMemcpyDeviceToHost));
-		HIP_TRY(hipMemcpy(&p->maxarea, p->d_maxarea, sizeof(int), hipMemcpyDeviceToHost));
-		HIP_TRY(hipMemcpy(&p->fov_any, p->d_any, sizeof(int), hipMemcpyDeviceToHost));
+		HIP_TRY(hipMemcpyAsync(&cnt, p->d_count, sizeof cnt, hipMemcpyDeviceToHost, ss));
+		HIP_TRY(hipMemcpyAsync(&p->maxarea, p->d_maxarea, sizeof(int), hipMemcpyDeviceToHost,
+			ss));
+		HIP_TRY(hipMemcpyAsync(&p->fov_any, p->d_any, sizeof(int), hipMemcpyDeviceToHost, ss));
+		HIP_TRY(hipStreamSynchronize(ss));
 		p->n_updates = (long long) cnt;
 		p->surveys.push_back({nw, zl, p->maxarea, p->fov_any, p->n_updates});
 	}
@@ -398,10 +408,10 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	PLAN_TRY(hipMalloc(&p->d_maxarea, sizeof(int)));
 	PLAN_TRY(hipMalloc(&p->d_any, sizeof(int)));
 	{
-		// FP32: [na][ns+2][nt+2] floats; FP16/BF16: [na][ns+1][nt+2] column pairs
+		// FP32: [na][ns+2][nt+2] floats; FP16/BF16 views are staged straight
+		// from the caller's buffer and need no copy
 		p->padded_bytes = (fmt == FDKHIP_F32)
-			? (size_t) na * (g->ns + 2) * (g->nt + 2) * sizeof(float)
-			: (size_t) na * (g->ns + 1) * (g->nt + 2) * sizeof(uint32_t);
+			? (size_t) na * (g->ns + 2) * (g->nt + 2) * sizeof(float) : 0;
 		if (p->padded_bytes > 0)
 			PLAN_TRY(hipMalloc(&p->d_padded, p->padded_bytes));
 	}
@@ -483,8 +493,9 @@ static int plan_launch(fdkhip_plan *p, const void *proj, const float *scale, flo
 		HIP_TRY(launch_fill(image, ncol, p->geom.nz, work ? zbeg : 0, work ? zend : 0, s));
 	if (ev)
 		HIP_TRY(hipEventRecord(ev[1], s));
-	if (work)
-		HIP_TRY(launch_pad(proj, p->d_padded, p->geom.ns, p->geom.nt, p->na, p->fmt, s));
+	if (work && p->fmt == FDKHIP_F32)
+		HIP_TRY(launch_pad(static_cast<const float *>(proj), static_cast<float *>(p->d_padded),
+			p->geom.ns, p->geom.nt, p->na, s));
 	if (ev)
 		HIP_TRY(hipEventRecord(ev[2], s));
 	if (work) {
@@ -1755,8 +1766,9 @@ static int batch_launch(fdkhip_batch *b, const void *const *proj, const float *c
 		const bool work = p->ngroups > 0 && p->na > 0;
 		const int zend = std::min(p->z0 + p->ngroups * p->nw * p->zl, p->geom.nz);
 		HIP_TRY(launch_fill(image[i], ncol, p->geom.nz, work ? p->z0 : 0, work ? zend : 0, s));
-		if (work)
-			HIP_TRY(launch_pad(proj[i], p->d_padded, p->geom.ns, p->geom.nt, p->na, p->fmt, s));
+		if (work && p->fmt == FDKHIP_F32)
+			HIP_TRY(launch_pad(static_cast<const float *>(proj[i]),
+				static_cast<float *>(p->d_padded), p->geom.ns, p->geom.nt, p->na, s));
 	}
 	if (ev)
 		HIP_TRY(hipEventRecord(ev[1], s));

@@ -24,7 +24,7 @@ struct GeomF {
 struct BackArgs {
 	GeomF g;
 	const void *proj;        // [na][ns][nt] (t fastest), storage format fmt
-	const void *padded;      // [na][ns+2][nt+2] zero-padded copy of proj (same format)
+	const void *padded;      // FP32: [na][ns+2][nt+2] zero-padded copy of proj
 	const float *scale;      // [na] per-view scale (F16/BF16), else unused
 	const float2 *trig;      // [na] {cosf(beta), sinf(beta)} of the float beta
 	const uint8_t *mask;     // [ny][nx] or null
@@ -110,8 +110,8 @@ hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
 // out[b*out_b + c*out_c + r] = scale * in[b*in_b + clamp(r - rshift, 0, R-1)*in_r + c]
 hipError_t launch_transpose(const float *in, float *out, int B, int R, int C, long in_b,
 	long in_r, long out_b, long out_c, int Rout, int rshift, float scale, hipStream_t s);
-hipError_t launch_pad(const void *src, void *dst, int ns, int nt, int na, int fmt,
-	hipStream_t s);
+// zero-padded copy of FP32 views: [na][ns][nt] -> [na][ns+2][nt+2]
+hipError_t launch_pad(const float *src, float *dst, int ns, int nt, int na, hipStream_t s);
 hipError_t launch_zero_i32(int *p, int n, hipStream_t s);
 hipError_t launch_st(const StArgs &a, hipStream_t s);
 hipError_t launch_quantize(const float *proj, int ns, int nt, int na, int fmt,

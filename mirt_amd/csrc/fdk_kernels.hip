@@ -52,21 +52,11 @@ __device__ __forceinline__ int to_int_x86(float f)
 	return (f >= -2147483648.0f && f < 2147483648.0f) ? (int) f : (int) 0x80000000u;
 }
 
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
-
-// (lo, hi) rounded to FP16 (nearest even) in one 32-bit word, lo in bits 0-15
-__device__ __forceinline__ uint32_t pack_f16x2(float lo, float hi)
+// FP32 value of an FP16 sample (exact); in an FMA with an FP32 operand the
+// compiler folds the conversion into one v_fma_mix_f32
+__device__ __forceinline__ float f16_to_f32(uint32_t bits16)
 {
-	return (uint32_t) __builtin_bit_cast(unsigned short, (_Float16) lo)
-		| ((uint32_t) __builtin_bit_cast(unsigned short, (_Float16) hi) << 16);
-}
-
-// wl*lo + wr*hi of an FP16 column pair and FP16 horizontal weights packed as
-// (wl, wr), accumulated in FP32 by one v_dot2_f32_f16
-__device__ __forceinline__ float dot_pair_f16(uint32_t pair, uint32_t wpk)
-{
-	return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, pair),
-		__builtin_bit_cast(half2v, wpk), 0.f, false);
+	return (float) __builtin_bit_cast(_Float16, (unsigned short) bits16);
 }
 
 // FP32 sample idx of the zero-padded copy of FP32 views
@@ -252,8 +242,10 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 //   u    = U[k] + j * t_inc                             (slice zg0 + k*ZL + j)
 // each step one fmaf (one rounding each; nearer the exact value than the
 // reference's running sum).  The range [zlo, zhi) is every slice of the
-// group with 0 < u < nt + 1, found with that same arithmetic; first/last
-// are u at zlo and zhi - 1.  Uk[0] receives U0.
+// group with 0 < u < nt + 1, found with that same arithmetic (u grows with
+// z: the first guess from iz_min/iz_max is moved both ways until it is
+// exact, so every slice outside the range has u <= 0 or u >= nt + 1);
+// first/last are u at zlo and zhi - 1.  Uk[0] receives U0.
 template <int NW, int ZL>
 __device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
 	int zg0, int zg1, float (&Uk)[NW], int &zlo, int &zhi, float &first, float &last)
@@ -266,9 +258,14 @@ __device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
 		return fmaf((float) (z - zg0 - j0), ti, fmaf((float) j0, ti, U0));
 	};
 	const float uhi = (float) g.nt + 1.f;
-	int lo = max(c.iz_min - 1, zg0), hi = min(c.iz_max + 1, zg1);
-	while (lo < hi && uz(lo) <= 0.f)
+	int lo = min(max(c.iz_min - 1, zg0), zg1);
+	while (lo > zg0 && uz(lo - 1) > 0.f)
+		--lo;
+	while (lo < zg1 && uz(lo) <= 0.f)
 		++lo;
+	int hi = max(min(c.iz_max + 1, zg1), lo);
+	while (hi < zg1 && uz(hi) < uhi)
+		++hi;
 	while (hi > lo && uz(hi - 1) >= uhi)
 		--hi;
 	zlo = lo;
@@ -282,11 +279,21 @@ __device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
 // quantities, the slice range [zlo, zhi), the vertical coordinate at each
 // chunk's start (t_bin for the reference walk, u = t_bin + 1 for DIRECT) and
 // the padded detector rows [row_lo, row_hi] its bilinear reads touch.
+// DIRECT consumers compute every slice of a chunk they work on, with u
+// clamped to [0, u_hi]: a slice below the range (u <= 0) then reads padded
+// rows 0 and 1 with vertical weight 0, i.e. exactly the zero pad row; one
+// above it (u >= nt + 1) reads rows nt + 1 and nt + 2 with weight 0, two
+// zero rows (row nt + 2 is one more zero row of the staged footprint).  So
+// the rows read include 0 when the range starts inside the group and
+// nt + 2 when it ends inside it.  Where the range reaches the group's end,
+// u_hi = u at zhi - 1 keeps the (unwritten) slices past the volume's last
+// slice inside the footprint.
 struct ColRange {
 	ColSetup c;
 	int zlo, zhi;
 	float first, last;     // t (walk) or u (DIRECT) at zlo and zhi - 1
 	int row_lo, row_hi;    // padded rows read
+	float u_hi;            // DIRECT: upper clamp of u
 };
 
 template <int NW, int ZL, bool DIRECT>
@@ -295,13 +302,16 @@ __device__ __forceinline__ bool col_range(const GeomF &g, float xx, float yy, fl
 {
 	r.zlo = r.zhi = 0;
 	r.first = r.last = 0.f;
+	r.u_hi = 0.f;
 	if constexpr (DIRECT) {
 		if (column_setup_fast(g, xx, yy, cs.x, cs.y, r.c) != SETUP_ACTIVE)
 			return false;
 		if (!setup_direct<NW, ZL>(g, r.c, zg0, zg1, Tk, r.zlo, r.zhi, r.first, r.last))
 			return false;
-		r.row_lo = (int) r.first;             // u > 0: truncation is floor
-		r.row_hi = (int) r.last + 1;
+		const bool cut_hi = r.zhi < zg1;
+		r.row_lo = (r.zlo > zg0) ? 0 : (int) r.first;         // u > 0: truncation is floor
+		r.row_hi = cut_hi ? g.nt + 2 : (int) r.last + 1;
+		r.u_hi = cut_hi ? (float) g.nt + 1.f : r.last;
 	} else {
 		if (column_setup(g, xx, yy, cs.x, cs.y, r.c) != SETUP_ACTIVE)
 			return false;
@@ -315,17 +325,20 @@ __device__ __forceinline__ bool col_range(const GeomF &g, float xx, float yy, fl
 
 // Wave-wide reductions (wave64, every lane active) with DPP: quad swaps,
 // row half-mirror and mirror give each row of 16 its result, row_bcast:15
-// and :31 carry it across rows; lane 63 holds the wave's result.
+// and :31 carry it across rows; lane 63 holds the wave's result.  The
+// operation's identity as the DPP "old" value (what lanes outside the row
+// mask receive) lets the compiler fold each step into one v_min/v_max_dpp.
 template <bool MAX>
 __device__ __forceinline__ int wave_reduce(int v)
 {
 	auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
-	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xb1, 0xf, 0xf, false));   // quad_perm 1,0,3,2
-	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4e, 0xf, 0xf, false));   // quad_perm 2,3,0,1
-	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xf, 0xf, false));  // row_half_mirror
-	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xf, 0xf, false));  // row_mirror
-	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
-	v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+	constexpr int id = MAX ? (int) 0x80000000 : 0x7fffffff;
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0xb1, 0xf, 0xf, false));   // quad_perm 1,0,3,2
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x4e, 0xf, 0xf, false));   // quad_perm 2,3,0,1
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));  // row_half_mirror
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));  // row_mirror
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
 	return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ __forceinline__ int wave_min(int v)
@@ -354,55 +367,122 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks)
 // with a word offset so that the compiler emits LDS (ds_*) instructions
 extern __shared__ float fdk_lds[];
 
-// LDS layout of one stage buffer (4-byte words); the kernel holds two.
-//   slots [vstage][slot]        staged footprints
+// LDS layout of one stage buffer (4-byte words); the kernel holds two, and
+// after them two zero words that DIRECT lanes without work in a view read.
+//   slots [vstage][slot]        staged footprints: FP32 samples (walk modes),
+//                               16-bit column pairs (DIRECT)
 //   P     [vstage][64][8]       per (view, column) record, two 16-byte halves
 //                               read with one ds_read_b128 each:
-//           P0 = {cb, wl, wr, w2 | u_first}   (FP16: wl = the FP16 pair (wl, wr),
-//                                             wr = w2 * view scale)
-//                                             cb: LDS word of the lane's left
-//                                             column at row 1 (walk) / padded
-//                                             row 0 (DIRECT); for views
-//                                             gathered from HBM: is
-//           P1 = {t_inc, zlo | zhi << 16, is | U0, u_last}
+//     walk:   P0 = {cb, wl, wr, w2}      P1 = {t_inc, zlo | zhi << 16, is, t_last}
+//     DIRECT: P0 = {cb, wl, wr, U0}      P1 = {t_inc, zlo | zhi << 16, u_hi, 0}
+//             (DIRECT wl, wr: FP32 horizontal weights times w2 times the
+//             view scale)
+//           cb: LDS word, relative to the stage buffer, of the lane's left
+//               column at row 1 (walk) / padded row 0 (DIRECT); for views
+//               gathered from HBM: is
 //   T     [vstage][NW][64]      walk only: t_bin at each consumer wave's first
 //                               active slice (DIRECT evaluates it from U0)
 //   vinfo [vstage][4]           s_lo, t_lo, tw, (mode << 16) | staged columns
 template <int NW, bool TTAB>
 struct Lds {
-	int P, T, vinfo, total;
+	int P, T, vinfo, total, zero;
 	__device__ __forceinline__ Lds(int vstage, int slot)
 	{
 		P = vstage * slot;
 		T = P + vstage * 64 * 8;
 		vinfo = T + (TTAB ? vstage * NW * 64 : 0);
 		total = vinfo + vstage * 4;
+		zero = 2 * total;
 	}
 };
 
 enum { MODE_LDS = 0, MODE_GLOBAL = 1, MODE_NONE = 2 };
 
+// Detector footprint of one view over the tile and z-group in padded
+// coordinates, the union over the wave's active lanes: columns
+// [s_lo, s_hi], rows [t_lo, t_hi] (DIRECT: up to row nt + 2, a zero row).
+// Staged words: (s_hi - s_lo + 1) x tw FP32 samples (walk) or
+// (s_hi - s_lo) x tw column pairs (DIRECT), tw = t_hi - t_lo + 1.
+struct Footprint {
+	int s_lo, s_hi, t_lo, t_hi;
+};
 
-// Asynchronous copy of one sample per lane from HBM into the LDS word
+template <bool DIRECT>
+__device__ __forceinline__ Footprint view_footprint(const GeomF &g, bool act, const ColRange &cr)
+{
+	Footprint f;
+	f.s_lo = max(wave_min(act ? cr.c.is + 1 : 0x7fffffff), 0);
+	f.s_hi = min(wave_max(act ? cr.c.is + 2 : (int) 0x80000000), g.ns + 1);
+	f.t_lo = max(wave_min(act ? cr.row_lo : 0x7fffffff), 0);
+	f.t_hi = min(wave_max(act ? cr.row_hi : (int) 0x80000000), DIRECT ? g.nt + 2 : g.nt + 1);
+	return f;
+}
+
+// Asynchronous copy of one FP32 sample per lane from HBM into the LDS word
 // `ldst + lane` (global_load_lds: the LDS address is wave-uniform plus
-// 4 bytes per lane; 16-bit samples land zero-extended in their word).
-template <int FMT>
+// 4 bytes per lane).
 __device__ __forceinline__ void stage_word(const char *src, float *ldst)
 {
 	const auto g = (const void __attribute__((address_space(1))) *) src;
 	const auto l = (void __attribute__((address_space(3))) *) ldst;
-	if constexpr (FMT == FMT_F32)
-		__builtin_amdgcn_global_load_lds(g, l, 4, 0, 0);
-	else
-		__builtin_amdgcn_global_load_lds(g, l, 2, 0, 0);
+	__builtin_amdgcn_global_load_lds(g, l, 4, 0, 0);
+}
+
+// DIRECT staging of one view's footprint: slot word (k, r), k in [0, sw),
+// r in [0, tw), holds the padded samples (s_lo + k, t_lo + r) in its low and
+// (s_lo + k + 1, t_lo + r) in its high half, read straight from the caller's
+// [ns][nt] 16-bit view and zero outside [1, ns] x [1, nt] (the zero border
+// that fdk_ts_put_view gives each view, fdk,ts,t.c:13-21).  Lane = detector
+// row: the wave walks the columns keeping the previous column's sample, with
+// eight 2-byte loads in flight (64 consecutive samples of a column per load,
+// one 128-byte line).  Each sample is read from HBM once per footprint.
+__device__ __forceinline__ void stage_pairs(const uint16_t *view, const GeomF &g,
+	const Footprint &f, int sw, int tw, int slot, int lane)
+{
+	// zero rows of the footprint: padded rows 0, nt + 1 and nt + 2
+	for (int q = 0; q < 3; ++q) {
+		const int zr = (q == 0) ? 0 : g.nt + q;
+		if (zr < f.t_lo || zr > f.t_hi)
+			continue;
+		for (int k = lane; k < sw; k += 64)
+			fdk_lds[slot + k * tw + (zr - f.t_lo)] = 0.f;
+	}
+	const int ra = max(f.t_lo, 1), rb = min(f.t_hi, g.nt);   // detector rows
+	for (int r0 = ra; r0 <= rb; r0 += 64) {
+		const int r = r0 + lane;
+		const bool rv = r <= rb;
+		const int srow = (rv ? r : rb) - 1;    // sample row (a valid one past the end)
+		const int lw = slot + (r - f.t_lo);
+		uint32_t prev = 0u;                    // padded column s_lo (0: the zero pad)
+		if (f.s_lo >= 1)
+			prev = view[(long) (f.s_lo - 1) * g.nt + srow];
+		for (int k0 = 0; k0 < sw; k0 += 8) {
+			uint32_t cur[8];
+#pragma unroll
+			for (int q = 0; q < 8; ++q) {
+				// padded column s_lo + k0 + q + 1 is sample column s_lo + k0 + q;
+				// clamped so that the loads past the footprint stay in the view
+				const int sc = min(f.s_lo + k0 + q, g.ns - 1);
+				cur[q] = view[(long) sc * g.nt + srow];
+			}
+#pragma unroll
+			for (int q = 0; q < 8; ++q) {
+				const int k = k0 + q;
+				const uint32_t hi = (f.s_lo + k + 1 <= g.ns) ? cur[q] : 0u;
+				if (k < sw && rv)
+					fdk_lds[lw + k * tw] = __uint_as_float(prev | (hi << 16));
+				prev = hi;
+			}
+		}
+	}
 }
 
 // Producer side: per-(column, view) setup of `nv` views into the stage
 // buffer at word offset `bo`, and staging of their footprints, by producer
-// wave `pw` of NP.  The footprint goes to LDS with asynchronous
-// global-to-LDS loads from the zero-padded copy of the views (one dword
-// per lane: the FP32 sample, or the 16-bit sample zero-extended).  The
-// caller waits for the loads (vmcnt) before the stage barrier.
+// wave `pw` of NP.  Walk modes (FP32) stage from the zero-padded copy with
+// asynchronous global-to-LDS loads, which the caller waits for (vmcnt)
+// before the stage barrier; DIRECT (FP16/BF16) stages column pairs from the
+// caller's views (stage_pairs).
 template <int NW, int NP, int ZL, int FMT, bool FAST>
 __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 	const Lds<NW, FMT == FMT_F32> &L,
@@ -412,50 +492,49 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 	constexpr bool DIRECT = FMT != FMT_F32;
 	for (int vv = pw; vv < nv; vv += NP) {
 		const int v = v0 + vv;
-		ColRange cr;
-		float Tk[NW];
+		// zero-initialised: a lane whose column is invalid never runs
+		// col_range, and its record must still describe an empty range
+		ColRange cr{};
+		float Tk[NW] = {};
 		const bool act = colvalid
 			&& col_range<NW, ZL, DIRECT>(g, xx, yy, a.trig[v], zg0, zg1, cr, Tk);
 		const ColSetup &c = cr.c;
-		// footprint of this view over the tile and z-group (padded coords),
-		// kept inside the padded view [0, ns+1] x [0, nt+1]
-		const int s_lo = max(wave_min(act ? c.is + 1 : 0x7fffffff), 0);
-		const int s_hi = min(wave_max(act ? c.is + 2 : (int) 0x80000000), g.ns + 1);
-		const int t_lo = max(wave_min(act ? cr.row_lo : 0x7fffffff), 0);
-		const int t_hi = min(wave_max(act ? cr.row_hi : (int) 0x80000000), g.nt + 1);
+		const Footprint f = view_footprint<DIRECT>(g, act, cr);
 		int mode = MODE_NONE, tw = 0, sw = 0;
-		if (s_lo <= s_hi && t_lo <= t_hi) {
-			tw = t_hi - t_lo + 1;
-			sw = s_hi - s_lo + (DIRECT ? 0 : 1);     // staged columns (DIRECT: pairs)
+		if (f.s_lo <= f.s_hi && f.t_lo <= f.t_hi) {
+			tw = f.t_hi - f.t_lo + 1;
+			sw = f.s_hi - f.s_lo + (DIRECT ? 0 : 1);     // staged columns (DIRECT: pairs)
 			mode = ((long) sw * tw <= a.slot) ? MODE_LDS : MODE_GLOBAL;
 		}
-		// LDS word of padded row 0 (DIRECT) or of row 1, t = 0 (walk) in the
-		// lane's left column
-		const int cb = act ? vv * a.slot + (c.is + 1 - s_lo) * tw + (DIRECT ? 0 : 1) - t_lo : 0;
-		float wl = act ? c.wl : 0.f, wr = act ? c.wr : 0.f;
-		const float w2 = act ? c.w2 : 0.f;
-		if constexpr (FMT == FMT_F16) {
-			// FP16 horizontal weights, packed; fan-beam weight times view scale
-			const float w2s = w2 * a.scale[v];
-			wl = __uint_as_float(pack_f16x2(wl, wr));
-			wr = w2s;
-		} else if (FAST) {
-			const float sc = (FMT != FMT_F32) ? a.scale[v] : 1.f;
-			wl = w2 * wl * sc;
-			wr = w2 * wr * sc;
-		}
-		const int zhi_st = act ? cr.zhi : cr.zlo;
 		const int is_st = act ? c.is : 0;
+		const int zhi_st = act ? cr.zhi : cr.zlo;
 		float4 *P4 = reinterpret_cast<float4 *>(fdk_lds + bo + L.P) + (vv * 64 + lane) * 2;
-		P4[0] = make_float4(__int_as_float(mode == MODE_LDS ? cb : is_st), wl, wr,
-			DIRECT ? cr.first : w2);
-		P4[1] = make_float4(act ? c.t_inc : 0.f, __int_as_float(cr.zlo | (zhi_st << 16)),
-			DIRECT ? (act ? Tk[0] : 0.f) : __int_as_float(is_st), cr.last);
-		// walk: t_bin at each chunk's first active slice; chunks the lane's
-		// range misses get t_first, so that their (discarded) reads stay
-		// inside the staged footprint.  DIRECT: the consumer evaluates u at
-		// its chunk from U0 and clamps it to [first, last].
-		if constexpr (!DIRECT) {
+		if constexpr (DIRECT) {
+			// LDS word of padded row 0 in the lane's column pair; lanes without
+			// work in this view read the two zero words with zero weights
+			const int cb = !act ? L.zero - bo
+				: (mode == MODE_LDS ? vv * a.slot + (c.is + 1 - f.s_lo) * tw - f.t_lo : c.is);
+			const float w2s = act ? c.w2 * a.scale[v] : 0.f;
+			P4[0] = make_float4(__int_as_float(cb), act ? w2s * c.wl : 0.f,
+				act ? w2s * c.wr : 0.f, act ? Tk[0] : 0.f);
+			P4[1] = make_float4(act ? c.t_inc : 0.f,
+				__int_as_float(act ? cr.zlo | (zhi_st << 16) : 0), act ? cr.u_hi : 0.f, 0.f);
+		} else {
+			// LDS word of row 1, t = 0 in the lane's left column
+			const int cb = act ? vv * a.slot + (c.is + 1 - f.s_lo) * tw + 1 - f.t_lo : 0;
+			float wl = act ? c.wl : 0.f, wr = act ? c.wr : 0.f;
+			const float w2 = act ? c.w2 : 0.f;
+			if (FAST) {
+				wl = w2 * wl;
+				wr = w2 * wr;
+			}
+			P4[0] = make_float4(__int_as_float(mode == MODE_LDS ? cb : is_st), wl, wr, w2);
+			P4[1] = make_float4(act ? c.t_inc : 0.f,
+				__int_as_float(act ? cr.zlo | (zhi_st << 16) : 0), __int_as_float(is_st),
+				act ? cr.last : 0.f);
+			// t_bin at each chunk's first active slice; chunks the lane's range
+			// misses get t_first, so that their (discarded) reads stay inside
+			// the staged footprint
 #pragma unroll
 			for (int k = 0; k < NW; ++k) {
 				const bool meets = cr.zlo < zg0 + (k + 1) * ZL && zg0 + k * ZL < cr.zhi;
@@ -465,24 +544,25 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 		}
 		if (lane == 0) {
 			int *vi = reinterpret_cast<int *>(fdk_lds) + bo + L.vinfo + vv * 4;
-			vi[0] = s_lo;
-			vi[1] = t_lo;
+			vi[0] = f.s_lo;
+			vi[1] = f.t_lo;
 			vi[2] = tw;
 			vi[3] = (mode << 16) | (sw & 0xffff);
 		}
 		if (mode != MODE_LDS)
 			continue;
-		// stage the footprint of the zero-padded view (the padded copy plays
-		// fdk_ts_put_view, fdk,ts,t.c:13-21): LDS word e of the slot <- padded
-		// word (s_lo + e / tw, t_lo + e % tw) -- an FP32 sample, or for
-		// FP16/BF16 the column pair (s, s + 1)
-		const int area = sw * tw;
 		const int slot = bo + vv * a.slot;
+		if constexpr (DIRECT) {
+			stage_pairs(static_cast<const uint16_t *>(a.proj) + (size_t) v * g.ns * g.nt, g, f,
+				sw, tw, slot, lane);
+			continue;
+		}
+		// FP32: LDS word e of the slot <- padded sample (s_lo + e / tw, t_lo + e % tw)
+		const int area = sw * tw;
 		const int pnt = g.nt + 2;
 		constexpr int esz = 4;
-		const int pcols = DIRECT ? g.ns + 1 : g.ns + 2;
 		const char *pv = static_cast<const char *>(a.padded)
-			+ ((size_t) v * pcols + s_lo) * pnt * esz + (size_t) t_lo * esz;
+			+ ((size_t) v * (g.ns + 2) + f.s_lo) * pnt * esz + (size_t) f.t_lo * esz;
 		// byte offsets from the view's footprint corner stay below 2^32 (one
 		// view), so the loads take a scalar base plus a 32-bit lane offset
 		if (tw == pnt) {
@@ -490,7 +570,7 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 			unsigned off = (unsigned) lane * esz;
 			for (int e0 = 0; e0 < area; e0 += 64, off += 64 * esz)
 				if (e0 + lane < area)
-					stage_word<FMT_F32>(pv + off, fdk_lds + slot + e0);
+					stage_word(pv + off, fdk_lds + slot + e0);
 		} else {
 			// rows of tw samples: (r, tc) of element e, advanced by 64 per step;
 			// off = (r * pnt + tc) * esz
@@ -504,7 +584,7 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 			const unsigned wrap_step = (unsigned) (pnt - tw) * esz;
 			for (int e0 = 0; e0 < area; e0 += 64) {
 				if (e0 + lane < area)
-					stage_word<FMT_F32>(pv + off, fdk_lds + slot + e0);
+					stage_word(pv + off, fdk_lds + slot + e0);
 				tc += dtc;
 				off += step;
 				if (tc >= tw) {
@@ -524,7 +604,7 @@ template <int FMT, bool FAST>
 __device__ __forceinline__ float slice_update(float acc, float t, int c1, int c2,
 	float wl, float wr, float w2)
 {
-	static_assert(FMT == FMT_F32, "FP16/BF16 views take slice_update_u");
+	static_assert(FMT == FMT_F32, "FP16/BF16 views take slice_update_d");
 	const float fl = floorf(t);
 	const int it = (int) fl;
 	const float wu = t - fl;
@@ -544,12 +624,22 @@ __device__ __forceinline__ float slice_update(float acc, float t, int c1, int c2
 	}
 }
 
+// FP32 value of a 16-bit sample (FP16 or BF16; exact)
+template <int FMT>
+__device__ __forceinline__ float sample16(uint32_t bits)
+{
+	if constexpr (FMT == FMT_F16)
+		return f16_to_f32(bits);
+	else
+		return __uint_as_float(bits << 16);
+}
+
 // FP32 values of the two halves of a column-pair word (FP16 or BF16)
 template <int FMT>
 __device__ __forceinline__ float pair_lo(uint32_t w)
 {
 	if constexpr (FMT == FMT_F16)
-		return __half2float(__ushort_as_half((unsigned short) (w & 0xffffu)));
+		return f16_to_f32(w & 0xffffu);
 	else
 		return __uint_as_float(w << 16);
 }
@@ -557,89 +647,74 @@ template <int FMT>
 __device__ __forceinline__ float pair_hi(uint32_t w)
 {
 	if constexpr (FMT == FMT_F16)
-		return __half2float(__ushort_as_half((unsigned short) (w >> 16)));
+		return f16_to_f32(w >> 16);
 	else
 		return __uint_as_float(w & 0xffff0000u);
 }
 
-// FP16 slice update: the horizontal weights (wl, wr) are FP16 in wpk and,
-// sign-flipped, in wneg; the fan-beam weight times the view scale is w2s.
-// CHAIN: the first row's dot product against wneg is -q1 exactly; the second
-// row's accumulates onto it, giving q2 - q1 for the vertical interpolation
-// q1 + wu*(q2 - q1) in one FMA (4 VALU after the address).  No operand
-// modifier is asked of the dot products' FP32 accumulator: in the VOP3P
-// encoding its neg_hi bit means abs, not negation, so a negated accumulator
-// there would add -|q1|.  Otherwise the two rows' dot products are
-// independent and q2 - q1 is one more subtraction (5 VALU): consumers of 16
-// or fewer slices have too few slices per view to hide the chained form's
-// latency and run faster this way; 32-slice consumers run faster chained.
-template <bool CHAIN>
-__device__ __forceinline__ float slice_update_f16(float acc, float u, int c1, uint32_t wpk,
-	uint32_t wneg, float w2s)
+// Bilinear interpolation of FP16/BF16 samples with FP32 weights, added to
+// acc in FP32: a0, b0 on the lower row, a1, b1 on the upper one, wl, wr the
+// horizontal weights (w2 and the view scale folded in), wu the vertical one.
+// -q1 and q2 - q1 come from two chains of two FMAs (for FP16 each one
+// v_fma_mix_f32, which widens the 16-bit sample inside the instruction),
+// then acc + q1 + wu * (q2 - q1).
+__device__ __forceinline__ float bilerp_d(float acc, float a0, float b0, float a1, float b1,
+	float wl, float wr, float wu)
 {
-	const int iu = (int) u;
-	const float wu = __builtin_amdgcn_fractf(u);
-	const uint32_t w0 = __float_as_uint(fdk_lds[c1 + iu]);
-	const uint32_t w1 = __float_as_uint(fdk_lds[c1 + iu + 1]);
-	if constexpr (CHAIN) {
-		const float nq1 = dot_pair_f16(w0, wneg);
-		const float d = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, w1),
-			__builtin_bit_cast(half2v, wpk), nq1, false);
-		return fmaf(w2s, fmaf(wu, d, -nq1), acc);
-	} else {
-		const float q1 = dot_pair_f16(w0, wpk);
-		const float q2 = dot_pair_f16(w1, wpk);
-		return fmaf(w2s, fmaf(wu, q2 - q1, q1), acc);
-	}
+	const float nq1 = fmaf(-wr, b0, fmaf(-wl, a0, 0.f));
+	const float d = fmaf(wr, b1, fmaf(wl, a1, nq1));
+	return fmaf(wu, d, acc - nq1);
 }
 
-// The same update for the BF16 path in u = t_bin + 1 (> 0): the padded row
-// is the truncation of u, the vertical weight its fraction; c1 addresses
-// padded row 0 of the lane's staged column pair (is, is + 1); w2 and the
-// view scale are folded into wl/wr (FP32: BF16 weights would be too coarse).
+// DIRECT slice update at u = t_bin + 1 (> 0, or clamped: see ColRange): the
+// padded row is the truncation of u, the vertical weight its fraction; c1
+// addresses padded row 0 of the lane's staged column pair (is, is + 1), so
+// one ds_read2_b32 fetches both rows' pairs.
 template <int FMT>
-__device__ __forceinline__ float slice_update_u(float acc, float u, int c1, float wl, float wr)
+__device__ __forceinline__ float slice_update_d(float acc, float u, int c1, float wl, float wr)
 {
-	static_assert(FMT == FMT_BF16, "FP16 views take slice_update_f16");
 	const int iu = (int) u;
 	const float wu = __builtin_amdgcn_fractf(u);
-	const float wd = 1.f - wu;
 	const uint32_t w0 = __float_as_uint(fdk_lds[c1 + iu]);
 	const uint32_t w1 = __float_as_uint(fdk_lds[c1 + iu + 1]);
-	const float q1 = fmaf(wr, pair_hi<FMT>(w0), wl * pair_lo<FMT>(w0));
-	const float q2 = fmaf(wr, pair_hi<FMT>(w1), wl * pair_lo<FMT>(w1));
-	acc = fmaf(wd, q1, acc);
-	return fmaf(wu, q2, acc);
+	return bilerp_d(acc, pair_lo<FMT>(w0), pair_hi<FMT>(w0), pair_lo<FMT>(w1),
+		pair_hi<FMT>(w1), wl, wr, wu);
 }
 
-// Consumer side: consumer wave `wave` adds the views of the stage buffer at
-// `bo` to its ZL register accumulators (slices [zc, zc + ZL) of its column).
-// The next view's record is read while the current one is added, so the
-// per-view LDS round trips overlap the slice arithmetic.
+// padded sample (c, r) of a 16-bit view [ns][nt]: zero outside [1, ns] x [1, nt]
+// (the load itself is clamped into the view, so that it needs no branch)
+__device__ __forceinline__ uint32_t padded16(const uint16_t *view, const GeomF &g, int c, int r)
+{
+	const bool in = c >= 1 && c <= g.ns && r >= 1 && r <= g.nt;
+	const int cc = min(max(c, 1), g.ns) - 1, rr = min(max(r, 1), g.nt) - 1;
+	const uint32_t v = view[cc * g.nt + rr];
+	return in ? v : 0u;
+}
+
+// Consumer side of the walk modes (FP32): consumer wave `wave` adds the
+// views of the stage buffer at `bo` to its ZL register accumulators (slices
+// [zc, zc + ZL) of its column), in view order (EXACT reproduces the
+// reference's sum order).  The next view's record is read while the current
+// one is added, so the per-view LDS round trips overlap the slice arithmetic.
 template <int NW, int ZL, int FMT, bool FAST>
-__device__ __forceinline__ void consume(const BackArgs &a, int bo,
-	const Lds<NW, FMT == FMT_F32> &L,
+__device__ __forceinline__ void consume_walk(const BackArgs &a, int bo, const Lds<NW, true> &L,
 	int v0, int nv, int wave, int lane, int zc, float (&acc)[ZL])
 {
-	constexpr bool DIRECT = FMT != FMT_F32;
 	const GeomF &g = a.g;
 	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
 	const float *Tw = fdk_lds + bo + L.T + wave * 64 + lane;
 	const int2 *VI = reinterpret_cast<const int2 *>(fdk_lds + bo + L.vinfo + 2);
-	const float jc = (float) (wave * ZL);   // DIRECT: this wave's first slice in the group
 	float4 np0 = P4[0], np1 = P4[1];
-	float nt0 = DIRECT ? 0.f : Tw[0];
+	float nt0 = Tw[0];
 	int2 nvi = VI[0];
 	for (int vv = 0; vv < nv; ++vv) {
 		const float4 p0 = np0, p1 = np1;
 		const int2 vi = nvi;
-		// t_bin (walk) or u (DIRECT) at this wave's first slice
-		const float t0 = DIRECT ? fmaf(jc, p1.x, p1.z) : nt0;
+		const float t0 = nt0;      // t_bin at the lane's first active slice of this chunk
 		if (vv + 1 < nv) {
 			np0 = P4[(vv + 1) * 128];
 			np1 = P4[(vv + 1) * 128 + 1];
-			if constexpr (!DIRECT)
-				nt0 = Tw[(vv + 1) * NW * 64];
+			nt0 = Tw[(vv + 1) * NW * 64];
 			nvi = VI[(vv + 1) * 2];
 		}
 		const int mode = __builtin_amdgcn_readfirstlane(vi.y) >> 16;
@@ -649,157 +724,175 @@ __device__ __forceinline__ void consume(const BackArgs &a, int bo,
 		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
 		const int jlo = max(zlo - zc, 0);
 		const int jhi = min(zhi - zc, ZL);
-		const bool any = jlo < jhi;
-		if (!__any(any))
+		if (!__any(jlo < jhi))
 			continue;                  // no lane of this wave meets the view
 		const int cb = __float_as_int(p0.x);
 		const float wl = p0.y, wr = p0.z, w2 = p0.w;
-		// FP16: the packed horizontal weights with both signs flipped
-		const uint32_t wneg = __float_as_uint(wl) ^ 0x80008000u;
 		const float ti = p1.x;
+		const unsigned nj = (unsigned) max(jhi - jlo, 0);
 		if (mode == MODE_LDS) {
-			const int tw = __builtin_amdgcn_readfirstlane(vi.x);
 			const int c1 = bo + cb;
+			const int tw = __builtin_amdgcn_readfirstlane(vi.x);
 			const int c2 = c1 + tw;
 			if (__all(jlo == 0 && jhi == ZL)) {
 				// every lane covers the whole chunk: straight-line slices
-				if constexpr (DIRECT) {
+				float t = t0;
 #pragma unroll
-					for (int j = 0; j < ZL; ++j) {
-						const float u = fmaf((float) j, ti, t0);
-						if constexpr (FMT == FMT_F16)
-							acc[j] = slice_update_f16<(ZL >= 32)>(acc[j], u, c1, __float_as_uint(wl), wneg, wr);
-						else
-							acc[j] = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
-					}
-				} else {
-					float t = t0;
-#pragma unroll
-					for (int j = 0; j < ZL; ++j) {
-						acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
-						t += ti;
-					}
+				for (int j = 0; j < ZL; ++j) {
+					acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
+					t += ti;
 				}
 			} else {
 				// partial chunk: slices outside [jlo, jhi) read inside the
 				// staged footprint and are discarded by a select; a lane whose
-				// range misses this chunk (jhi <= jlo) adds nothing
-				const unsigned nj = (unsigned) max(jhi - jlo, 0);
-				if constexpr (DIRECT) {
-					// u clamped into the lane's [u_first, u_last]
-					const float uf = w2, ul = p1.w;
-#pragma unroll
-					for (int g0 = 0; g0 < ZL; g0 += 8) {
-						// groups of 8 slices that no lane of the wave adds are skipped
-						if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
-							continue;
-						// 16-slice consumers: a group every lane adds whole runs
-						// straight-line, with the same u as the clamped slices
-						// below (8-slice consumers meet that case as a whole
-						// chunk above; in 32-slice consumers the extra code
-						// costs more registers than it saves)
-						if (ZL == 16 && __all(jlo <= g0 && jhi >= g0 + 8)) {
-#pragma unroll
-							for (int j = g0; j < g0 + 8; ++j) {
-								const float u = fmaf((float) j, ti, t0);
-								if constexpr (FMT == FMT_F16)
-									acc[j] = slice_update_f16<(ZL >= 32)>(acc[j], u, c1,
-										__float_as_uint(wl), wneg, wr);
-								else
-									acc[j] = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
-							}
-							continue;
-						}
-#pragma unroll
-						for (int j = g0; j < g0 + 8; ++j) {
-							const bool r = (unsigned) (j - jlo) < nj;
-							const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), uf, ul);
-							float up;
-							if constexpr (FMT == FMT_F16)
-								up = slice_update_f16<(ZL >= 32)>(acc[j], u, c1, __float_as_uint(wl), wneg, wr);
-							else
-								up = slice_update_u<FMT>(acc[j], u, c1, wl, wr);
-							acc[j] = r ? up : acc[j];
-						}
-					}
-				} else {
-					// at t0, the lane's first active slice of this chunk
-					float t = t0;
-#pragma unroll
-					for (int g0 = 0; g0 < ZL; g0 += 8) {
-						// groups of 8 slices that no lane of the wave adds are
-						// skipped (t advances only on added slices)
-						if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
-							continue;
-#pragma unroll
-						for (int j = g0; j < g0 + 8; ++j) {
-							const bool r = (unsigned) (j - jlo) < nj;
-							const float tv = r ? t : t0;
-							const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
-							acc[j] = r ? up : acc[j];
-							t = r ? t + ti : t;
-						}
-					}
-				}
-			}
-		} else if (any) {
-			// footprint larger than the LDS slot: gather from the padded copy
-			const int is = cb;                    // the record holds is for these views
-			const int pnt = g.nt + 2;
-			if constexpr (DIRECT) {
-				// column pair (is, is + 1), padded row 0
-				const uint32_t *pc = static_cast<const uint32_t *>(a.padded)
-					+ ((size_t) (v0 + vv) * (g.ns + 1) + is + 1) * pnt;
-#pragma unroll
-				for (int j = 0; j < ZL; ++j) {
-					if (j >= jlo && j < jhi) {
-						const float u = fmaf((float) j, ti, t0);
-						const int iu = (int) u;
-						const float wu = __builtin_amdgcn_fractf(u);
-						const uint32_t w0 = pc[iu], w1 = pc[iu + 1];
-						if constexpr (FMT == FMT_F16) {
-							const float bu = wr * wu;
-							acc[j] = fmaf(wr - bu, dot_pair_f16(w0, __float_as_uint(wl)), acc[j]);
-							acc[j] = fmaf(bu, dot_pair_f16(w1, __float_as_uint(wl)), acc[j]);
-						} else {
-							const float q1 = fmaf(wr, pair_hi<FMT>(w0),
-								fmaf(wl, pair_lo<FMT>(w0), 0.f));
-							const float q2 = fmaf(wr, pair_hi<FMT>(w1),
-								fmaf(wl, pair_lo<FMT>(w1), 0.f));
-							acc[j] = fmaf(1.f - wu, q1, acc[j]);
-							acc[j] = fmaf(wu, q2, acc[j]);
-						}
-					}
-				}
-			} else {
-				const size_t col = ((size_t) (v0 + vv) * (g.ns + 2) + is + 1) * pnt;
-				const size_t c1 = col + 1, c2 = c1 + pnt;  // (s = is, t = 0), (is+1, 0)
+				// range misses this chunk (jhi <= jlo) adds nothing.  Groups
+				// of 8 slices that no lane of the wave adds are skipped (t
+				// advances only on added slices)
 				float t = t0;
 #pragma unroll
-				for (int j = 0; j < ZL; ++j) {
-					if (j >= jlo && j < jhi) {
-						const int it = (int) floorf(t);
-						const float wu = t - (float) it;
-						const float wd = 1.f - wu;
-						const float a0 = load_sample(a.padded, c1 + it);
-						const float a1 = load_sample(a.padded, c1 + it + 1);
-						const float b0 = load_sample(a.padded, c2 + it);
-						const float b1 = load_sample(a.padded, c2 + it + 1);
-						if (FAST) {
-							const float q1 = fmaf(wr, b0, wl * a0);
-							const float q2 = fmaf(wr, b1, wl * a1);
-							acc[j] = fmaf(wd, q1, acc[j]);
-							acc[j] = fmaf(wu, q2, acc[j]);
-						} else {
-							const float q1 = wl * a0 + wr * b0;
-							const float q2 = wl * a1 + wr * b1;
-							const float p0v = wd * q1 + wu * q2;
-							acc[j] += w2 * p0v;
-						}
-						t += ti;
+				for (int g0 = 0; g0 < ZL; g0 += 8) {
+					if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
+						continue;
+#pragma unroll
+					for (int j = g0; j < g0 + 8; ++j) {
+						const bool r = (unsigned) (j - jlo) < nj;
+						const float tv = r ? t : t0;
+						const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
+						acc[j] = r ? up : acc[j];
+						t = r ? t + ti : t;
 					}
 				}
 			}
+		} else {
+			// footprint larger than the LDS slot: gather from the padded copy
+			// in HBM, every slice computed (no per-lane branches, so the
+			// accumulators stay in place), the ones outside [jlo, jhi) read at
+			// t0, inside the padded view, and discarded by a select
+			const int is = cb;                    // the record holds is for these views
+			const int pnt = g.nt + 2;
+			const size_t col = ((size_t) (v0 + vv) * (g.ns + 2) + is + 1) * pnt;
+			const size_t c1 = col + 1, c2 = c1 + pnt;  // (s = is, t = 0), (is+1, 0)
+			float t = t0;
+#pragma unroll
+			for (int j = 0; j < ZL; ++j) {
+				const bool r = (unsigned) (j - jlo) < nj;
+				const float tv = r ? t : t0;
+				const int it = (int) floorf(tv);
+				const float wu = tv - (float) it;
+				const float wd = 1.f - wu;
+				const float a0 = load_sample(a.padded, c1 + it);
+				const float a1 = load_sample(a.padded, c1 + it + 1);
+				const float b0 = load_sample(a.padded, c2 + it);
+				const float b1 = load_sample(a.padded, c2 + it + 1);
+				float up;
+				if (FAST) {
+					const float q1 = fmaf(wr, b0, wl * a0);
+					const float q2 = fmaf(wr, b1, wl * a1);
+					up = fmaf(wu, q2, fmaf(wd, q1, acc[j]));
+				} else {
+					const float q1 = wl * a0 + wr * b0;
+					const float q2 = wl * a1 + wr * b1;
+					const float p0v = wd * q1 + wu * q2;
+					up = acc[j] + w2 * p0v;
+				}
+				acc[j] = r ? up : acc[j];
+				t = r ? t + ti : t;
+				// one slice's gathers at a time (register budget)
+				__builtin_amdgcn_sched_barrier(0);
+			}
+		}
+	}
+}
+
+// Consumer side of DIRECT (FP16/BF16 storage): the views staged in LDS
+// first, in view order, then -- after the stage's LDS views, which only
+// changes the FP32 summation order -- any view whose footprint exceeded the
+// slot, gathered from HBM.  Keeping the two apart keeps one code path over
+// the accumulators in the main loop, so they stay in place in registers.
+// Every slice of a chunk a wave works on is computed with u clamped to
+// [0, u_hi] (ColRange): slices outside a lane's range read zero rows with
+// vertical weight 0, and lanes without work read the zero words.
+template <int NW, int ZL, int FMT>
+__device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const Lds<NW, false> &L,
+	int v0, int nv, int wave, int lane, int zc, float (&acc)[ZL])
+{
+	const GeomF &g = a.g;
+	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
+	const int2 *VI = reinterpret_cast<const int2 *>(fdk_lds + bo + L.vinfo + 2);
+	const float jc = (float) (wave * ZL);   // this wave's first slice in the z-group
+	float4 np0 = P4[0], np1 = P4[1];
+	int2 nvi = VI[0];
+	bool hbm_views = false;
+	for (int vv = 0; vv < nv; ++vv) {
+		const float4 p0 = np0, p1 = np1;
+		const int2 vi = nvi;
+		if (vv + 1 < nv) {
+			np0 = P4[(vv + 1) * 128];
+			np1 = P4[(vv + 1) * 128 + 1];
+			nvi = VI[(vv + 1) * 2];
+		}
+		const int mode = __builtin_amdgcn_readfirstlane(vi.y) >> 16;
+		hbm_views = hbm_views || mode == MODE_GLOBAL;
+		if (mode != MODE_LDS)
+			continue;
+		const int zlohi = __float_as_int(p1.y);
+		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
+		const int jlo = max(zlo - zc, 0);
+		const int jhi = min(zhi - zc, ZL);
+		if (!__any(jlo < jhi))
+			continue;                  // no lane of this wave meets the view
+		const int c1 = bo + __float_as_int(p0.x);
+		const float wl = p0.y, wr = p0.z;
+		const float ti = p1.x;
+		const float t0 = fmaf(jc, ti, p0.w);   // u at this wave's first slice
+		const float uh = p1.z;
+#pragma unroll
+		for (int g0 = 0; g0 < ZL; g0 += 8) {
+			// groups of 8 slices that no lane of the wave adds are skipped
+			if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
+				continue;
+#pragma unroll
+			for (int j = g0; j < g0 + 8; ++j) {
+				const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
+				acc[j] = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+			}
+		}
+	}
+	if (!hbm_views)
+		return;
+	// views whose footprint exceeded the LDS slot: gather from the caller's
+	// views, zero outside the detector; lanes without slices in a view read
+	// columns left of the detector (zeros)
+	for (int vv = 0; vv < nv; ++vv) {
+		const int2 vi = VI[vv * 2];
+		if ((__builtin_amdgcn_readfirstlane(vi.y) >> 16) != MODE_GLOBAL)
+			continue;
+		const float4 p0 = P4[vv * 128], p1 = P4[vv * 128 + 1];
+		const int zlohi = __float_as_int(p1.y);
+		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
+		const int jlo = max(zlo - zc, 0);
+		const int jhi = min(zhi - zc, ZL);
+		if (!__any(jlo < jhi))
+			continue;
+		const int is = (jlo < jhi) ? __float_as_int(p0.x) : -3;   // the record holds is
+		const float wl = p0.y, wr = p0.z;
+		const float ti = p1.x;
+		const float t0 = fmaf(jc, ti, p0.w);
+		const float uh = p1.z;
+		const uint16_t *pv = static_cast<const uint16_t *>(a.proj)
+			+ (size_t) (v0 + vv) * g.ns * g.nt;
+#pragma unroll
+		for (int j = 0; j < ZL; ++j) {
+			const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
+			const int iu = (int) u;
+			const float wu = __builtin_amdgcn_fractf(u);
+			acc[j] = bilerp_d(acc[j],
+				sample16<FMT>(padded16(pv, g, is + 1, iu)),
+				sample16<FMT>(padded16(pv, g, is + 2, iu)),
+				sample16<FMT>(padded16(pv, g, is + 1, iu + 1)),
+				sample16<FMT>(padded16(pv, g, is + 2, iu + 1)), wl, wr, wu);
+			// one slice's four gathers at a time (register budget)
+			__builtin_amdgcn_sched_barrier(0);
 		}
 	}
 }
@@ -831,6 +924,8 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 			(a.mask == nullptr || a.mask[(long) iy * g.nx + ix] != 0);
 		const float yy = g.dy * ((float) iy - g.wy);   // fdk,ts.c:114,120
 		const float xx = g.dx * ((float) ix - g.wx);
+		if (FMT != FMT_F32 && pw == 0 && lane < 2)
+			fdk_lds[L.zero + lane] = 0.f;   // read by DIRECT lanes without work
 		produce<NW, NP, ZL, FMT, FAST>(a, 0, L, 0, min(a.vstage, g.na), pw, lane,
 			colvalid, xx, yy, zg0, zg1);
 		__builtin_amdgcn_s_waitcnt(0);   // staged footprints have landed in LDS
@@ -854,8 +949,12 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 		__syncthreads();
 		for (int s = 0; s < nstage; ++s) {
 			const int v0 = s * a.vstage;
-			consume<NW, ZL, FMT, FAST>(a, (s & 1) * L.total, L, v0, min(a.vstage, g.na - v0),
-				wave, lane, zc, acc);
+			if constexpr (FMT == FMT_F32)
+				consume_walk<NW, ZL, FMT, FAST>(a, (s & 1) * L.total, L, v0,
+					min(a.vstage, g.na - v0), wave, lane, zc, acc);
+			else
+				consume_direct<NW, ZL, FMT>(a, (s & 1) * L.total, L, v0,
+					min(a.vstage, g.na - v0), wave, lane, zc, acc);
 			__syncthreads();
 		}
 		// one write per voxel, [nz nx ny]
@@ -871,10 +970,11 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 	}
 }
 
-// 4 consumer + 4 producer waves: two workgroups share a CU (at most 128
-// VGPRs); 8 + 8: one
+// 4 consumer + 4 producer waves: two workgroups share a CU; 8 + 8: one.
+// Either way 4 waves per SIMD, so at most 128 VGPRs (the second launch
+// bound is the minimum waves per SIMD).
 template <int NW, int NP, int ZL, int FMT, bool FAST>
-__global__ void __launch_bounds__((NW + NP) * 64, NW >= 8 ? 1 : 2)
+__global__ void __launch_bounds__((NW + NP) * 64, 4)
 back_kernel(const BackArgs a)
 {
 	const int nblocks = a.ntiles * a.ngroups;
@@ -892,7 +992,7 @@ back_kernel(const BackArgs a)
 // NW*ZL slices overflows the shared slot takes z-groups half as tall, whose
 // footprints fit, instead of gathering from HBM.
 template <int NW, int NP, int ZL, int FMT, bool FAST>
-__global__ void __launch_bounds__((NW + NP) * 64)
+__global__ void __launch_bounds__((NW + NP) * 64, 4)
 batch_kernel(const BatchArgs b)
 {
 	__shared__ long long next_item;
@@ -981,16 +1081,13 @@ survey_kernel(const BackArgs a)
 		if (code == SETUP_ACTIVE && group == 0)
 			cnt += (unsigned long long) max(c.iz_max - c.iz_min, 0);
 		// the footprint exactly as the backprojection kernel stages it
-		ColRange cr;
+		ColRange cr{};
 		float Tk[NW];
 		const bool act = colvalid
 			&& col_range<NW, ZL, DIRECT>(g, xx, yy, cs, zg0, zg1, cr, Tk);
-		const int s_lo = max(wave_min(act ? cr.c.is + 1 : 0x7fffffff), 0);
-		const int s_hi = min(wave_max(act ? cr.c.is + 2 : (int) 0x80000000), g.ns + 1);
-		const int t_lo = max(wave_min(act ? cr.row_lo : 0x7fffffff), 0);
-		const int t_hi = min(wave_max(act ? cr.row_hi : (int) 0x80000000), g.nt + 1);
-		if (s_lo <= s_hi && t_lo <= t_hi) {
-			const long area = (long) (s_hi - s_lo + (DIRECT ? 0 : 1)) * (t_hi - t_lo + 1);
+		const Footprint f = view_footprint<DIRECT>(g, act, cr);
+		if (f.s_lo <= f.s_hi && f.t_lo <= f.t_hi) {
+			const long area = (long) (f.s_hi - f.s_lo + (DIRECT ? 0 : 1)) * (f.t_hi - f.t_lo + 1);
 			maxarea = max(maxarea, (int) min(area, (long) 0x7fffffff));
 		}
 	}
@@ -1039,31 +1136,6 @@ __global__ void pad_kernel(const T *src, T *dst, int ns, int nt, long total)
 		if (s >= 1 && s <= ns && t >= 1 && t <= nt)
 			val = src[(v * ns + (s - 1)) * nt + (t - 1)];
 		dst[i] = val;
-	}
-}
-
-// FP16/BF16 views as column pairs: word (c, t) of view v holds the padded
-// samples (c, t) in its low and (c + 1, t) in its high half, c in [0, ns],
-// t in [0, nt+1], so one 32-bit read serves both columns of a bilinear
-// interpolation.  [na][ns+1][nt+2] words.
-__global__ void pair_pad_kernel(const uint16_t *src, uint32_t *dst, int ns, int nt, long total)
-{
-	const int pnt = nt + 2, pcols = ns + 1;
-	for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < total;
-		i += (long) gridDim.x * blockDim.x) {
-		const int t = (int) (i % pnt);
-		const long q = i / pnt;
-		const int c = (int) (q % pcols);
-		const long v = q / pcols;
-		uint32_t lo = 0, hi = 0;
-		if (t >= 1 && t <= nt) {
-			const uint16_t *col = src + v * ns * (long) nt + (t - 1);
-			if (c >= 1)          // padded column c = sample column c - 1
-				lo = col[(long) (c - 1) * nt];
-			if (c + 1 <= ns)     // padded column c + 1 = sample column c
-				hi = col[(long) c * nt];
-		}
-		dst[i] = lo | (hi << 16);
 	}
 }
 
@@ -1270,7 +1342,7 @@ size_t back_lds_bytes(int nw, int vstage, int slot, int fmt)
 {
 	const size_t words = (size_t) vstage * slot + (size_t) vstage * 64 * 8
 		+ (fmt == FMT_F32 ? (size_t) vstage * nw * 64 : 0) + (size_t) vstage * 4;
-	return 2 * words * 4;
+	return 2 * words * 4 + 2 * 4;   // two stage buffers and the zero words
 }
 
 #define FDK_DISPATCH_FMT(NW, ZL)                                                               \
@@ -1400,20 +1472,11 @@ hipError_t launch_fill(float *image, long ncol, int nz, int zlo, int zhi, hipStr
 	return hipGetLastError();
 }
 
-hipError_t launch_pad(const void *src, void *dst, int ns, int nt, int na, int fmt,
-	hipStream_t s)
+hipError_t launch_pad(const float *src, float *dst, int ns, int nt, int na, hipStream_t s)
 {
-	if (fmt == FMT_F32) {
-		const long total = (long) na * (ns + 2) * (nt + 2);
-		const int blocks = (int) min((total + 255) / 256, (long) 8192);
-		hipLaunchKernelGGL(pad_kernel<float>, dim3(blocks), dim3(256), 0, s,
-			(const float *) src, (float *) dst, ns, nt, total);
-	} else {
-		const long total = (long) na * (ns + 1) * (nt + 2);
-		const int blocks = (int) min((total + 255) / 256, (long) 8192);
-		hipLaunchKernelGGL(pair_pad_kernel, dim3(blocks), dim3(256), 0, s,
-			(const uint16_t *) src, (uint32_t *) dst, ns, nt, total);
-	}
+	const long total = (long) na * (ns + 2) * (nt + 2);
+	const int blocks = (int) min((total + 255) / 256, (long) 8192);
+	hipLaunchKernelGGL(pad_kernel<float>, dim3(blocks), dim3(256), 0, s, src, dst, ns, nt, total);
 	return hipGetLastError();
 }
 

@@ -417,12 +417,17 @@ def test_feldkamp_weight1_center_value():
     assert w[0, 0, 0] == pytest.approx(600.0 / math.sqrt(1000.0 ** 2 + 4 + 1))
 
 
+def _ell_oracle(cg, ell, oversample=1, zs=None):
+    from oracle.ellipsoid import ellipsoid_proj
+    return ellipsoid_proj(cg.s, cg.t, cg.ar, cg.source_zs if zs is None else zs,
+                          cg.dso, cg.dod, cg.dfs, ell, oversample)
+
+
 def test_ellipsoid_proj_parallel_closed_form():
     from mirt_amd.geom import ct_geom
-    from mirt_amd.phantom import ellipsoid_proj
     cg = ct_geom("fan", ns=41, nt=5, na=3, ds=1.0, dt=1.0, dsd=np.inf)
     ell = np.array([[0, 0, 0, 10, 10, 10, 0, 0, 0.5]])
-    p = ellipsoid_proj(cg, ell).numpy()
+    p = _ell_oracle(cg, ell)
     u = cg.s[:, None]
     v = cg.t[None, :]
     want = 2 * 0.5 * np.sqrt(np.clip(100 - u ** 2 - v ** 2, 0, None))
@@ -430,36 +435,79 @@ def test_ellipsoid_proj_parallel_closed_form():
         assert np.allclose(p[:, :, ia], want, atol=1e-12)
 
 
+def _ray(cg, b, s, t, zs=0.0):
+    """Source position and unit direction of the ray to detector sample (s, t)
+    at angle b: source on +y at b = 0, flat detector at y = -(dsd - dso),
+    arc detector on the circle of radius dsd about the source."""
+    dsd = cg.dsd
+    src = np.array([-cg.dso * math.sin(b), cg.dso * math.cos(b), zs])
+    if cg.dfs == 0:
+        gam = s / dsd
+        d_loc = np.array([dsd * math.sin(gam), -dsd * math.cos(gam), t])
+    else:
+        d_loc = np.array([s, -dsd, t])
+    rot = np.array([[math.cos(b), -math.sin(b), 0], [math.sin(b), math.cos(b), 0], [0, 0, 1]])
+    d = rot @ d_loc
+    return src, d / np.linalg.norm(d)
+
+
 def test_ellipsoid_proj_cone_chords():
     """Arc and flat cone beams: sphere chord from the ray's distance to the centre."""
     from mirt_amd.geom import ct_geom
-    from mirt_amd.phantom import ellipsoid_proj
     for dfs in (0.0, np.inf):
         cg = ct_geom("fan", ns=33, nt=7, na=2, ds=2.0, dt=2.0, dsd=949.0, dso=541.0,
                      dfs=dfs, offset_s=0.25)
         c, r, val = np.array([10.0, -5.0, 3.0]), 40.0, 0.7
         ell = np.array([[*c, r, r, r, 0, 0, val]])
-        p = ellipsoid_proj(cg, ell).numpy()
+        p = _ell_oracle(cg, ell)
         for ia, b in enumerate(cg.ar):
-            src = np.array([-541.0 * math.sin(b), 541.0 * math.cos(b), 0.0])
             for i in range(cg.ns):
                 for j in range(cg.nt):
-                    s, t = cg.s[i], cg.t[j]
-                    if dfs == 0:
-                        gam = s / 949.0
-                        d_loc = np.array([949.0 * math.sin(gam), -949.0 * math.cos(gam), t])
-                    else:
-                        d_loc = np.array([s, -949.0, t])
-                    # rotate the beta = 0 frame (source on +y) by beta
-                    rot = np.array([[math.cos(b), -math.sin(b), 0],
-                                    [math.sin(b), math.cos(b), 0], [0, 0, 1]])
-                    det = src + rot @ d_loc
-                    d = det - src
-                    d /= np.linalg.norm(d)
+                    src, d = _ray(cg, b, cg.s[i], cg.t[j])
                     w = c - src
                     dist2 = w @ w - (w @ d) ** 2
                     chord = 2 * val * math.sqrt(max(r * r - dist2, 0))
                     assert p[i, j, ia] == pytest.approx(chord, abs=1e-9)
+
+
+def test_ellipsoid_proj_rotated_ellipsoid_by_ray_marching():
+    """A rotated, unequal-axis ellipsoid off centre, with a source height:
+    every sample equals the length (times value) of the ray inside the
+    ellipsoid, counted by marching along the ray -- pins the rotation sign,
+    the axis order and the source-height shift of ellipsoid_proj.m:97-118."""
+    from mirt_amd.geom import ct_geom
+    rng = np.random.default_rng(5)
+    cx, cy, cz, rx, ry, rz, ang, val = 12.0, -7.0, 4.0, 35.0, 15.0, 20.0, 30.0, 0.9
+    ell = np.array([[cx, cy, cz, rx, ry, rz, ang, 0, val]])
+    th = math.radians(ang)
+    rot = np.array([[math.cos(th), math.sin(th), 0], [-math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    for dfs in (0.0, np.inf):
+        cg = ct_geom("fan", ns=24, nt=6, na=3, ds=3.0, dt=3.0, dsd=949.0, dso=541.0, dfs=dfs)
+        zs = np.array([0.0, 5.0, -8.0])
+        p = _ell_oracle(cg, ell, zs=zs)
+        step = 0.01
+        lam = np.arange(400.0, 700.0, step)
+        for _ in range(12):
+            ia, i, j = rng.integers(cg.na), rng.integers(cg.ns), rng.integers(cg.nt)
+            src, d = _ray(cg, cg.ar[ia], cg.s[i], cg.t[j], zs[ia])
+            pts = src[None, :] + lam[:, None] * d[None, :] - np.array([cx, cy, cz])
+            q = pts @ rot.T                      # ellipsoid frame
+            inside = (q[:, 0] / rx) ** 2 + (q[:, 1] / ry) ** 2 + (q[:, 2] / rz) ** 2 <= 1
+            assert p[i, j, ia] == pytest.approx(val * inside.sum() * step, abs=3 * step * val)
+
+
+def test_ellipsoid_proj_oversample_is_the_subsample_mean():
+    from mirt_amd.geom import ct_geom
+    from oracle.ellipsoid import ellipsoid_proj
+    cg = ct_geom("fan", ns=9, nt=5, na=2, ds=1.5, dt=1.2, dsd=949.0, dso=541.0, dfs=0.0)
+    ell = np.array([[3, 2, 1, 20, 12, 9, 40, 0, 1.0], [-5, 0, -2, 8, 8, 8, 0, 0, 0.5]])
+    p2 = _ell_oracle(cg, ell, oversample=2)
+    off = np.array([-0.25, 0.25])
+    fine = ellipsoid_proj((off[:, None] * 1.5 + cg.s[None, :]).T.reshape(-1),
+                          (off[:, None] * 1.2 + cg.t[None, :]).T.reshape(-1),
+                          cg.ar, 0.0, cg.dso, cg.dod, cg.dfs, ell)
+    want = fine.reshape(9, 2, 5, 2, 2).mean(axis=(1, 3))
+    assert np.allclose(p2, want, rtol=1e-14, atol=1e-14)
 
 
 # --------------------------------------------------------------------------
