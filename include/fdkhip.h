@@ -382,6 +382,29 @@ int fdkhip_ellipsoid_proj(const double *s, int32_t ns, const double *t, int32_t 
 	double dso, double dod, double dfs, const double *params, int32_t nell,
 	int32_t oversample, double *proj_dev, void *stream);
 
+/*
+ * Allocation-free backprojection.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * fdkhip_plan_create(g, na, beta, mask, fmt, FDKHIP_MATH_FAST) followed by
+ * fdkhip_plan_execute_direct(plan, proj_dev, scale_dev, image_dev, stream)
+ * (bit for bit: the same launch shape, layout and kernels), for F16/BF16
+ * views only (FP32 views need a plan's padded working copy).  The per-view
+ * table, the mask and the set-up counters live in a static device array of
+ * FDKHIP_NOALLOC_MAX_VIEWS * 8 + FDKHIP_NOALLOC_MAX_COLUMNS + 256 bytes
+ * (about 4.3 MiB) that each GPU the library runs on holds until the process
+ * ends; larger scans are refused (FDKHIP_EUNSUPPORTED).  The call copies its
+ * tables and runs the set-up on `stream` and waits for them; the
+ * backprojection itself is asynchronous on `stream`.  Every call on a GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (other
+ * streams, other host threads) race; calls in sequence on one stream do not.
+ */
+#define FDKHIP_NOALLOC_MAX_VIEWS   16384
+#define FDKHIP_NOALLOC_MAX_COLUMNS (2048 * 2048)
+int fdkhip_back_noalloc(const fdkhip_geom *g, int32_t na, const double *beta,
+	const uint8_t *mask, int32_t fmt, const void *proj_dev, const float *scale_dev,
+	float *image_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,8 @@ _SIGS = {
     "fdkhip_ellipsoid_proj": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, C.c_int32,
                                         C.c_double, C.c_double, C.c_double, _P, C.c_int32,
                                         C.c_int32, _P, _P]),
+    "fdkhip_back_noalloc": (C.c_int, [C.POINTER(Geom), C.c_int32, _P, _P, C.c_int32, _P, _P,
+                                      _P, _P]),
 }
 
 _lib = None

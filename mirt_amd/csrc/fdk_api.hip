@@ -559,6 +559,119 @@ extern "C" int fdkhip_plan_execute(fdkhip_plan *p, const void *proj, const float
 }
 
 // ---------------------------------------------------------------------------
+// backprojection of FP16/BF16 views without any device allocation
+
+// Device scratch of fdkhip_back_noalloc: the per-view {cos, sin} table, the
+// mask and the survey's three counters.  A file-scope device array: every
+// GPU the library runs on holds its own copy until the process ends.  Its
+// name is a host-side stand-in in host code; kernels and copies use the
+// device address hipGetSymbolAddress returns (once per device, below).
+constexpr size_t NOALLOC_TRIG = (size_t) FDKHIP_NOALLOC_MAX_VIEWS * sizeof(float2);
+constexpr size_t NOALLOC_MASK = (size_t) FDKHIP_NOALLOC_MAX_COLUMNS;
+constexpr size_t NOALLOC_COUNTERS = 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_TRIG + NOALLOC_MASK
+	+ NOALLOC_COUNTERS];
+
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch[64];   // device address of `scratch`, per device id
+
+// the device address of `scratch` on the current device `dev`, looked up on
+// the first call on that device
+static int scratch_address(int dev, unsigned char **out)
+{
+	if (dev < 0 || dev >= 64)
+		return fail(FDKHIP_EUNSUPPORTED, "device id %d out of range", dev);
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch[dev]) {
+		void *ptr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !ptr)
+			return fail(FDKHIP_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+		g_scratch[dev] = static_cast<unsigned char *>(ptr);
+	}
+	*out = g_scratch[dev];
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_back_noalloc(const fdkhip_geom *g, int32_t na, const double *beta,
+	const uint8_t *mask_host, int32_t fmt, const void *proj_dev, const float *scale_dev,
+	float *image_dev, void *stream)
+{
+	int rc = check_geom(g);
+	if (rc)
+		return rc;
+	if (fmt != FDKHIP_F16 && fmt != FDKHIP_BF16)
+		return fail(FDKHIP_EUNSUPPORTED, "fdkhip_back_noalloc takes F16 or BF16 views "
+			"(FP32 views need the padded working copy of a plan)");
+	if (na < 0)
+		return fail(FDKHIP_EARG, "na must be >= 0 (na=%d)", na);
+	if (na > FDKHIP_NOALLOC_MAX_VIEWS)
+		return fail(FDKHIP_EUNSUPPORTED, "na=%d: at most %d views without allocation", na,
+			FDKHIP_NOALLOC_MAX_VIEWS);
+	if ((long long) g->nx * g->ny > FDKHIP_NOALLOC_MAX_COLUMNS)
+		return fail(FDKHIP_EUNSUPPORTED, "nx*ny=%lld: at most %d columns without allocation",
+			(long long) g->nx * g->ny, FDKHIP_NOALLOC_MAX_COLUMNS);
+	if (g->nz > 65535)
+		return fail(FDKHIP_EUNSUPPORTED, "nz=%d: at most 65535 slices per volume", g->nz);
+	if (na > 0 && !beta)
+		return fail(FDKHIP_EARG, "beta is NULL");
+	const long nvox = (long) g->nx * g->ny * g->nz;
+	if (nvox > 0 && !image_dev)
+		return fail(FDKHIP_EARG, "image is NULL");
+	if (na > 0 && (long) g->ns * g->nt > 0 && (!proj_dev || !scale_dev))
+		return fail(FDKHIP_EARG, "proj and the per-view scale are required");
+	if (nvox == 0)
+		return FDKHIP_OK;
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess)
+		return fail(FDKHIP_EHIP, "no current HIP device");
+	unsigned char *sp = nullptr;
+	rc = scratch_address(dev, &sp);
+	if (rc)
+		return rc;
+	hipStream_t s = (hipStream_t) stream;
+
+	// a plan on the stack whose device tables live in the scratch
+	fdkhip_plan p;
+	p.geom = *g;
+	p.na = na;
+	p.fmt = fmt;
+	p.math = FDKHIP_MATH_FAST;
+	p.gf = narrow_geom(g, na);
+	p.device = dev;
+	int lds = 0;
+	if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)
+		== hipSuccess && lds > 0)
+		p.lds_max = (size_t) lds;
+	p.d_trig = reinterpret_cast<float2 *>(sp);
+	p.d_mask = mask_host ? sp + NOALLOC_TRIG : nullptr;
+	p.d_count = reinterpret_cast<unsigned long long *>(sp + NOALLOC_TRIG + NOALLOC_MASK);
+	p.d_maxarea = reinterpret_cast<int *>(sp + NOALLOC_TRIG + NOALLOC_MASK + 64);
+	p.d_any = reinterpret_cast<int *>(sp + NOALLOC_TRIG + NOALLOC_MASK + 128);
+	p.survey_stream = s;
+	// the tables, ordered after any earlier call's kernels on this stream
+	// that still read them; waited for before the host copies go away
+	const std::vector<float2> trig = na > 0 ? trig_table(beta, na) : std::vector<float2>();
+	if (na > 0)
+		HIP_TRY(hipMemcpyAsync(p.d_trig, trig.data(), sizeof(float2) * na,
+			hipMemcpyHostToDevice, s));
+	if (mask_host)
+		HIP_TRY(hipMemcpyAsync(p.d_mask, mask_host, (size_t) g->nx * g->ny,
+			hipMemcpyHostToDevice, s));
+	HIP_TRY(hipStreamSynchronize(s));
+	// the launch shape and layout a plan of this scan would use
+	support_z(g, mask_host, p.zs_lo, p.zs_hi);
+	rc = configure(&p, p.zs_hi - p.zs_lo <= 64 ? 4 : 8, 16, 0, 0);
+	if (rc == FDKHIP_OK)
+		rc = apply_tuned(&p);
+	if (rc)
+		return rc;
+	if (p.gf.dz / p.gf.dt < 0 && p.fov_any)
+		return fail(FDKHIP_EGEOM, "todo: dz/dt < 0 not done");
+	return plan_launch(&p, proj_dev, scale_dev, image_dev, s);
+}
+
+// ---------------------------------------------------------------------------
 // quantisation
 
 extern "C" int fdkhip_quantize_views(const float *proj, int32_t ns, int32_t nt, int32_t na,
