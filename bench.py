@@ -62,6 +62,17 @@ def host_cores():
         return os.cpu_count() or 1
 
 
+def cpu_threads():
+    """Threads for the CPU reference: every core this process may run on,
+    limited by OMP_NUM_THREADS where the host sets it (its CPU share)."""
+    n = host_cores()
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
 def oracle_geom(cg, ig):
     from oracle import oracle as O
     return O.geom((ig.nx, ig.ny, ig.nz), (ig.dx, ig.dy, ig.dz),
@@ -137,7 +148,7 @@ def cpu_reference_rate(g, tsa_host, beta, mask, nx_ny_nz, target_s=15.0, nthread
     if not O.have_reference("fixed"):
         return None
     if nthread is None:
-        nthread = max(1, min(16, host_cores()))
+        nthread = cpu_threads()
     na = tsa_host.shape[0]
     probe = np.arange(0, na, max(1, na // 4))[:4]
     O.ref_back_ts("fixed", g, tsa_host[probe], beta[probe], mask, nthread=nthread)  # warm-up
@@ -354,8 +365,29 @@ def run_single(args, dev):
         log(f"end to end (host buffers, FP32 EXACT): {e2e:.1f} ms")
         imgs = {k: v.cpu().numpy() for k, v in images.items()}
         result["errors"] = column_errors(g, tsa_host, beta, m, imgs)
+        if not args.no_full_error:
+            result["errors"].update(volume_errors(g, tsa_host, beta, m, imgs, head))
         log(f"cpu baseline + oracle errors in {time.perf_counter() - t0:.1f} s")
+    if args.workload == "clinical_arc" and not args.fmt_only and not args.no_batch_leg:
+        result["uneven_scans"] = batch_leg(args, dev)
     return result
+
+
+def volume_errors(g, tsa_host, beta, mask, images, head):
+    """Relative L2 and max-abs of the headline image over the whole volume
+    against the FP64 oracle on the same FP32 views (multithreaded C)."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    r64 = O.back_ts_f64(g, tsa_host, beta, mask)
+    t = time.perf_counter() - t0
+    out = {"whole_volume_seconds_fp64": t}
+    for name, img in images.items():
+        out[f"relL2_{name}_vs_fp64_whole_volume"] = O.rel_l2(img, r64)
+        out[f"maxabs_{name}_vs_fp64_whole_volume"] = float(np.abs(img - r64).max())
+    out["maxabs_fp64_whole_volume"] = float(np.abs(r64).max())
+    log(f"whole-volume FP64 oracle in {t:.1f} s: {head} relL2 "
+        f"{out[f'relL2_{head}_vs_fp64_whole_volume']:.3e}")
+    return out
 
 
 # per in-range voxel-view update of cbct_back_mat, in MATLAB's expression
@@ -603,7 +635,7 @@ def run_helix(args, dev):
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         if O.have_reference("fixed"):
-            nthread = max(1, min(16, host_cores()))
+            nthread = cpu_threads()
             tsa_host = tsa.cpu().numpy()
             g = oracle_geom(cg, ig)
 
@@ -676,7 +708,35 @@ def slab_times(args, geom, beta, mask, ig, tsa, head, cfg, axis):
             "parts": rows}
 
 
-def run_batch(args, dev):
+def batch_leg(args, dev):
+    """The uneven_scans batch as an extra entry of the default run: all 32
+    scans, FP16 storage, autotuned, graph-replayed; ms, updates, and every
+    scan's error against the FP64 oracle on seeded columns."""
+    import copy
+    a = copy.copy(args)
+    a.workload, a.path, a.nscan, a.launch = "uneven_scans", "f16", 32, None
+    a.steps, a.warmup = min(args.steps, 5), min(args.warmup, 1)
+    a.fmt_only, a.no_cpu_baseline = True, True
+    a.errors_scans, a.error_cols = 32, 128
+    t0 = time.perf_counter()
+    r = run_batch(a, dev, errors_without_baseline=True)
+    errs = r.get("errors", {})
+    worst = max((v["relL2_f16_vs_fp64"] for v in errs.values()), default=None)
+    leg = {"metric": r["metric"], "value": r["value"], "unit": r["unit"],
+           "ms_per_step": r["ms_per_step"], "steps": a.steps, "warmup": a.warmup,
+           "updates_nominal": r["updates_nominal"], "updates_effective": r["updates_effective"],
+           "gups_effective": r["updates_effective"] / (r["ms_per_step"] * 1e-3) / 1e9,
+           "kernel_ms": r["kernel_ms"], "launch": r["config"]["launch"],
+           "roofline": r["roofline"], "scans_checked": len(errs),
+           "worst_relL2_f16_vs_fp64": worst,
+           "columns_per_scan": a.error_cols,
+           "seconds": time.perf_counter() - t0}
+    log(f"uneven_scans leg: {leg['ms_per_step']:.3f} ms/step, worst scan relL2 vs FP64 "
+        f"{worst if worst is None else f'{worst:.2e}'}, {leg['seconds']:.1f} s")
+    return leg
+
+
+def run_batch(args, dev, errors_without_baseline=False):
     import torch
     from mirt_amd import FdkBatch, FdkPlan, geom_of, quantize_views
     from mirt_amd.cbct_back import _views
@@ -690,7 +750,7 @@ def run_batch(args, dev):
     for sc in scans:
         tsa = filtered_views(sc, dev)
         if args.errors_scans > len(hosts):
-            hosts.append(tsa.cpu().numpy())
+            hosts.append(tsa.to("cpu", non_blocking=False).numpy())
         a = prepare(tsa, fmt)
         projs.append(a[0])
         scales.append(a[1] if len(a) > 1 else None)
@@ -764,17 +824,19 @@ def run_batch(args, dev):
                              "from the rocprofv3 summary in profiles/"},
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline and hosts:
+    if (not args.no_cpu_baseline or errors_without_baseline) and hosts:
         errs = {}
         for i, h in enumerate(hosts):
             sc = scans[i]
             g = oracle_geom(sc.cg, sc.ig)
             m = np.ascontiguousarray(np.asarray(masks[i]).T).astype(np.uint8)
-            e = column_errors(g, h, betas[i], m, {head: outs[i].cpu().numpy()}, ncol=256)
+            e = column_errors(g, h, betas[i], m, {head: outs[i].cpu().numpy()},
+                              ncol=args.error_cols)
             errs[sc.name] = {k: v for k, v in e.items() if "fp64" in k}
-            if i == 0:
+            if i == 0 and not args.no_cpu_baseline:
                 result["cpu_baseline"] = cpu_reference_rate(
                     g, h, betas[i], m, (sc.ig.nx, sc.ig.ny, sc.ig.nz), target_s=10.0)
+            hosts[i] = None
         result["errors"] = errs
     return result
 
@@ -796,7 +858,13 @@ def main():
                     help="also time the G z-slabs a G-GPU split would give each GPU")
     ap.add_argument("--errors-scans", type=int, default=4,
                     help="uneven_scans: scans checked against the FP64 oracle")
+    ap.add_argument("--error-cols", type=int, default=256,
+                    help="uneven_scans: seeded columns per scan checked against FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-full-error", action="store_true",
+                    help="skip the whole-volume FP64 comparison of the headline")
+    ap.add_argument("--no-batch-leg", action="store_true",
+                    help="clinical_arc: skip the uneven_scans batch entry")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB three times (torch copy kernel): the reference "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
