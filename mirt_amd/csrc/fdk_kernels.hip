@@ -241,11 +241,12 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 //   U[k] = U0 + k * ZL * t_inc                          (chunk k's first slice)
 //   u    = U[k] + j * t_inc                             (slice zg0 + k*ZL + j)
 // each step one fmaf (one rounding each; nearer the exact value than the
-// reference's running sum).  The range [zlo, zhi) is every slice of the
-// group with 0 < u < nt + 1, found with that same arithmetic (u grows with
-// z: the first guess from iz_min/iz_max is moved both ways until it is
-// exact, so every slice outside the range has u <= 0 or u >= nt + 1);
-// first/last are u at zlo and zhi - 1.  Uk[0] receives U0.
+// reference's running sum).  [zlo, zhi) is [iz_min - 1, iz_max + 1) within
+// the group: column_setup_fast's bounds are within one slice of where u
+// leaves (0, nt + 1), so the range holds every slice with 0 < u < nt + 1,
+// and u <= 0 below it and u >= nt + 1 above it; the few slices it holds
+// outside (0, nt + 1) read zero rows with weight 0 under the consumer's
+// clamp (ColRange).  first/last are u at zlo and zhi - 1.  Uk[0] receives U0.
 template <int NW, int ZL>
 __device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
 	int zg0, int zg1, float (&Uk)[NW], int &zlo, int &zhi, float &first, float &last)
@@ -253,21 +254,12 @@ __device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
 	const float ti = c.t_inc;
 	const float U0 = fmaf((float) (zg0 - c.iz_min), ti, c.t_bin + 1.f);
 	Uk[0] = U0;
-	auto uz = [&](int z) {
-		const int j0 = ((z - zg0) / ZL) * ZL;
-		return fmaf((float) (z - zg0 - j0), ti, fmaf((float) j0, ti, U0));
+	auto uz = [&](int z) {           // z in [zg0, zg1)
+		const unsigned k = (unsigned) (z - zg0), j0 = k & ~(unsigned) (ZL - 1);
+		return fmaf((float) (k - j0), ti, fmaf((float) j0, ti, U0));
 	};
-	const float uhi = (float) g.nt + 1.f;
-	int lo = min(max(c.iz_min - 1, zg0), zg1);
-	while (lo > zg0 && uz(lo - 1) > 0.f)
-		--lo;
-	while (lo < zg1 && uz(lo) <= 0.f)
-		++lo;
-	int hi = max(min(c.iz_max + 1, zg1), lo);
-	while (hi < zg1 && uz(hi) < uhi)
-		++hi;
-	while (hi > lo && uz(hi - 1) >= uhi)
-		--hi;
+	const int lo = min(max(c.iz_min - 1, zg0), zg1);
+	const int hi = max(min(c.iz_max + 1, zg1), lo);
 	zlo = lo;
 	zhi = hi;
 	first = lo < hi ? uz(lo) : 0.f;
@@ -286,8 +278,8 @@ __device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
 // zero rows (row nt + 2 is one more zero row of the staged footprint).  So
 // the rows read include 0 when the range starts inside the group and
 // nt + 2 when it ends inside it.  Where the range reaches the group's end,
-// u_hi = u at zhi - 1 keeps the (unwritten) slices past the volume's last
-// slice inside the footprint.
+// u_hi = u at zhi - 1 (within [0, nt + 1]) keeps the (unwritten) slices past
+// the volume's last slice inside the footprint.
 struct ColRange {
 	ColSetup c;
 	int zlo, zhi;
@@ -308,10 +300,12 @@ __device__ __forceinline__ bool col_range(const GeomF &g, float xx, float yy, fl
 			return false;
 		if (!setup_direct<NW, ZL>(g, r.c, zg0, zg1, Tk, r.zlo, r.zhi, r.first, r.last))
 			return false;
-		const bool cut_hi = r.zhi < zg1;
-		r.row_lo = (r.zlo > zg0) ? 0 : (int) r.first;         // u > 0: truncation is floor
-		r.row_hi = cut_hi ? g.nt + 2 : (int) r.last + 1;
-		r.u_hi = cut_hi ? (float) g.nt + 1.f : r.last;
+		// every slice the consumer computes reads rows floor(uc), floor(uc) + 1
+		// of uc = u clamped to [0, u_hi] (truncation is floor for uc >= 0)
+		const float ntp1 = (float) g.nt + 1.f;
+		r.u_hi = (r.zhi < zg1) ? ntp1 : fmaxf(fminf(r.last, ntp1), 0.f);
+		r.row_lo = (r.zlo > zg0) ? 0 : (int) fminf(fmaxf(r.first, 0.f), r.u_hi);
+		r.row_hi = (int) r.u_hi + 1;
 	} else {
 		if (column_setup(g, xx, yy, cs.x, cs.y, r.c) != SETUP_ACTIVE)
 			return false;
@@ -434,8 +428,10 @@ __device__ __forceinline__ void stage_word(const char *src, float *ldst)
 // [ns][nt] 16-bit view and zero outside [1, ns] x [1, nt] (the zero border
 // that fdk_ts_put_view gives each view, fdk,ts,t.c:13-21).  Lane = detector
 // row: the wave walks the columns keeping the previous column's sample, with
-// eight 2-byte loads in flight (64 consecutive samples of a column per load,
-// one 128-byte line).  Each sample is read from HBM once per footprint.
+// sixteen 2-byte loads in flight (64 consecutive samples of a column per
+// load, one 128-byte line).  Each sample is read from HBM once per footprint.
+// Columns are tw words apart (the caller makes tw odd, so that lanes reading
+// the same row of neighbouring columns fall in different LDS banks).
 __device__ __forceinline__ void stage_pairs(const uint16_t *view, const GeomF &g,
 	const Footprint &f, int sw, int tw, int slot, int lane)
 {
@@ -456,17 +452,17 @@ __device__ __forceinline__ void stage_pairs(const uint16_t *view, const GeomF &g
 		uint32_t prev = 0u;                    // padded column s_lo (0: the zero pad)
 		if (f.s_lo >= 1)
 			prev = view[(long) (f.s_lo - 1) * g.nt + srow];
-		for (int k0 = 0; k0 < sw; k0 += 8) {
-			uint32_t cur[8];
+		for (int k0 = 0; k0 < sw; k0 += 16) {
+			uint32_t cur[16];
 #pragma unroll
-			for (int q = 0; q < 8; ++q) {
+			for (int q = 0; q < 16; ++q) {
 				// padded column s_lo + k0 + q + 1 is sample column s_lo + k0 + q;
 				// clamped so that the loads past the footprint stay in the view
 				const int sc = min(f.s_lo + k0 + q, g.ns - 1);
 				cur[q] = view[(long) sc * g.nt + srow];
 			}
 #pragma unroll
-			for (int q = 0; q < 8; ++q) {
+			for (int q = 0; q < 16; ++q) {
 				const int k = k0 + q;
 				const uint32_t hi = (f.s_lo + k + 1 <= g.ns) ? cur[q] : 0u;
 				if (k < sw && rv)
@@ -502,7 +498,8 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 		const Footprint f = view_footprint<DIRECT>(g, act, cr);
 		int mode = MODE_NONE, tw = 0, sw = 0;
 		if (f.s_lo <= f.s_hi && f.t_lo <= f.t_hi) {
-			tw = f.t_hi - f.t_lo + 1;
+			// DIRECT: an odd column stride (bank spread, stage_pairs)
+			tw = DIRECT ? ((f.t_hi - f.t_lo + 1) | 1) : f.t_hi - f.t_lo + 1;
 			sw = f.s_hi - f.s_lo + (DIRECT ? 0 : 1);     // staged columns (DIRECT: pairs)
 			mode = ((long) sw * tw <= a.slot) ? MODE_LDS : MODE_GLOBAL;
 		}
@@ -1087,7 +1084,8 @@ survey_kernel(const BackArgs a)
 			&& col_range<NW, ZL, DIRECT>(g, xx, yy, cs, zg0, zg1, cr, Tk);
 		const Footprint f = view_footprint<DIRECT>(g, act, cr);
 		if (f.s_lo <= f.s_hi && f.t_lo <= f.t_hi) {
-			const long area = (long) (f.s_hi - f.s_lo + (DIRECT ? 0 : 1)) * (f.t_hi - f.t_lo + 1);
+			const int rows = f.t_hi - f.t_lo + 1;
+			const long area = (long) (f.s_hi - f.s_lo + (DIRECT ? 0 : 1)) * (DIRECT ? rows | 1 : rows);
 			maxarea = max(maxarea, (int) min(area, (long) 0x7fffffff));
 		}
 	}

@@ -17,8 +17,12 @@ rest of the volume is held to size-independent properties:
 * additivity: back(p + q) vs back(p) + back(q) within relative L2 1e-5
   (FAST FP32) over the whole volume -- FP32 rounding of ~1000-view sums of
   white noise is ~sqrt(na) * 2^-24 ~ 2e-6, a dropped or doubled view ~3e-2;
-* batches of uneven scans (first 8 of the seeded 32): every scan's sampled
-  columns bit-for-bit equal to the oracle in EXACT mode.
+* batches of uneven scans (all 32 seeded scans of the batched configuration,
+  one persistent launch replayed from a HIP graph): in EXACT mode every
+  scan's sampled columns bit-for-bit equal to the oracle; with FP16 storage,
+  autotuned, every scan within 1e-3 of the FP64 oracle on sampled columns;
+* whole volume at ``clinical_arc``: FP16 and BF16 storage of the filtered
+  phantom views within 1e-3 of the FP64 oracle over every voxel.
 """
 from __future__ import annotations
 
@@ -141,20 +145,85 @@ def test_fullsize_additivity(device, name):
 
 
 def test_fullsize_uneven_batch(device):
+    """All 32 uneven scans in one EXACT batch (graph replay, twice):
+    sampled columns of every scan bit for bit equal to the oracle."""
+    import torch
     from mirt_amd import FdkBatch, geom_of
     from mirt_amd.cbct_back import _views
     from mirt_amd.workloads import uneven_scans
-    scans = uneven_scans(32)[:8]
+    scans = uneven_scans(32)
     print("uneven batch: " + ", ".join(
         f"{s.ig.nx}^2x{s.ig.nz}/{s.cg.na}v/{s.cg.ns}x{s.cg.nt}" for s in scans), flush=True)
     betas = [_views(s.cg, s.cg.na) for s in scans]
     batch = FdkBatch([(geom_of(s.cg, s.ig), b, None) for s, b in zip(scans, betas)])
     projs = [_noise_views(s.cg, device, seed=100 + i) for i, s in enumerate(scans)]
     outs = batch(projs)
+    first = [o.clone() for o in outs]
+    batch(projs, outs=outs)                    # second replay of the same graph
+    torch.cuda.synchronize()
     for i, s in enumerate(scans):
+        assert torch.equal(outs[i], first[i]), i
         g = _oracle_geom(s.cg, s.ig)
         cols = _cols(s.ig, 24, seed=i)
         mask = np.ones((s.ig.ny, s.ig.nx), np.uint8)
         want = O.back_ts_f32(g, projs[i].cpu().numpy(), betas[i], mask, cols=cols)
         got = outs[i].cpu().numpy().reshape(-1, s.ig.nz)[cols]
         assert np.array_equal(got, want), (i, O.rel_l2(got, want))
+
+
+def test_fullsize_uneven_batch_f16(device):
+    """The batched configuration at its specification: 32 seeded uneven scans
+    of FDK-filtered phantom views, FP16 storage with per-view scale, one
+    autotuned persistent launch replayed from a HIP graph; every scan within
+    1e-3 of the FP64 oracle on 64 sampled columns (all slices, all views)."""
+    import torch
+    from mirt_amd import FdkBatch, geom_of, quantize_views
+    from mirt_amd.cbct_back import _views
+    from mirt_amd.workloads import filtered_views, uneven_scans
+    scans = uneven_scans(32)
+    betas = [_views(s.cg, s.cg.na) for s in scans]
+    views, projs, scales = [], [], []
+    for s in scans:
+        tsa = filtered_views(s, device)
+        q, sc = quantize_views(tsa, "f16")
+        views.append(tsa)
+        projs.append(q)
+        scales.append(sc)
+    batch = FdkBatch([(geom_of(s.cg, s.ig), b, s.ig.mask_or) for s, b in zip(scans, betas)],
+                     fmt="f16")
+    outs = [torch.empty((s.ig.ny, s.ig.nx, s.ig.nz), device=device) for s in scans]
+    ms = batch.autotune(projs, scales, outs, reps=1)
+    print(f"uneven f16 batch autotuned: {batch.config}, {ms:.2f} ms", flush=True)
+    batch(projs, scales, outs)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, s in enumerate(scans):
+        g = _oracle_geom(s.cg, s.ig)
+        cols = _cols(s.ig, 64, seed=40 + i)
+        mask = np.ascontiguousarray(np.asarray(s.ig.mask_or).T).astype(np.uint8)
+        r64 = O.back_ts_f64(g, views[i].cpu().numpy(), betas[i], mask, cols=cols)
+        err = O.rel_l2(outs[i].cpu().numpy().reshape(-1, s.ig.nz)[cols], r64)
+        worst = max(worst, err)
+        assert err <= 1e-3, (i, err)
+    print(f"  worst scan relL2 vs FP64: {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_fullsize_whole_volume_vs_fp64(device, fmt):
+    """clinical_arc, FDK-filtered phantom views stored FP16 or BF16 with a
+    per-view scale: relative L2 over the whole 512 x 512 x 256 volume against
+    the FP64 oracle <= 1e-3 (FP64 on every column, multithreaded)."""
+    from mirt_amd import FdkPlan, geom_of, quantize_views
+    from mirt_amd.cbct_back import _views
+    from mirt_amd.workloads import WORKLOADS, filtered_views
+    scan = WORKLOADS["clinical_arc"]()
+    cg, ig = scan.cg, scan.ig
+    beta = _views(cg, cg.na)
+    tsa = filtered_views(scan, device)
+    q, s = quantize_views(tsa, fmt)
+    img = FdkPlan(geom_of(cg, ig), beta, ig.mask_or, fmt=fmt)(q, s).cpu().numpy()
+    mask = np.ascontiguousarray(np.asarray(ig.mask_or).T).astype(np.uint8)
+    r64 = O.back_ts_f64(_oracle_geom(cg, ig), tsa.cpu().numpy(), beta, mask)
+    err = O.rel_l2(img, r64)
+    print(f"clinical_arc whole volume {fmt} vs FP64: {err:.3e}", flush=True)
+    assert err <= 1e-3, err
