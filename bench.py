@@ -212,9 +212,11 @@ def column_errors(g, tsa_host, beta, mask, images, ncol=1024, seed=7):
 
 def filter_times(args, cg, dev):
     """The FDK weight + ramp filter before the backprojection, on synthetic
-    FP32 views of the workload's size: fdkhip_fdk_filter_views (weight kernel
-    + SGEMM with the Toeplitz matrix + transpose) against torch's FP32
-    FFT route (feldkamp_weight1 + fdk_filter + the permute to [nt ns na])."""
+    FP32 views of the workload's size: fdkhip_fdk_filter_views (one fused
+    kernel: weight, FFT of the zero-padded rows in LDS, response, inverse
+    FFT, transpose) against torch's FP32 FFT route (feldkamp_weight1 +
+    fdk_filter + the permute to [nt ns na]).  Bound: HBM, one read of the
+    views and one write of the result."""
     import torch
     from mirt_amd.fdk import fdk_filter, feldkamp_weight1, weight_filter_views
     ns, nt, na = cg.ns, cg.nt, cg.na
@@ -225,14 +227,16 @@ def filter_times(args, cg, dev):
     p = sta.permute(2, 1, 0)                                   # (ns, nt, na) view
     fft = time_steps(lambda: fdk_filter(feldkamp_weight1(p, cg), cg).permute(2, 0, 1)
                      .contiguous(), args.steps, args.warmup)
-    flop = 2.0 * ns * ns * nt * na
-    tflops = flop / (native * 1e-3) / 1e12
-    log(f"fdk filter: {native:.3f} ms native (SGEMM {tflops:.1f} TFLOP/s), "
+    nbytes = 8.0 * ns * nt * na
+    gbs = nbytes / (native * 1e-3) / 1e9
+    log(f"fdk filter: {native:.3f} ms fused kernel ({gbs:.0f} GB/s), "
         f"{fft:.3f} ms torch FP32 FFT")
-    return {"ms": native, "path": "fdkhip_fdk_filter_views", "torch_fft_f32_ms": fft,
-            "gemm_tflops": tflops, "fp32_matrix_peak_tflops": FP32_PEAK_TFLOPS,
-            "note": f"{ns}x{nt}x{na} FP32 views, weight + Toeplitz SGEMM + transpose "
-                    f"(2*ns^2*nt*na = {flop:.3g} flop); includes output allocation"}
+    return {"ms": native, "path": "fdkhip_fdk_filter_views (fused weight + FFT filter + "
+                                  "transpose)", "torch_fft_f32_ms": fft,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                         "frac": gbs / 8000.0, "algorithmic_bytes": nbytes},
+            "note": f"{ns}x{nt}x{na} FP32 views; algorithmic bytes = 4 B read + 4 B written "
+                    "per sample; time includes the output allocation"}
 
 
 def run_single(args, dev):

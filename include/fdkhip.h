@@ -343,25 +343,23 @@ int fdkhip_quantize_views(const float *proj_dev, int32_t ns, int32_t nt,
 	int32_t na, int32_t fmt, void *out_dev, float *scale_dev, void *stream);
 
 /*
- * The FDK steps before the backprojection, on the device: the cosine
+ * The FDK steps before cbct_back on device-resident views: the cosine
  * pre-weight of feldkamp.m:103-132 (feldkamp_weight1) and the zero-padded
- * ramp filter along s of fdk_filter.m:1-63, in FP32.  proj is MATLAB's
- * [ns nt na] (s fastest); ww1 is the [ns nt] weight table; toeplitz is the
- * filter as an ns x ns matrix, toeplitz[s + ns*s'] = h(s' - s), h the
- * filter's spatial taps (the inverse DFT of fdk_filter's response, lags
- * -(ns-1)..ns-1: the zero padding to 2^ceil(log2(2ns-1)) makes the padded
- * circular convolution exactly this product).  work is scratch of ns*nt*na
- * floats (may be proj itself, which is then overwritten); tsa receives the
- * filtered views as [nt ns na] (t fastest), the layout fdkhip_plan_execute
- * takes.  The weighted views go through one rocBLAS SGEMM (every row of
- * every view against the Toeplitz matrix) and a tiled transpose.  All
- * pointers are device pointers; tsa must not alias proj or work.
- * Asynchronous on `stream`.  The library keeps one rocBLAS handle per device
- * for the life of the process; calls from several threads are serialised
- * around it.
+ * ramp filter along s of fdk_filter.m:1-63, in FP32, fused into one kernel.
+ * proj is MATLAB's [ns nt na] (s fastest); ww1 is the [ns nt] weight table;
+ * response holds the filter's npad-point frequency response (real and even:
+ * fdk_filter's H times ds, with H[k] = H[npad - k]), npad the padded row
+ * length, a power of two from max(ns, 2) to 4096 (fdk_filter's is
+ * 2^ceil(log2(2 ns - 1))); tsa receives the filtered views as [nt ns na]
+ * (t fastest), the layout fdkhip_plan_execute takes.  Each row is weighted,
+ * transformed by an FFT of npad points in LDS, multiplied by the response,
+ * transformed back and cropped to ns samples.  All pointers are device
+ * pointers; tsa must not alias proj.  Asynchronous on `stream`.  The
+ * library keeps one table of npad twiddles per device and length for the
+ * life of the process (built on the first call with that length).
  */
 int fdkhip_fdk_filter_views(const float *proj_dev, int32_t ns, int32_t nt,
-	int32_t na, const float *ww1_dev, const float *toeplitz_dev, float *work_dev,
+	int32_t na, const float *ww1_dev, const float *response_dev, int32_t npad,
 	float *tsa_dev, void *stream);
 
 /*

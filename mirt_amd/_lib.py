@@ -93,8 +93,8 @@ _SIGS = {
     "fdkhip_batch_config": (C.c_int, [_P, _P]),
     "fdkhip_batch_set_config": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
     "fdkhip_batch_set_zper": (C.c_int, [_P, _P]),
-    "fdkhip_fdk_filter_views": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P,
-                                          _P]),
+    "fdkhip_fdk_filter_views": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P,
+                                          C.c_int32, _P, _P]),
     "fdkhip_batch_scan_zper": (C.c_int, [_P, _P]),
     "fdkhip_batch_autotune": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32,
                                         C.POINTER(C.c_float)]),

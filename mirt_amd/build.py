@@ -22,7 +22,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libfdkhip.so")
 
 SOURCES = ["fdk_kernels.hip", "mat_kernel.hip", "helix_kernel.hip", "phantom_kernel.hip",
-           "fdk_api.hip"]
+           "filter_kernel.hip", "fdk_api.hip"]
 ARCH = os.environ.get("MIRT_AMD_ARCH", "gfx950")
 
 
@@ -60,10 +60,7 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
         objs.append(obj)
     tmp = LIB + ".tmp"
-    rocm = os.path.dirname(os.path.dirname(os.path.realpath(cc)))
-    # rocBLAS: the SGEMM of fdkhip_fdk_filter_views
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + [
-        f"-L{rocm}/lib", f"-Wl,-rpath,{rocm}/lib", "-lrocblas"]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

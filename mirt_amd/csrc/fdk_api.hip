@@ -7,7 +7,6 @@
 // the same libm so that the device sees the reference's exact values.
 
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <math.h>
 #include <stdarg.h>
@@ -692,85 +691,6 @@ extern "C" int fdkhip_quantize_views(const float *proj, int32_t ns, int32_t nt, 
 	(void) hipFreeAsync(maxbits, s);
 	if (e != hipSuccess)
 		return fail(FDKHIP_EHIP, "quantize: %s", hipGetErrorString(e));
-	return FDKHIP_OK;
-}
-
-// ---------------------------------------------------------------------------
-// FDK pre-weight and ramp filter on the device (feldkamp.m:103-132,
-// fdk_filter.m:1-63): work = proj .* ww1 per view, then each view's rows
-// filtered along s as one product with the filter's Toeplitz matrix
-
-// work[v][t][s] = proj[v][t][s] * ww1[t][s]; grid.x over ns*nt, grid.y strides views
-__global__ void weight_views_kernel(const float *__restrict__ proj,
-	const float *__restrict__ ww1, float *work, int nst, int na)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= nst)
-		return;
-	const float w = ww1[i];
-	for (int v = blockIdx.y; v < na; v += gridDim.y) {
-		const size_t k = (size_t) v * nst + i;
-		work[k] = proj[k] * w;
-	}
-}
-
-static std::mutex g_blas_mutex;
-static rocblas_handle g_blas[64];
-
-static int blas_handle(rocblas_handle *h)
-{
-	int dev = 0;
-	HIP_TRY(hipGetDevice(&dev));
-	if (dev < 0 || dev >= 64)
-		return fail(FDKHIP_EUNSUPPORTED, "device index %d out of range", dev);
-	std::lock_guard<std::mutex> lock(g_blas_mutex);
-	if (!g_blas[dev] && rocblas_create_handle(&g_blas[dev]) != rocblas_status_success) {
-		g_blas[dev] = nullptr;
-		return fail(FDKHIP_EHIP, "rocblas_create_handle failed");
-	}
-	*h = g_blas[dev];
-	return FDKHIP_OK;
-}
-
-extern "C" int fdkhip_fdk_filter_views(const float *proj, int32_t ns, int32_t nt, int32_t na,
-	const float *ww1, const float *toeplitz, float *work, float *tsa, void *stream)
-{
-	if (ns < 0 || nt < 0 || na < 0)
-		return fail(FDKHIP_EARG, "negative size");
-	if (na == 0 || (long) ns * nt == 0)
-		return FDKHIP_OK;
-	if (!proj || !ww1 || !toeplitz || !work || !tsa)
-		return fail(FDKHIP_EARG, "NULL device pointer");
-	if ((long) ns * nt > 0x7fffffffL / 64 || (long) nt * na > 0x7fffffffL)
-		return fail(FDKHIP_EARG, "views of %d x %d x %d samples too large", ns, nt, na);
-	if (work == tsa || proj == tsa)
-		return fail(FDKHIP_EARG, "tsa must not alias proj or work");
-	hipStream_t s = (hipStream_t) stream;
-	const int nst = ns * nt;
-	// 1. tsa (as scratch, [ns nt na]) = proj .* ww1
-	dim3 grid((nst + 255) / 256, std::min(na, 1024));
-	hipLaunchKernelGGL(weight_views_kernel, grid, dim3(256), 0, s, proj, ww1, tsa, nst, na);
-	HIP_TRY(hipGetLastError());
-	rocblas_handle h = nullptr;
-	int rc = blas_handle(&h);
-	if (rc)
-		return rc;
-	{
-		std::lock_guard<std::mutex> lock(g_blas_mutex);
-		if (rocblas_set_stream(h, s) != rocblas_status_success)
-			return fail(FDKHIP_EHIP, "rocblas_set_stream failed");
-		// 2. every row at once: work (ns x nt*na, s fastest) = Toeplitzᵀ (ns x ns) * tsa,
-		// where the toeplitz buffer read column-major is M[s, s'] = h(s' - s)
-		const float one = 1.f, zero = 0.f;
-		const rocblas_status st = rocblas_sgemm(h, rocblas_operation_transpose,
-			rocblas_operation_none, ns, nt * na, ns, &one, toeplitz, ns, tsa, ns, &zero,
-			work, ns);
-		if (st != rocblas_status_success)
-			return fail(FDKHIP_EHIP, "rocblas_sgemm: status %d", (int) st);
-	}
-	// 3. [ns nt na] -> [nt ns na]
-	HIP_TRY(launch_transpose(work, tsa, na, nt, ns, (long) nst, ns, (long) nst, nt, nt, 0, 1.f,
-		s));
 	return FDKHIP_OK;
 }
 

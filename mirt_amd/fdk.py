@@ -7,8 +7,8 @@
   along s, ``ifft(...,'symmetric')`` taken as the real inverse transform of
   the non-negative frequencies);
 * :func:`weight_filter_views` -- the weight and the filter on the device
-  through ``fdkhip_fdk_filter_views`` (a HIP weight kernel, one rocBLAS SGEMM
-  with the filter's Toeplitz matrix over every row, a tiled transpose), FP32,
+  through ``fdkhip_fdk_filter_views`` (one fused HIP kernel: weight, FFT of
+  the zero-padded rows in LDS, the response, inverse FFT, transpose), FP32,
   producing the ``[nt ns na]`` views the plans take;
 * :func:`feldkamp` -- fbp/feldkamp.m:1-98: weight, filter, backproject.
 
@@ -124,11 +124,22 @@ def filter_taps(cg: CTGeom, ns: int, window: str = "ramp") -> np.ndarray:
 
 
 def toeplitz_matrix(cg: CTGeom, ns: int, window: str = "ramp") -> np.ndarray:
-    """The filter as the FP64 C-order ``(ns, ns)`` array ``T[s', s] = tap(s' - s)``
-    (in memory: ``T[s + ns*s']``, the layout fdkhip_fdk_filter_views takes)."""
+    """The filter as the FP64 C-order ``(ns, ns)`` array ``T[s', s] = tap(s' - s)``:
+    row filtering as a matrix product (a direct-convolution form of
+    fdk_filter, used to check it)."""
     taps = filter_taps(cg, ns, window)
     k = np.arange(ns)
     return taps[(k[:, None] - k[None, :]) + ns - 1]
+
+
+def padded_response(cg: CTGeom, ns: int, window: str = "ramp"):
+    """(npad, H): fdk_filter's padded length 2^ceil(log2(2 ns - 1)) and its
+    response times ds on all npad frequencies, H[k] = H[npad - k] as
+    ``irfft`` extends the non-negative half (fdk_filter.m:21-35)."""
+    npadh = 2 ** math.ceil(math.log2(2 * ns - 1))
+    H = fdk_filter_response(cg, npadh, window)
+    k = np.arange(npadh)
+    return npadh, H[np.minimum(k, npadh - k) % npadh]
 
 
 _FILTER_TABLES = {}
@@ -141,9 +152,10 @@ def _filter_tables(cg: CTGeom, ns, nt, window, w1cyl, device):
     if key not in _FILTER_TABLES:
         ww1 = torch.as_tensor(weight_table(cg, ns, nt, w1cyl=w1cyl).T.copy(),  # [ns nt], s fastest
                               dtype=torch.float32, device=device)
-        T = torch.as_tensor(toeplitz_matrix(cg, ns, window), dtype=torch.float32, device=device)
+        npad, H = padded_response(cg, ns, window)
+        Hd = torch.as_tensor(H, dtype=torch.float32, device=device)
         _FILTER_TABLES.clear()
-        _FILTER_TABLES[key] = (ww1, T)
+        _FILTER_TABLES[key] = (ww1, npad, Hd)
     return _FILTER_TABLES[key]
 
 
@@ -163,12 +175,12 @@ def weight_filter_views(proj, cg: CTGeom, *, window: str = "ramp", w1cyl: bool =
         raise ValueError("proj must be a contiguous float32 device tensor (na, nt, ns)")
     na, nt, ns = proj.shape
     lib = _lib.load()
-    ww1, T = _filter_tables(cg, ns, nt, window, w1cyl, proj.device)
-    work = torch.empty_like(proj)
-    tsa = torch.empty((na, ns, nt), dtype=torch.float32, device=proj.device)
+    ww1, npad, H = _filter_tables(cg, ns, nt, window, w1cyl, proj.device)
     s = stream if stream is not None else torch.cuda.current_stream(proj.device)
+    with torch.cuda.stream(s):      # the output is allocated on the stream it is written on
+        tsa = torch.empty((na, ns, nt), dtype=torch.float32, device=proj.device)
     P = lambda t: C.c_void_p(t.data_ptr())
-    _lib.check(lib.fdkhip_fdk_filter_views(P(proj), ns, nt, na, P(ww1), P(T), P(work),
+    _lib.check(lib.fdkhip_fdk_filter_views(P(proj), ns, nt, na, P(ww1), P(H), npad,
                                            P(tsa), C.c_void_p(s.cuda_stream)))
     return tsa
 

@@ -394,9 +394,10 @@ def test_fdk_filter_matches_direct_convolution():
 @pytest.mark.parametrize("ns", [37, 64])
 @pytest.mark.parametrize("window", ["ramp", "hann"])
 def test_toeplitz_filter_matches_fdk_filter(ns, window):
-    """The Toeplitz matrix handed to fdkhip_fdk_filter_views (taps = inverse
-    DFT of fdk_filter's response) filters rows exactly as fdk_filter's
-    zero-padded FFT does, for arc and flat detectors."""
+    """The Toeplitz matrix of the filter's taps (inverse DFT of fdk_filter's
+    response: the direct-convolution form the GPU filter is checked against)
+    filters rows exactly as fdk_filter's zero-padded FFT does, for arc and
+    flat detectors."""
     from mirt_amd.fdk import fdk_filter, toeplitz_matrix
     from mirt_amd.geom import ct_geom
     for dfs in (0.0, np.inf):
@@ -406,6 +407,91 @@ def test_toeplitz_filter_matches_fdk_filter(ns, window):
         T = toeplitz_matrix(cg, ns, window)                 # T[s', s]
         got = np.einsum("ab,btv->atv", T, p)
         assert np.allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("ns", [2, 5, 37, 64, 888])
+@pytest.mark.parametrize("window", ["ramp", "hann"])
+def test_padded_response_is_fdk_filters_response(ns, window):
+    """padded_response (the full-length H that fdkhip_fdk_filter_views takes)
+    filters a zero-padded row by a complex FFT exactly as fdk_filter's
+    real FFT with the non-negative half of the response does."""
+    from mirt_amd.fdk import fdk_filter, padded_response
+    from mirt_amd.geom import ct_geom
+    for dfs in (0.0, np.inf):
+        cg = ct_geom("fan", ns=ns, nt=2, na=1, ds=0.9, dt=1.0, dsd=949.0, dso=541.0, dfs=dfs)
+        p = np.random.default_rng(ns).standard_normal((ns, 2, 1))
+        npad, H = padded_response(cg, ns, window)
+        assert npad == 2 ** math.ceil(math.log2(2 * ns - 1))
+        assert np.array_equal(H[1:], H[1:][::-1])               # real and even
+        got = np.real(np.fft.ifft(H[:, None, None] * np.fft.fft(p, n=npad, axis=0),
+                                  axis=0))[:ns]
+        want = fdk_filter(torch.as_tensor(p), cg, window).numpy()
+        assert np.allclose(got, want, rtol=0, atol=1e-12 * max(np.abs(want).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def fft_core_host(tmp_path_factory):
+    """tests/fft_core_host.cpp: the FFT passes of mirt_amd/csrc/fft_core.h
+    (the filter kernel's own index and butterfly code) built for the host."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    out = str(tmp_path_factory.mktemp("fft") / "libfftcore.so")
+    subprocess.run([cxx, "-O1", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off",
+                    "-I" + os.path.join(ROOT, "mirt_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "fft_core_host.cpp"), "-o", out], check=True)
+    lib = C.CDLL(out)
+    lib.fft_core_twiddles.restype = C.c_int
+    lib.fft_core_twiddles.argtypes = [C.c_int, C.c_void_p]
+    lib.fft_core_forward.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    return lib
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+def test_fft_core_passes_match_numpy(fft_core_host, n):
+    """The Stockham passes (radix 8, then 4 or 2), the pass-by-pass twiddle
+    table and the half-input first pass (a row zero-padded to twice its
+    length or more, the upper half never read) give the DFT: FP32 against
+    numpy's FP64 FFT, relative L2 below 4e-7."""
+    lib = fft_core_host
+    rng = np.random.default_rng(n)
+    tw = np.zeros(max(8 * n, 8), np.complex64)
+    ne = lib.fft_core_twiddles(n, tw.ctypes.data)
+    assert 0 <= ne <= tw.size
+    for half in (0, 1):
+        x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+        if half:
+            x[n // 2:] = 0
+        buf = x.copy()
+        if half:
+            buf[n // 2:] = np.nan          # must not be read
+        lib.fft_core_forward(buf.ctypes.data, n, tw.ctypes.data, half)
+        ref = np.fft.fft(x.astype(np.complex128))
+        assert np.linalg.norm(buf - ref) <= 4e-7 * np.linalg.norm(ref)
+    # the inverse as the filter kernel takes it: conj(fft(conj(X))) / n
+    X = np.fft.fft(rng.standard_normal(n)).astype(np.complex64)
+    buf = np.conj(X)
+    lib.fft_core_forward(buf.ctypes.data, n, tw.ctypes.data, 0)
+    got = np.conj(buf) / n
+    ref = np.fft.ifft(X.astype(np.complex128))
+    assert np.linalg.norm(got - ref) <= 4e-7 * np.linalg.norm(ref)
+
+
+def test_fft_core_twiddle_layout(fft_core_host):
+    """Pass p of span Ns and radix R owns Ns*R entries, entry r*R + m being
+    exp(-2 pi i m r / (Ns R)); the passes follow each other (N = 2048: spans
+    1, 8, 64, 512 with radices 8, 8, 8, 4)."""
+    lib = fft_core_host
+    tw = np.zeros(8 * 2048, np.complex64)
+    ne = lib.fft_core_twiddles(2048, tw.ctypes.data)
+    assert ne == 1 * 8 + 8 * 8 + 64 * 8 + 512 * 4
+    off = 8                                  # after the span-1 pass (no twiddles used)
+    for ns_, r in ((8, 8), (64, 8), (512, 4)):
+        want = np.exp(-2j * np.pi * np.outer(np.arange(ns_), np.arange(r)) / (ns_ * r))
+        assert np.allclose(tw[off:off + ns_ * r].reshape(ns_, r), want, atol=1e-7)
+        off += ns_ * r
 
 
 def test_feldkamp_weight1_center_value():
