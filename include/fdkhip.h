@@ -183,6 +183,28 @@ int fdkhip_cbct_back_mat(const fdkhip_geom *g, const void *proj_sta, int32_t pro
 	double offset_source, const uint8_t *mask, double scale, double *image_xyz);
 
 /*
+ * The helical cbct_back_mat (mex/src/fdk-helix/cbct_back_helix.m:185-350),
+ * the use_mex = 0 route of cbct_back_helix: as fdkhip_cbct_back_mat (double
+ * precision, the MATLAB edge rules and weights, views at beta[ia] and
+ * source heights source_zs[ia], all na views of proj_sta [ns nt na]) except
+ * that slice iz sums views slice_views[2 iz] : ia_skip : slice_views[2 iz + 1]
+ * (0-based, inclusive; first > last: none), the range the file picks around
+ * the view whose source height is nearest the slice (:224-236), and is
+ * multiplied by slice_scale[iz] (its "der angle" scale, :341-345; NULL: 1).
+ * cone_par = 1: views rebinned to cone-parallel geometry (:245-265): mag =
+ * dsd / (sqrt(dso^2 - x_beta^2) - (dso - y_betas)), sprime = x_beta, source
+ * height source_zs[ia] + asin(-x_beta / dso) * source_dz_per_rad, no
+ * distance weight.  (The file's w3d weighting is not offered: it calls Min
+ * and Max, which the reference does not define.)  image_xyz [nx ny nz]
+ * double, zero outside mask (may be NULL).  Synchronous.
+ */
+int fdkhip_cbct_back_helix_mat(const fdkhip_geom *g, const void *proj_sta,
+	int32_t proj_fp64, int32_t na, const double *beta, const double *source_zs,
+	int32_t ia_skip, double offset_source, const int32_t *slice_views,
+	const double *slice_scale, int32_t cone_par, double source_dz_per_rad,
+	const uint8_t *mask, double *image_xyz);
+
+/*
  * fdkhip_cbct_back_mat with proj_dev (all na views, [ns nt na]) and image_dev
  * ([nx ny nz] double) in device memory; beta, source_zs and mask stay host
  * pointers.  Runs on `stream` and returns when the kernel has finished;
@@ -236,6 +258,8 @@ int fdkhip_plan_config(const fdkhip_plan *plan, int32_t cfg[6]);
 int fdkhip_plan_set_config(fdkhip_plan *plan, int32_t waves, int32_t zper,
 	int32_t vstage, int32_t slot_cap);
 
+/* Waits for the plan's device to finish its queued work (a launch of the
+ * plan may still be running on any stream), then frees the plan. */
 int fdkhip_plan_destroy(fdkhip_plan *plan);
 
 /* Number of (voxel, view) updates the reference loop performs for this plan
@@ -330,6 +354,7 @@ int fdkhip_batch_profile(fdkhip_batch *batch, const void *const *proj,
 /* Sum of fdkhip_plan_count_updates over the scans. */
 int fdkhip_batch_count_updates(fdkhip_batch *batch, int64_t *n_updates);
 
+/* Waits for the device as fdkhip_plan_destroy does, then frees the batch. */
 int fdkhip_batch_destroy(fdkhip_batch *batch);
 
 /*

@@ -71,6 +71,9 @@ _SIGS = {
                                                   C.POINTER(C.c_float)]),
     "fdkhip_cbct_back_mat": (C.c_int, [C.POINTER(Geom), _P, C.c_int32, C.c_int32, _P, _P,
                                        C.c_int32, C.c_double, _P, C.c_double, _P]),
+    "fdkhip_cbct_back_helix_mat": (C.c_int, [C.POINTER(Geom), _P, C.c_int32, C.c_int32, _P,
+                                             _P, C.c_int32, C.c_double, _P, _P, C.c_int32,
+                                             C.c_double, _P, _P]),
     "fdkhip_cbct_back_mat_direct": (C.c_int, [C.POINTER(Geom), _P, C.c_int32, C.c_int32, _P,
                                               _P, C.c_int32, C.c_double, _P, C.c_double, _P,
                                               _P, C.POINTER(C.c_float)]),

@@ -415,9 +415,12 @@ class FdkPlan:
         self.handle = h
 
     @classmethod
-    def from_geoms(cls, cg: CTGeom, ig: ImageGeom, *, ia_skip=1, **kw):
+    def from_geoms(cls, cg: CTGeom, ig: ImageGeom, *, ia_skip=1, extrapolate_t=0, **kw):
+        """Plan for views 1:ia_skip:na of ``cg``; ``extrapolate_t`` rows added
+        above and below (the views it takes then have nt + 2 extrapolate_t
+        rows)."""
         beta = _views(cg, cg.na)[::ia_skip]
-        return cls(geom_of(cg, ig), beta, ig.mask_or, **kw)
+        return cls(geom_of(cg, ig, extrapolate_t), beta, ig.mask_or, **kw)
 
     def __del__(self):
         h = getattr(self, "handle", None)

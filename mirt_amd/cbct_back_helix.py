@@ -78,16 +78,17 @@ def cbct_back_helix(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: i
     with the 13 axial operands (default :func:`fdk_mex`), as the MATLAB does.
     ``cone_par`` = 1 takes views rebinned to cone-parallel geometry; ``w3d``
     is the exponent of the w3d view weights (cone-parallel only).
-    ``use_mex=0`` (the MATLAB backprojector of that file) is not provided.
+    ``use_mex=0`` runs the file's MATLAB backprojector (double precision,
+    per-slice view ranges) on the GPU: :func:`cbct_back_helix_mat`.
     """
     if use_mex is None:
         use_mex = 1
     if nthread is None:
         nthread = 1
     if not use_mex:
-        raise NotImplementedError(
-            "use_mex=0 (cbct_back_helix's MATLAB backprojector) is not provided; "
-            "use_mex=1 computes the helical backprojection on the GPU")
+        return cbct_back_helix_mat(proj, cg, ig, ia_skip=ia_skip, offset_source=offset_source,
+                                   scale_dang=scale_dang, extrapolate_t=extrapolate_t,
+                                   cone_par=cone_par, w3d=w3d)
     del offset_source   # accepted as in cbct_back_helix.m; the mex path ignores it
     ia_skip = int(ia_skip)
     mask = ig.mask_or
@@ -146,4 +147,84 @@ def cbct_back_helix(proj, cg: CTGeom, ig: ImageGeom, *, use_mex=None, ia_skip: i
     return img
 
 
-__all__ = ["cbct_back_helix", "helix_operands", "slice_scales"]
+def helix_mat_slices(cg: CTGeom, ig: ImageGeom, ia_skip: int = 1):
+    """Views and scale of each slice in the helical MATLAB backprojector
+    (cbct_back_helix.m:212-236, 341-345), in double as MATLAB computes them:
+    ``(views, scale)`` with ``views[iz] = (first, last)`` 0-based inclusive
+    (slice iz sums views first:ia_skip:last) and its ``scale_dang`` factor.
+    na1 = na / orbit * 360 must be an integer (to 1e-4), as the file checks."""
+    na, orbit = cg.na, cg.orbit
+    na1 = na / orbit * 360
+    if abs(round(na1) - na1) > 1e-4:
+        raise FdkMexError("bad na/orbit")
+    na1_half = math.ceil(na1 / 2)
+    wz = (ig.nz - 1) / 2 + ig.offset_z
+    zc = (np.arange(ig.nz) - wz) * ig.dz
+    zs = np.broadcast_to(np.asarray(cg.source_zs, np.float64), (na,))
+    views = np.empty((ig.nz, 2), np.int32)
+    scale = np.empty(ig.nz)
+    for iz in range(ig.nz):
+        if cg.pitch == 0:
+            ia_min, ia_max = 1, na
+        else:
+            ia1_middle = int(np.argmin(np.abs(zc[iz] - zs))) + 1          # imin, 1-based
+            ia_min = max(ia1_middle - na1_half, 1)
+            ia_max = min(ia1_middle + (na1_half - 1), na)
+        views[iz] = (ia_min - 1, ia_max - 1)
+        orbit_iz = ((ia_max - ia_min + 1) / ia_skip) * abs(orbit) / (na / ia_skip)
+        sc = math.pi / float(deg2rad(abs(orbit_iz)))
+        scale[iz] = sc * float(deg2rad(abs(orbit))) / (na / ia_skip)
+    return views, scale
+
+
+def cbct_back_helix_mat(proj, cg: CTGeom, ig: ImageGeom, *, ia_skip: int = 1,
+                        offset_source: float = 0.0, scale_dang: bool = True,
+                        extrapolate_t=0, cone_par: int = 0, w3d: int = 0):
+    """The helical MATLAB backprojector, ``cbct_back_mat`` of
+    mex/src/fdk-helix/cbct_back_helix.m:185-350, on the GPU
+    (``fdkhip_cbct_back_helix_mat``): double precision, slice iz sums its own
+    views ia_min:ia_skip:ia_max (:helix_mat_slices), cone-parallel geometry
+    when ``cone_par``, the MATLAB edge rules (zero beyond detector columns
+    [1, ns], rows clamped), then the slice's ``scale_dang`` factor.  ``proj``
+    is ``[ns nt na]``; float32 views stay float32 on the way in, anything
+    else enters as float64.  Returns the float64 ``[nx ny nz]`` image
+    (F-ordered), zero outside ``ig.mask_or``.  ``w3d`` with ``cone_par``
+    is rejected: the reference's code for it calls ``Min`` and ``Max``,
+    which it does not define."""
+    import ctypes as C
+    from . import _lib
+    from .cbct_back import _ptr, geom_of
+    if not extrapolate_t:
+        warnings.warn("mat version always extrapolates in t by replication", stacklevel=2)
+    if cone_par and w3d:
+        raise FdkMexError("cone_par with w3d in the MATLAB backprojector calls Min/Max, "
+                          "which the reference does not define")
+    ia_skip = int(ia_skip)
+    if ia_skip < 1:
+        raise ValueError("ia_skip must be >= 1")
+    ns, nt, na = cg.ns, cg.nt, cg.na
+    proj = np.asarray(proj)
+    if proj.ndim == 2:
+        proj = proj[:, :, None]
+    if proj.shape != (ns, nt, na):
+        raise ValueError(f"proj shape {proj.shape} != [ns nt na] = {(ns, nt, na)}")
+    views, scale = helix_mat_slices(cg, ig, ia_skip)
+    dz_per_rad = float(cg.source_dz_per_view / (float(deg2rad(abs(cg.orbit))) / (na / ia_skip)))
+    fp64 = proj.dtype != np.float32
+    proj = np.asfortranarray(proj, dtype=np.float64 if fp64 else np.float32)
+    betas = np.ascontiguousarray(_views(cg, na), dtype=np.float64)
+    zs = np.ascontiguousarray(np.broadcast_to(np.asarray(cg.source_zs, np.float64), (na,)))
+    mask = np.asfortranarray(ig.mask_or != 0).astype(np.uint8, order="F")
+    img = np.empty((ig.nx, ig.ny, ig.nz), dtype=np.float64, order="F")
+    views = np.ascontiguousarray(views)
+    scale = np.ascontiguousarray(scale) if scale_dang else None
+    lib = _lib.load()
+    _lib.check(lib.fdkhip_cbct_back_helix_mat(
+        C.byref(geom_of(cg, ig)), _ptr(proj), int(fp64), na, _ptr(betas), _ptr(zs), ia_skip,
+        float(offset_source), _ptr(views), None if scale is None else _ptr(scale),
+        int(bool(cone_par)), dz_per_rad, _ptr(mask), _ptr(img)))
+    return img
+
+
+__all__ = ["cbct_back_helix", "cbct_back_helix_mat", "helix_mat_slices", "helix_operands",
+           "slice_scales"]

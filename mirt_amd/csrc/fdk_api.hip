@@ -239,8 +239,24 @@ static BackArgs make_args(const fdkhip_plan *p, const void *proj, const float *s
 	return a;
 }
 
+// Waits for all work on `device`: a graph exec or a buffer about to be
+// destroyed may still be in use by a launch queued on any stream.
+static void quiesce(int device)
+{
+	int cur = 0;
+	if (hipGetDevice(&cur) != hipSuccess)
+		return;
+	if (cur != device)
+		(void) hipSetDevice(device);
+	(void) hipDeviceSynchronize();
+	if (cur != device)
+		(void) hipSetDevice(cur);
+}
+
 static void drop_graph(fdkhip_plan *p)
 {
+	if (p->gexec || p->graph)
+		quiesce(p->device);
 	if (p->gexec) (void) hipGraphExecDestroy(p->gexec);
 	if (p->graph) (void) hipGraphDestroy(p->graph);
 	p->gexec = nullptr;
@@ -334,6 +350,7 @@ static int plan_free(fdkhip_plan *p)
 {
 	if (!p)
 		return FDKHIP_OK;
+	quiesce(p->device);
 	drop_graph(p);
 	if (p->cap_stream) (void) hipStreamDestroy(p->cap_stream);
 	(void) hipFree(p->d_trig);
@@ -1342,6 +1359,73 @@ extern "C" int fdkhip_cbct_back_mat_direct(const fdkhip_geom *g, const void *pro
 	return mat_run(a, beta, source_zs, ia_skip, mask, proj_fp64 != 0, s, kernel_ms);
 }
 
+extern "C" int fdkhip_cbct_back_helix_mat(const fdkhip_geom *g, const void *proj_sta,
+	int32_t proj_fp64, int32_t na, const double *beta, const double *source_zs,
+	int32_t ia_skip, double offset_source, const int32_t *slice_views,
+	const double *slice_scale, int32_t cone_par, double source_dz_per_rad,
+	const uint8_t *mask, double *image_xyz)
+{
+	MatArgs a;
+	int rc = mat_args(g, na, beta, 1, offset_source, 1.0, a);   // every view on the device
+	if (rc)
+		return rc;
+	if (ia_skip < 1)
+		return fail(FDKHIP_EARG, "ia_skip must be >= 1 (ia_skip=%d)", ia_skip);
+	if (cone_par != 0 && cone_par != 1)
+		return fail(FDKHIP_EARG, "cone_par must be 0 or 1 (cone_par=%d)", cone_par);
+	const size_t nvox = (size_t) g->nx * g->ny * g->nz;
+	if (nvox == 0)
+		return FDKHIP_OK;
+	if (!image_xyz)
+		return fail(FDKHIP_EARG, "image is NULL");
+	if (!slice_views)
+		return fail(FDKHIP_EARG, "slice_views is NULL");
+	if (na > 0 && !source_zs)
+		return fail(FDKHIP_EARG, "source_zs is NULL");
+	std::vector<int2> zr((size_t) g->nz);
+	for (int z = 0; z < g->nz; ++z) {
+		const int lo = slice_views[2 * z], hi = slice_views[2 * z + 1];
+		if (lo <= hi && (lo < 0 || hi >= na))
+			return fail(FDKHIP_EARG, "slice %d views [%d, %d] outside [0, %d)", z, lo, hi, na);
+		zr[z] = make_int2(lo, hi);
+	}
+	const size_t view = (size_t) g->ns * g->nt;
+	const size_t esz = proj_fp64 ? sizeof(double) : sizeof(float);
+	if (na > 0 && view > 0 && !proj_sta)
+		return fail(FDKHIP_EARG, "proj is NULL");
+	DevBuf dproj, dimg, dzr, dzs;
+	hipError_t e = hipMalloc(&dimg.p, nvox * sizeof(double));
+	if (e == hipSuccess && na > 0 && view > 0)
+		e = hipMalloc(&dproj.p, view * na * esz);
+	if (e == hipSuccess)
+		e = hipMalloc(&dzr.p, zr.size() * sizeof(int2));
+	if (e == hipSuccess && slice_scale)
+		e = hipMalloc(&dzs.p, (size_t) g->nz * sizeof(double));
+	if (e != hipSuccess)
+		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	if (dproj.p)
+		HIP_TRY(hipMemcpy(dproj.p, proj_sta, view * na * esz, hipMemcpyHostToDevice));
+	HIP_TRY(hipMemcpy(dzr.p, zr.data(), zr.size() * sizeof(int2), hipMemcpyHostToDevice));
+	if (dzs.p)
+		HIP_TRY(hipMemcpy(dzs.p, slice_scale, (size_t) g->nz * sizeof(double),
+			hipMemcpyHostToDevice));
+	a.proj = dproj.p;
+	a.view_stride = (long) view;
+	a.image = (double *) dimg.p;
+	a.zrange = (const int2 *) dzr.p;
+	a.vstep = ia_skip;
+	a.cone_par = cone_par;
+	a.dz_per_rad = source_dz_per_rad;
+	a.zscale = (const double *) dzs.p;
+	if (a.nv == 0 || view == 0) {   // no view adds anything: img2 = 0
+		HIP_TRY(hipMemset(dimg.p, 0, nvox * sizeof(double)));
+	} else if ((rc = mat_run(a, beta, source_zs, 1, mask, proj_fp64 != 0, nullptr, nullptr))) {
+		return rc;
+	}
+	HIP_TRY(hipMemcpy(image_xyz, dimg.p, nvox * sizeof(double), hipMemcpyDeviceToHost));
+	return FDKHIP_OK;
+}
+
 // ---------------------------------------------------------------------------
 // autotuning: launch shape and views per LDS stage, chosen by timing
 
@@ -1543,8 +1627,19 @@ struct fdkhip_batch {
 	std::vector<float *> g_image;
 };
 
+static int batch_device(const fdkhip_batch *b)
+{
+	int dev = 0;
+	if (!b->plans.empty() && b->plans[0])
+		return b->plans[0]->device;
+	(void) hipGetDevice(&dev);
+	return dev;
+}
+
 static void batch_drop_graph(fdkhip_batch *b)
 {
+	if (b->gexec || b->graph)
+		quiesce(batch_device(b));
 	if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
 	if (b->graph) (void) hipGraphDestroy(b->graph);
 	b->gexec = nullptr;
@@ -1558,6 +1653,7 @@ static int batch_free(fdkhip_batch *b)
 {
 	if (!b)
 		return FDKHIP_OK;
+	quiesce(batch_device(b));
 	batch_drop_graph(b);
 	if (b->cap_stream) (void) hipStreamDestroy(b->cap_stream);
 	for (auto *p : b->plans)

@@ -95,6 +95,12 @@ struct MatArgs {
 	long view_stride;        // elements from one used view to the next
 	const uint8_t *mask;     // [ny][nx] or NULL
 	double *image;           // [nz][ny][nx]
+	// helical MATLAB backprojector (cbct_back_helix.m): NULL / 0 otherwise
+	const int2 *zrange;      // [nz] first and last view of each slice (inclusive)
+	int vstep;               // a slice uses views first, first + vstep, ... <= last
+	int cone_par;            // views rebinned to cone-parallel geometry
+	double dz_per_rad;       // source height step per radian (cone_par)
+	const double *zscale;    // [nz] per-slice scale, or NULL for `scale`
 };
 
 size_t back_lds_bytes(int nw, int vstage, int slot, int fmt);

@@ -18,6 +18,14 @@
 // view order starting from 0, as img2 = img2 + p0 does.  The per-view cos and
 // sin come from the host's libm.
 //
+// The helical version (mex/src/fdk-helix/cbct_back_helix.m:185-350) adds,
+// when MatArgs::zrange is set: per-slice view ranges (slice iz uses views
+// ia_min:ia_skip:ia_max around the view whose source height is nearest, the
+// host computes them), the cone-parallel geometry of rebinned views
+// (cone_par: mag = dsd / (sqrt(dso^2 - x_beta^2) - (dso - y_betas)),
+// sprime = x_beta, a source height offset asin(-x_beta / dso) *
+// source_dz_per_rad, no distance weight) and a per-slice scale.
+//
 // Launch: one wave per 8 x 8 tile of image columns and MAT_ZC slices; lane =
 // column, the lane's MAT_ZC accumulators stay in registers over all views.
 // Column geometry (x_beta, y_betas, mag, bs and its weights) is computed once
@@ -85,16 +93,45 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 	}
 	const double ns1 = (double) a.ns + 1.0, ntd = (double) a.nt;
 	const double dsd2 = a.dsd * a.dsd;
+	// the views each slice takes (helical: its own range, cbct_back_helix.m:224-236)
+	int vlo = 0, vhi = a.nv - 1;
+	int lo[MAT_ZC], hi[MAT_ZC];
+	if (a.zrange) {
+		vlo = a.nv;
+		vhi = -1;
+#pragma unroll
+		for (int k = 0; k < MAT_ZC; ++k) {
+			lo[k] = k < nk ? a.zrange[z0 + k].x : 0;
+			hi[k] = k < nk ? a.zrange[z0 + k].y : -1;
+			if (k < nk) {
+				vlo = min(vlo, lo[k]);
+				vhi = max(vhi, hi[k]);
+			}
+		}
+	}
 
-	for (int v = 0; v < a.nv; ++v) {
+	for (int v = max(vlo, 0); v <= min(vhi, a.nv - 1); ++v) {
+		bool use[MAT_ZC], any = false;
+#pragma unroll
+		for (int k = 0; k < MAT_ZC; ++k) {
+			use[k] = k < nk && (!a.zrange || (v >= lo[k] && v <= hi[k] &&
+				(v - lo[k]) % a.vstep == 0));
+			any = any || use[k];
+		}
+		if (!any)
+			continue;
 		const double4 vw = a.views[v];              // {cos beta, sin beta, source_zs}
 		const TP *pv = (const TP *) a.proj + (long) v * a.view_stride;
 		// cbct_back.m:204-205
 		const double x_beta = xc * vw.x + yc * vw.y;
 		const double y_betas = a.dso - (-xc * vw.y + yc * vw.x);
-		// cbct_back.m:208-220
-		double mag, sprime, r_loop = 0.0;
-		if (a.kind == KIND_PAR) {
+		// cbct_back.m:208-220, cbct_back_helix.m:245-265
+		double mag, sprime, r_loop = 0.0, zoff = 0.0;
+		if (a.cone_par) {
+			mag = a.dsd / (sqrt(a.dso * a.dso - x_beta * x_beta) - (a.dso - y_betas));
+			sprime = x_beta;
+			zoff = asin(-x_beta / a.dso) * a.dz_per_rad;
+		} else if (a.kind == KIND_PAR) {
 			mag = 1.0;
 			sprime = mag * x_beta;
 		} else {
@@ -123,9 +160,10 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 
 #pragma unroll
 		for (int k = 0; k < MAT_ZC; ++k) {
-			if (k < nk) {
+			if (use[k]) {
 				// cbct_back.m:219-233 for t: clamp to [1, nt], bt = nt uses rows nt-1, nt
-				const double tprime = mag * (zc[k] - vw.z);
+				const double tprime = a.cone_par ? mag * (zc[k] - (vw.z + zoff))
+					: mag * (zc[k] - vw.z);
 				double bt = tprime / a.dt + a.wt;
 				bt = fmax(bt, 1.0);                  // NaN -> 1, as MATLAB's max
 				bt = fmin(bt, ntd);
@@ -144,19 +182,22 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 				const double p1 = wl * a00 + wr * a10;
 				const double p2 = wl * a01 + wr * a11;
 				double p0 = wd * p1 + wu * p2;
-				if (a.kind == KIND_ARC)
-					p0 = p0 * dsd2 / wden;
-				else
-					p0 = p0 * wmag;
+				if (!a.cone_par) {                   // cone_par: no weight without w3d
+					if (a.kind == KIND_ARC)
+						p0 = p0 * dsd2 / wden;
+					else
+						p0 = p0 * wmag;
+				}
 				acc[k] = acc[k] + p0;                // cbct_back.m:251
 			}
 		}
 	}
 	// cbct_back.m:264-266: (scale) * img, one double product per voxel
+	// (cbct_back_helix.m:341-345: the slice's own scale)
 #pragma unroll
 	for (int k = 0; k < MAT_ZC; ++k)
 		if (k < nk)
-			out[(long) k * nxy] = a.scale * acc[k];
+			out[(long) k * nxy] = (a.zscale ? a.zscale[z0 + k] : a.scale) * acc[k];
 }
 
 } // namespace

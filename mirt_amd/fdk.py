@@ -81,8 +81,63 @@ def fbp_ramp(type: str, n: int, ds: float, dsd: float = None) -> np.ndarray:
     return h
 
 
-def fdk_filter_response(cg: CTGeom, npadh: int, window: str = "ramp") -> np.ndarray:
-    """H of fdk_filter.m:24-33 / fdk_fan_filter :41-63, times ds."""
+def _matlab_round(x: float) -> float:
+    """MATLAB round: half away from zero."""
+    return math.copysign(math.floor(abs(x) + 0.5), x)
+
+
+def fbp2_window(n: int, window) -> np.ndarray:
+    """fbp2_window.m: the apodizing window of length n, fft-shifted (its
+    centre sample ii = 0 first).  ``window``: '' / 'boxcar' / 'ramp' (ones),
+    'boxcar,c', 'hamming,c', 'hanning,c' (cutoff c), 'hann' (c = 1),
+    'hann50', 'hann75', 'hann80', or an array of length n (samples
+    ii = -n/2 .. n/2-1)."""
+    def support(cut):
+        w = _matlab_round(cut * n)
+        ii = np.arange(n) - n / 2
+        return w, ii, np.abs(ii) < w / 2
+
+    def boxcar(cut):
+        return support(cut)[2].astype(np.float64)
+
+    def hann(cut):
+        w, ii, inside = support(cut)
+        return 0.5 * (1 + np.cos(2 * math.pi * ii / w)) * inside
+
+    def hamming(cut):
+        w, ii, inside = support(cut)
+        return (0.54 + 0.46 * np.cos(2 * math.pi * ii / w)) * inside
+
+    def cutoff(prefix):
+        try:
+            return float(window[len(prefix):])
+        except ValueError:
+            raise ValueError(f"bad cutoff in window {window!r}") from None
+
+    if isinstance(window, str):
+        if window in ("", "boxcar", "ramp"):
+            win = np.ones(n)
+        elif window.startswith("boxcar,"):
+            win = boxcar(cutoff("boxcar,"))
+        elif window.startswith("hamming,"):
+            win = hamming(cutoff("hamming,"))
+        elif window.startswith("hanning,"):
+            win = hann(cutoff("hanning,"))
+        elif window in ("hann", "hann50", "hann75", "hann80"):
+            win = hann({"hann": 1.0, "hann50": 0.5, "hann75": 0.75, "hann80": 0.80}[window])
+        else:
+            raise ValueError(f"unknown window {window}")
+    else:
+        win = np.asarray(window, dtype=np.float64).ravel()
+        if win.size != n:
+            raise ValueError("bad window length")
+    return np.fft.fftshift(win)
+
+
+def fdk_filter_response(cg: CTGeom, npadh: int, window="ramp") -> np.ndarray:
+    """H of fdk_filter.m:24-33 / fdk_fan_filter :41-63, times ds.  ``window``:
+    'ramp', 'hann' (periodic Hann, ir_hann_periodic), any fbp2_window name,
+    or an array of npadh samples ii = -npadh/2 .. npadh/2-1."""
     if math.isinf(cg.dsd) or math.isinf(cg.dfs):
         h = fbp_ramp("flat", npadh, cg.ds)
     elif cg.dfs == 0:
@@ -90,12 +145,17 @@ def fdk_filter_response(cg: CTGeom, npadh: int, window: str = "ramp") -> np.ndar
     else:
         raise ValueError("dfs not done")
     H = np.real(np.fft.fft(np.fft.fftshift(h)))
-    if window == "ramp":
-        win = np.ones(npadh)
-    elif window == "hann":
-        win = 0.5 * (1 - np.cos(2 * math.pi * np.arange(npadh) / npadh))   # periodic
+    if isinstance(window, str):
+        if window == "ramp":
+            win = np.ones(npadh)
+        elif window == "hann":
+            win = 0.5 * (1 - np.cos(2 * math.pi * np.arange(npadh) / npadh))   # periodic
+        else:
+            win = np.fft.fftshift(fbp2_window(npadh, window))
     else:
-        raise ValueError(f"window {window} not provided")
+        win = np.asarray(window, dtype=np.float64).ravel()
+        if win.size != npadh:
+            raise ValueError("bad window length")
     H = H * np.fft.fftshift(win)
     return cg.ds * H
 
@@ -147,7 +207,9 @@ _FILTER_TABLES = {}
 
 def _filter_tables(cg: CTGeom, ns, nt, window, w1cyl, device):
     import torch
-    key = (ns, nt, cg.ds, cg.dt, cg.dsd, cg.dso, cg.dfs, cg.offset_s, cg.offset_t, window,
+    wkey = window if isinstance(window, str) else \
+        ("array", np.asarray(window, dtype=np.float64).tobytes())
+    key = (ns, nt, cg.ds, cg.dt, cg.dsd, cg.dso, cg.dfs, cg.offset_s, cg.offset_t, wkey,
            w1cyl, str(device))
     if key not in _FILTER_TABLES:
         ww1 = torch.as_tensor(weight_table(cg, ns, nt, w1cyl=w1cyl).T.copy(),  # [ns nt], s fastest
@@ -189,7 +251,9 @@ def feldkamp(cg: CTGeom, ig: ImageGeom, proj, *, window="ramp", w1cyl=True,
              ia_skip=1, extrapolate_t=0, fmt="f32", math_mode="exact"):
     """feldkamp.m:1-98.  ``proj`` ``[ns nt na]``: a numpy array (host path,
     returns numpy ``[nx ny nz]``) or a device torch tensor (returns a device
-    ``[nx ny nz]`` view; FP16/BF16 storage with ``fmt``)."""
+    ``[nx ny nz]`` view; FP16/BF16 storage with ``fmt``).  ``window`` as
+    fdk_filter takes it (fdk_filter_response); ``extrapolate_t`` replicates
+    the first and last filtered rows that many times (cbct_back.m:95-104)."""
     if cg.pitch != 0 or np.any(cg.source_zs != 0):
         raise ValueError("sorry, helical CT unsupported")
     if isinstance(proj, np.ndarray):
@@ -197,12 +261,17 @@ def feldkamp(cg: CTGeom, ig: ImageGeom, proj, *, window="ramp", w1cyl=True,
         p = fdk_filter(feldkamp_weight1(p, cg, w1cyl=w1cyl), cg, window)
         return cbct_back(p.numpy().astype(np.float32), cg, ig, ia_skip=ia_skip,
                          extrapolate_t=extrapolate_t)
-    if extrapolate_t:
-        raise NotImplementedError("extrapolate_t on the device path")
+    e = int(extrapolate_t)
+    if e > 0 and cg.offset_t != 0:
+        raise ValueError("offset_t not yet done for extrapolation")
     # [ns nt na] device views (torch shape (ns, nt, na)) -> MATLAB memory order
     sta = proj[:, :, ::ia_skip].permute(2, 1, 0).contiguous().to(torch.float32)
     tsa = weight_filter_views(sta, cg, window=window, w1cyl=w1cyl)        # [nt ns na]
-    plan = FdkPlan.from_geoms(cg, ig, ia_skip=ia_skip, fmt=fmt, math_mode=math_mode)
+    if e > 0:   # cbct_back.m:95-104: e copies of the first and last filtered rows
+        tsa = torch.cat([tsa[:, :, :1].expand(-1, -1, e), tsa,
+                         tsa[:, :, -1:].expand(-1, -1, e)], dim=2).contiguous()
+    plan = FdkPlan.from_geoms(cg, ig, ia_skip=ia_skip, extrapolate_t=e, fmt=fmt,
+                              math_mode=math_mode)
     if fmt == "f32":
         img = plan(tsa)
     else:

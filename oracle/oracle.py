@@ -15,6 +15,14 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 
 and one numpy function of its own:
 
+* :func:`back_helix_mat_f64` and :func:`helix_mat_slices` -- the helical
+  ``cbct_back_mat`` of ``mex/src/fdk-helix/cbct_back_helix.m:185-350``
+  (per-slice view ranges and scales, cone-parallel geometry; w3d excluded,
+  it calls functions the reference does not define).  Pinned in
+  ``tests/test_oracle_mat.py``: for pitch 0 it is :func:`back_mat_f64`
+  with the axial scale; each slice equals :func:`back_mat_f64` over its own
+  views; and linear views recover the sample positions that the
+  cone-parallel ray geometry gives, computed from first principles.
 * :func:`back_mat_f64` -- restatement of ``cbct_back_mat``
   (``fbp/cbct_back.m:142-270``), the reference's MATLAB backprojector, step
   by step and vectorised over the masked pixels as the MATLAB code is, in
@@ -361,3 +369,120 @@ def back_mat_f64(g: OrcGeom, proj_sta, beta, *, source_zs=None, ia_skip=1,
     img = np.zeros((nx, ny, nz))
     img[m] = img2                                    # embed (cbct_back.m:261)
     return scale * img
+
+
+def helix_mat_slices(g: OrcGeom, na: int, orbit: float, pitch: float, source_zs,
+                     ia_skip: int = 1):
+    """Per-slice view ranges and scales of the helical MATLAB backprojector
+    (mex/src/fdk-helix/cbct_back_helix.m:212-236, 341-345), in double as
+    MATLAB computes them: returns ``(first, last, scale)`` with 0-based
+    inclusive view ranges (slice iz uses views first:ia_skip:last) and the
+    ``scale_dang`` factor of each slice."""
+    na1 = na / orbit * 360                           # :212-216
+    if abs(np.round(na1) - na1) > 1e-4:
+        raise ValueError("bad na/orbit")
+    na1_half = math.ceil(na1 / 2)
+    wz = (g.nz - 1) / 2 + g.offset_z
+    zc = (np.arange(g.nz) - wz) * g.dz
+    zs = np.broadcast_to(np.asarray(source_zs, np.float64), (na,))
+    first = np.empty(g.nz, np.int64)
+    last = np.empty(g.nz, np.int64)
+    scale = np.empty(g.nz)
+    for iz in range(g.nz):
+        if pitch == 0:                               # :226-228
+            ia_min, ia_max = 1, na
+        else:                                        # :229-235, 1-based
+            ia1_middle = int(np.argmin(np.abs(zc[iz] - zs))) + 1
+            ia_min = max(ia1_middle - na1_half, 1)
+            ia_max = min(ia1_middle + (na1_half - 1), na)
+        first[iz], last[iz] = ia_min - 1, ia_max - 1
+        orbit_iz = ((ia_max - ia_min + 1) / ia_skip) * abs(orbit) / (na / ia_skip)
+        sc = math.pi / ((math.pi / 180) * abs(orbit_iz))
+        scale[iz] = sc * ((math.pi / 180) * abs(orbit)) / (na / ia_skip)
+    return first, last, scale
+
+
+def back_helix_mat_f64(g: OrcGeom, proj_sta, beta, source_zs, first, last, *, ia_skip=1,
+                       offset_source=0.0, mask=None, slice_scale=None, cone_par=0,
+                       source_dz_per_rad=0.0):
+    """numpy restatement of the helical ``cbct_back_mat``
+    (mex/src/fdk-helix/cbct_back_helix.m:185-350) without w3d: slice iz
+    sums views first[iz]:ia_skip:last[iz] (0-based, inclusive) as
+    ``img2 = img2 + p0`` in view order, with the cone-parallel geometry when
+    ``cone_par`` (:245-265: mag = dsd / (sqrt(dso^2 - x_beta^2) - (dso -
+    y_betas)), sprime = x_beta, source height offset asin(-x_beta / dso) *
+    source_dz_per_rad, no distance weight), then times ``slice_scale[iz]``
+    (or 1) and embedded in the mask.  Same edge rules, operation order and
+    NaN deviation as :func:`back_mat_f64`."""
+    nx, ny, nz, ns, nt = g.nx, g.ny, g.nz, g.ns, g.nt
+    proj = np.asarray(proj_sta, dtype=np.float64)
+    if proj.ndim == 2:
+        proj = proj[:, :, None]
+    na = proj.shape[2]
+    beta = np.asarray(beta, dtype=np.float64).reshape(-1)
+    zs = np.broadcast_to(np.asarray(source_zs, np.float64), (na,))
+    par = math.isinf(g.dsd) or math.isinf(g.dso)
+    flat = par or math.isinf(g.dfs)
+    if not cone_par and not flat and g.dfs != 0:
+        raise ValueError("dfs must be 0 (arc) or inf (flat)")
+    m = np.ones((nx, ny), bool) if mask is None else (np.asarray(mask) != 0)
+    wx = (nx - 1) / 2 + g.offset_x
+    wy = (ny - 1) / 2 + g.offset_y
+    wz = (nz - 1) / 2 + g.offset_z
+    xc, yc = np.meshgrid((np.arange(nx) - wx) * g.dx, (np.arange(ny) - wy) * g.dy,
+                         indexing="ij")
+    zc = (np.arange(nz) - wz) * g.dz
+    xc = xc[m]
+    yc = yc[m]
+    ws = (ns + 1) / 2 + g.offset_s
+    wt = (nt + 1) / 2 + g.offset_t
+    proj1 = np.zeros((ns + 1, nt))
+    img = np.zeros((nx, ny, nz))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for iz in range(nz):
+            img2 = np.zeros(xc.size)
+            for ia in range(int(first[iz]), int(last[iz]) + 1, int(ia_skip)):
+                cb, sb = math.cos(beta[ia]), math.sin(beta[ia])
+                x_beta = xc * cb + yc * sb
+                y_betas = g.dso - (-xc * sb + yc * cb)
+                if cone_par:
+                    mag = g.dsd / (np.sqrt(g.dso * g.dso - x_beta * x_beta) - (g.dso - y_betas))
+                    sprime = x_beta
+                    zoff = np.arcsin(-x_beta / g.dso) * source_dz_per_rad
+                    tprime = mag * (zc[iz] - (zs[ia] + zoff))
+                else:
+                    mag = np.ones_like(y_betas) if par else g.dsd / y_betas
+                    if flat:
+                        sprime = mag * x_beta
+                    else:
+                        r_loop = x_beta - offset_source
+                        sprime = g.dsd * np.arctan2(r_loop, y_betas)
+                    tprime = mag * (zc[iz] - zs[ia])
+                bs = sprime / g.ds + ws
+                bt = tprime / g.dt + wt
+                bs = np.where((bs >= 1) & (bs <= ns), bs, ns + 1.0)
+                bt = np.fmin(np.fmax(bt, 1.0), float(nt))
+                is_ = np.floor(bs)
+                it = np.floor(bt)
+                is_[is_ == ns + 1] = ns
+                it[it == nt] = nt - 1
+                wr = bs - is_
+                wl = 1 - wr
+                wu = bt - it
+                wd = 1 - wu
+                proj1[:ns, :] = proj[:, :, ia]
+                i0 = is_.astype(np.int64) - 1
+                t0 = it.astype(np.int64) - 1
+                p1 = wl * proj1[i0, t0] + wr * proj1[i0 + 1, t0]
+                p2 = wl * proj1[i0, t0 + 1] + wr * proj1[i0 + 1, t0 + 1]
+                p0 = wd * p1 + wu * p2
+                if not cone_par:
+                    if flat:
+                        p0 = p0 * (mag * mag)
+                    else:
+                        p0 = p0 * (g.dsd * g.dsd) / (r_loop * r_loop + y_betas * y_betas)
+                img2 = img2 + p0
+            if slice_scale is not None:
+                img2 = slice_scale[iz] * img2
+            img[:, :, iz][m] = img2
+    return img

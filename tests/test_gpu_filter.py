@@ -7,11 +7,14 @@ feldkamp_weight1 + fdk_filter, which tests/test_host.py pins to direct
 convolution and to fbp_ramp's closed form):
 * filtered views within relative L2 1e-6 and max |error| 1e-5 of the
   largest sample (FP32 weights, taps and accumulation over ns terms);
-* feldkamp's device route within relative L2 1e-5 of its host route.
+* feldkamp's device route within relative L2 1e-5 of its host route (FP64
+  weight and filter) with FP32 storage, 1e-3 with FP16 storage, 4e-3 with
+  BF16 storage.
 """
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 
 import numpy as np
 import pytest
@@ -94,17 +97,73 @@ def test_filter_sizes(device, ns, nt):
     assert err <= 1e-6, err
 
 
-def test_feldkamp_device_route_vs_host_route(device):
+def _window(name, ns):
+    """A window argument of feldkamp: a name, or (for "array") samples
+    ii = -npad/2 .. npad/2-1 of a Gaussian taper of the padded length."""
+    if name != "array":
+        return name
+    npad = 2 ** int(np.ceil(np.log2(2 * ns - 1)))
+    ii = np.arange(npad) - npad / 2
+    return np.exp(-(ii / (npad / 4)) ** 2)
+
+
+@pytest.mark.parametrize("window", ["ramp", "hann", "hamming,0.6", "array"])
+@pytest.mark.parametrize("fmt", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("ia_skip", [1, 3])
+def test_feldkamp_device_route_vs_host_route(device, ia_skip, fmt, window):
+    """feldkamp on device views (view skipping before the filter, the fused
+    weight + filter kernel, then the plan; FP16/BF16: quantised filtered
+    views) against the host route (FP64 weight and FFT filter, then the
+    backprojector on FP32 views).  Bars (relative L2): FP32 1e-5; FP16
+    storage 1e-3; BF16 storage 4e-3, its unit roundoff 2^-8 (eight times
+    FP16's: 7 stored fraction bits against 10)."""
     import torch
     from mirt_amd.fdk import feldkamp
     from mirt_amd.geom import ct_geom, image_geom
     cg = ct_geom("fan", ns=96, nt=24, na=72, ds=1.0, dt=1.0, dsd=949.0, dso=541.0, dfs=0.0)
     ig = image_geom(nx=40, ny=36, nz=16, dx=1.2, dz=1.0)
     p = np.asfortranarray(np.random.default_rng(3).random(cg.dim)).astype(np.float32)
-    host = feldkamp(cg, ig, np.asarray(p, np.float64))
-    dev = feldkamp(cg, ig, torch.from_numpy(np.ascontiguousarray(p)).to(device))
+    w = _window(window, cg.ns)
+    w1cyl = window != "hann"
+    print(f"feldkamp device route ia_skip={ia_skip} {fmt} {window} w1cyl={w1cyl}", flush=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        host = feldkamp(cg, ig, np.asarray(p, np.float64), ia_skip=ia_skip, window=w,
+                        w1cyl=w1cyl)
+        dev = feldkamp(cg, ig, torch.from_numpy(np.ascontiguousarray(p)).to(device),
+                       ia_skip=ia_skip, window=w, w1cyl=w1cyl, fmt=fmt)
     dev = dev.cpu().numpy()
     assert dev.shape == host.shape
     err = np.linalg.norm(dev - host) / np.linalg.norm(host)
-    print(f"feldkamp device vs host relL2 {err:.2e}", flush=True)
-    assert err <= 1e-5, err
+    print(f"  relL2 {err:.2e}", flush=True)
+    assert err <= {"f32": 1e-5, "f16": 1e-3, "bf16": 4e-3}[fmt], err
+
+
+@pytest.mark.parametrize("fmt", ["f32", "f16"])
+def test_feldkamp_device_route_extrapolate_t(device, fmt):
+    """extrapolate_t on the device route: the filtered first and last rows
+    replicated (cbct_back.m:95-104) before the plan, against the host route;
+    a volume taller than the detector so that the replicated rows are read;
+    offset_t with extrapolation is rejected as the reference does."""
+    import torch
+    from mirt_amd.fdk import feldkamp
+    from mirt_amd.geom import ct_geom, image_geom
+    cg = ct_geom("fan", ns=80, nt=16, na=60, ds=1.0, dt=1.0, dsd=949.0, dso=541.0,
+                 dfs=np.inf)
+    ig = image_geom(nx=36, ny=36, nz=30, dx=1.2, dz=1.0)
+    p = np.asfortranarray(np.random.default_rng(5).random(cg.dim)).astype(np.float32)
+    pd = torch.from_numpy(np.ascontiguousarray(p)).to(device)
+    print(f"feldkamp extrapolate_t {fmt}", flush=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        host = feldkamp(cg, ig, np.asarray(p, np.float64), extrapolate_t=9)
+        flat = feldkamp(cg, ig, np.asarray(p, np.float64))
+        dev = feldkamp(cg, ig, pd, extrapolate_t=9, fmt=fmt).cpu().numpy()
+    assert np.linalg.norm(host - flat) > 0.05 * np.linalg.norm(host)    # rows are read
+    err = np.linalg.norm(dev - host) / np.linalg.norm(host)
+    print(f"  relL2 {err:.2e}", flush=True)
+    assert err <= (1e-5 if fmt == "f32" else 1e-3), err
+    cg2 = ct_geom("fan", ns=80, nt=16, na=60, ds=1.0, dt=1.0, dsd=949.0, dso=541.0,
+                  dfs=np.inf, offset_t=0.5)
+    with pytest.raises(ValueError, match="offset_t not yet done"):
+        feldkamp(cg2, ig, pd, extrapolate_t=3)

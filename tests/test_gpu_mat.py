@@ -200,3 +200,48 @@ def test_mat_route_vs_mex_route_full_size(device):
           flush=True)
     assert err64 <= 1e-11
     assert err32 <= 3e-3
+
+
+@pytest.mark.parametrize("case", ["arc", "flat", "flat_cone_par", "flat_pitch0"])
+def test_cbct_back_helix_use_mex_0(device, case):
+    """cbct_back_helix(..., use_mex=0): the helical MATLAB backprojector
+    (mex/src/fdk-helix/cbct_back_helix.m:185-350) on the device against the
+    oracle's restatement (oracle.back_helix_mat_f64 with the slice ranges and
+    scales of oracle.helix_mat_slices): per-slice view ranges on a helix,
+    ia_skip 1 and 2, scale_dang on and off, cone-parallel views, a mask,
+    float32 and float64 views.  Same bar as the axial route (bit for bit
+    for flat; arc and cone_par within 1e-13, the device's atan2 / asin may
+    differ from the host's in the last place)."""
+    from mirt_amd import cbct_back_helix
+    from mirt_amd.cbct_back import _views
+    from mirt_amd.geom import ct_geom, image_geom
+    dfs = 0.0 if case == "arc" else np.inf
+    pitch = 0.0 if case == "flat_pitch0" else 0.8
+    cone_par = int(case == "flat_cone_par")
+    cg = ct_geom("fan", ns=90, nt=40, na=48, ds=1.3, dt=1.1, offset_s=0.75, offset_t=0.2,
+                 dsd=949.0, dso=541.0, dfs=dfs, orbit=360, pitch=pitch)
+    ig = image_geom(nx=34, ny=32, nz=23, dx=1.4, dz=1.6, offset_z=0.25, mask="all-but-edge")
+    g = O.geom((ig.nx, ig.ny, ig.nz), (ig.dx, ig.dy, ig.dz),
+               (ig.offset_x, ig.offset_y, ig.offset_z), cg.dso, cg.dsd, cg.dfs,
+               cg.ns, cg.nt, (cg.ds, cg.dt), (cg.offset_s, cg.offset_t))
+    rng = np.random.default_rng(11)
+    for ia_skip, dtype, scale_dang in [(1, np.float32, True), (2, np.float64, False)]:
+        print(f"helix mat {case} ia_skip={ia_skip} {dtype.__name__} scale={scale_dang}",
+              flush=True)
+        proj = np.asfortranarray(rng.standard_normal((cg.ns, cg.nt, cg.na)).astype(dtype))
+        with pytest.warns(UserWarning, match="always extrapolates"):
+            got = cbct_back_helix(proj, cg, ig, use_mex=0, ia_skip=ia_skip,
+                                  offset_source=1.5 if case == "arc" else 0.0,
+                                  scale_dang=scale_dang, cone_par=cone_par)
+        first, last, sc = O.helix_mat_slices(g, cg.na, cg.orbit, cg.pitch, cg.source_zs,
+                                             ia_skip)
+        if pitch:
+            assert len(set(zip(first, last))) > 1
+        dz_per_rad = cg.source_dz_per_view / ((np.pi / 180 * abs(cg.orbit)) / (cg.na / ia_skip))
+        want = O.back_helix_mat_f64(g, proj, _views(cg, cg.na), cg.source_zs, first, last,
+                                    ia_skip=ia_skip, offset_source=1.5 if case == "arc" else 0.0,
+                                    mask=ig.mask_or, slice_scale=sc if scale_dang else None,
+                                    cone_par=cone_par, source_dz_per_rad=dz_per_rad)
+        assert got.dtype == np.float64 and got.shape == (ig.nx, ig.ny, ig.nz)
+        assert np.abs(want).max() > 0
+        _compare("arc" if (case == "arc" or cone_par) else "flat", got, want)

@@ -151,6 +151,47 @@ def test_cbct_back_mat_argument_errors_before_device():
                                     P(img)) == _lib.FDKHIP_EARG
 
 
+def test_cbct_back_helix_mat_argument_errors_before_device():
+    """fdkhip_cbct_back_helix_mat rejects a slice's view range outside the
+    views, cone_par other than 0/1 and a missing range table before it
+    touches the device."""
+    from mirt_amd import _lib
+    lib = _lib.load()
+    g = _lib.Geom(4, 4, 2, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 541.0, 949.0, np.inf, 8, 6, 1.0, 1.0,
+                  0.0, 0.0)
+    proj = np.zeros((8, 6, 3), np.float32, order="F")
+    beta = np.zeros(3)
+    zs = np.zeros(3)
+    img = np.zeros((4, 4, 2))
+    P = lambda a: None if a is None else C.c_void_p(a.ctypes.data)
+    def call(views, cone_par=0):
+        return lib.fdkhip_cbct_back_helix_mat(C.byref(g), P(proj), 0, 3, P(beta), P(zs), 1,
+                                              0.0, P(views), None, cone_par, 0.0, None, P(img))
+    assert call(np.array([[0, 3], [0, 2]], np.int32)) == _lib.FDKHIP_EARG
+    assert "outside" in lib.fdkhip_last_error().decode()
+    assert call(np.array([[-1, 2], [0, 2]], np.int32)) == _lib.FDKHIP_EARG
+    assert call(np.array([[0, 2], [0, 2]], np.int32), cone_par=2) == _lib.FDKHIP_EARG
+    assert "cone_par" in lib.fdkhip_last_error().decode()
+    assert call(None) == _lib.FDKHIP_EARG
+
+
+def test_cbct_back_helix_mat_python_checks():
+    from mirt_amd import cbct_back_helix
+    from mirt_amd.cbct_back import FdkMexError
+    from mirt_amd.geom import ct_geom, image_geom
+    ig = image_geom(nx=8, ny=8, nz=4, dx=1.0, dz=1.0)
+    cg = ct_geom("fan", ns=16, nt=8, na=25, ds=1.0, dt=1.0, dsd=949.0, dso=541.0,
+                 dfs=np.inf, orbit=361, pitch=0.5)
+    proj = np.zeros((16, 8, 25), np.float32)
+    with pytest.warns(UserWarning), pytest.raises(FdkMexError, match="bad na/orbit"):
+        cbct_back_helix(proj, cg, ig, use_mex=0)
+    cg = ct_geom("fan", ns=16, nt=8, na=24, ds=1.0, dt=1.0, dsd=949.0, dso=541.0,
+                 dfs=np.inf, orbit=360, pitch=0.5)
+    with pytest.raises(FdkMexError, match="Min/Max"):
+        cbct_back_helix(np.zeros((16, 8, 24), np.float32), cg, ig, use_mex=0, cone_par=1,
+                        w3d=1, extrapolate_t=1)
+
+
 def test_cbct_back_mat_python_checks():
     from mirt_amd import cbct_back
     from mirt_amd.geom import ct_geom, image_geom
@@ -492,6 +533,46 @@ def test_fft_core_twiddle_layout(fft_core_host):
         want = np.exp(-2j * np.pi * np.outer(np.arange(ns_), np.arange(r)) / (ns_ * r))
         assert np.allclose(tw[off:off + ns_ * r].reshape(ns_, r), want, atol=1e-7)
         off += ns_ * r
+
+
+def test_fbp2_window_forms():
+    """fbp2_window.m: boxcar/ramp/'' are ones; every window is fft-shifted
+    (its centre ii = 0 first, value 1); 'hann' is 'hanning,1'; a cutoff c
+    keeps the round(c n) - 1 samples with |ii| < round(c n)/2 (even widths)
+    and zeroes the rest; the windows are symmetric about ii = 0; Hamming's
+    edge tends to 0.08; arrays pass through shifted; bad names and lengths
+    fail.  fdk_filter's response takes them (fdk_fan_filter :50-58)."""
+    from mirt_amd.fdk import fbp2_window, fdk_filter_response
+    from mirt_amd.geom import ct_geom
+    n = 128
+    for w in ("", "boxcar", "ramp"):
+        assert np.array_equal(fbp2_window(n, w), np.ones(n))
+    assert np.array_equal(fbp2_window(n, "hann"), fbp2_window(n, "hanning,1"))
+    for name, c in [("hann50", 0.5), ("hann75", 0.75), ("hann80", 0.8), ("boxcar,0.6", 0.6),
+                    ("hamming,0.6", 0.6), ("hanning,0.3", 0.3)]:
+        win = fbp2_window(n, name)
+        assert win[0] == 1.0                                   # ii = 0 first
+        centred = np.fft.ifftshift(win)                        # ii = -n/2 .. n/2-1
+        wdt = int(np.floor(c * n + 0.5))
+        assert np.count_nonzero(centred) == wdt - 1 + (wdt % 2)
+        k = np.arange(1, n // 2)
+        assert np.array_equal(centred[n // 2 + k], centred[n // 2 - k])
+        if name.startswith("hamming"):
+            edge = centred[n // 2 + (wdt - 1) // 2]
+            assert 0.08 < edge < 0.1
+    ramp = np.linspace(0, 1, n)
+    assert np.array_equal(fbp2_window(n, ramp), np.fft.fftshift(ramp))
+    with pytest.raises(ValueError, match="unknown window"):
+        fbp2_window(n, "triangle")
+    with pytest.raises(ValueError, match="bad window length"):
+        fbp2_window(n, np.ones(n - 1))
+    cg = ct_geom("fan", ns=40, nt=2, na=1, ds=1.0, dt=1.0, dsd=949.0, dso=541.0, dfs=np.inf)
+    H = fdk_filter_response(cg, 128, "ramp")
+    Hb = fdk_filter_response(cg, 128, "boxcar,0.5")
+    assert np.array_equal(Hb, H * np.fft.fftshift(np.fft.ifftshift(fbp2_window(128, "boxcar,0.5"))))
+    assert Hb[0] == H[0] and np.count_nonzero(Hb) < np.count_nonzero(H)
+    with pytest.raises(ValueError, match="bad window length"):
+        fdk_filter_response(cg, 128, np.ones(64))
 
 
 def test_feldkamp_weight1_center_value():

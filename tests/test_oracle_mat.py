@@ -246,3 +246,127 @@ def test_angle_scale64_closed_form():
                  dfs=0.0)
     assert abs(angle_scale64(cg) - math.pi / 984) < 1e-17
     assert abs(angle_scale64(cg, 3) - 3 * math.pi / 984) < 1e-16
+
+
+# --------------------------------------------------------------------------
+# the helical cbct_back_mat (mex/src/fdk-helix/cbct_back_helix.m:185-350)
+
+def _helix_case(name, na=24, seed=5, span=30.0):
+    g, proj, beta, mask = mat_case(name, seed=seed, zero_edges=False)
+    proj, beta = proj[:, :, :na], np.linspace(0, 2 * np.pi, na, endpoint=False) + 0.1
+    zs = np.linspace(-span / 2, span / 2, na)        # one turn of a helix
+    return g, proj, beta, mask, zs
+
+
+@pytest.mark.parametrize("name", ["flat", "arc"])
+@pytest.mark.parametrize("ia_skip", [1, 3])
+def test_helix_mat_pitch0_is_the_axial_mat(name, ia_skip):
+    """pitch = 0: every slice takes views 1:ia_skip:na and the per-slice
+    scale is the axial scale_dang factor, so the helical backprojector is
+    cbct_back_mat with its 0.5 * deg2rad(|orbit|) / (na / ia_skip)."""
+    g, proj, beta, mask, zs = _helix_case(name)
+    na = beta.size
+    first, last, sc = O.helix_mat_slices(g, na, 360.0, 0.0, zs, ia_skip)
+    assert np.all(first == 0) and np.all(last == na - 1)
+    axial = 0.5 * ((math.pi / 180) * 360.0) / (na / ia_skip)
+    np.testing.assert_allclose(sc, axial, rtol=4e-16)
+    got = O.back_helix_mat_f64(g, proj, beta, zs, first, last, ia_skip=ia_skip, mask=mask,
+                               slice_scale=sc)
+    want = O.back_mat_f64(g, proj, beta, source_zs=zs, ia_skip=ia_skip, mask=mask,
+                          scale=axial)
+    assert O.rel_l2(got, want) < 1e-15
+
+
+def test_helix_mat_slices_closed_form():
+    """na = 24 over 360 degrees: na1_half = 12; a slice whose nearest source
+    is view 5 (1-based) takes views 1..16, a share 16/24 of the orbit, and
+    its scale is pi / deg2rad(240) * deg2rad(360) / 24 = pi / 16."""
+    g = O.geom((4, 4, 3), (1.0, 1.0, 10.0), (0.0, 0.0, 0.0), 541.0, 949.0, np.inf,
+               8, 6, (1.0, 1.0), (0.0, 0.0))
+    zs = np.arange(24) * 10.0 - 40.0                  # slice zc = -10, 0, 10 -> views 4, 5, 6
+    first, last, sc = O.helix_mat_slices(g, 24, 360.0, 1.0, zs)
+    np.testing.assert_array_equal(first, [0, 0, 0])
+    np.testing.assert_array_equal(last, [14, 15, 16])
+    np.testing.assert_allclose(sc[1], math.pi / 16, rtol=1e-15)
+    # the ranges are the views within [-12, +11] of the nearest one (brute force)
+    zc = (np.arange(3) - 1.0) * 10.0
+    for iz in range(3):
+        mid = int(np.argmin(np.abs(zc[iz] - zs)))
+        want = [ia for ia in range(24) if -12 <= ia - mid <= 11]
+        assert (first[iz], last[iz]) == (want[0], want[-1])
+    with pytest.raises(ValueError, match="bad na/orbit"):
+        O.helix_mat_slices(g, 25, 361.0, 1.0, np.zeros(25))
+
+
+@pytest.mark.parametrize("name", ["flat", "arc"])
+@pytest.mark.parametrize("ia_skip", [1, 2])
+def test_helix_mat_slice_is_the_axial_mat_over_its_views(name, ia_skip):
+    """pitch != 0: slice iz is cbct_back_mat over its own views
+    first:ia_skip:last only (same sums in the same order), times its scale."""
+    g, proj, beta, mask, zs = _helix_case(name, span=float(g_span(name)))
+    na = beta.size
+    first, last, sc = O.helix_mat_slices(g, na, 360.0, 1.0, zs, ia_skip)
+    assert len(set(zip(first, last))) > 1                     # the ranges differ
+    got = O.back_helix_mat_f64(g, proj, beta, zs, first, last, ia_skip=ia_skip, mask=mask,
+                               slice_scale=sc)
+    for iz in range(g.nz):
+        sel = np.arange(first[iz], last[iz] + 1, ia_skip)
+        one = O.back_mat_f64(g, proj[:, :, sel], beta[sel], source_zs=zs[sel], mask=mask)
+        np.testing.assert_array_equal(got[:, :, iz], sc[iz] * one[:, :, iz])
+
+
+def g_span(name):
+    """z extent of a case's volume (so that the helix spans its slices)."""
+    c = MAT_CASES[name]
+    return c["nxyz"][2] * c["dxyz"][2]
+
+
+def test_helix_mat_cone_parallel_ray_geometry():
+    """Cone-parallel views (rebinned fan rays, one source per column): a view
+    linear in the 1-based bins, p = a + b bs + c bt, backprojects (bilinear
+    interpolation being exact on it, no weight) to a + b bs + c bt at the
+    sample each voxel's ray hits.  That sample is computed here from the ray
+    itself: the ray through voxel V is parallel to the view's central ray
+    u = (sin beta, -cos beta), at lateral offset s = V . (cos beta, sin beta);
+    its source S lies on the source circle (radius dso) at that offset; S's
+    angle on the circle (source at angle phi = (-dso sin phi, dso cos phi))
+    gives its height zs + (phi_S - beta) * dz_per_rad; the detector lies dsd
+    from S along u, so t = dsd / ((V - S) . u) * (z - z_S)."""
+    g = O.geom((12, 10, 6), (4.0, 4.0, 3.0), (0.3, -0.2, 0.1), 541.0, 949.0, np.inf,
+               120, 60, (2.0, 1.5), (0.4, -0.3))
+    beta = np.array([0.3, 1.9, 4.1])
+    zs = np.array([-2.0, 5.0, 11.0])
+    dz_per_rad = 7.5
+    a, b, c = 0.5, 0.01, -0.02
+    sb = np.arange(1, g.ns + 1, dtype=np.float64)
+    tb = np.arange(1, g.nt + 1, dtype=np.float64)
+    view = a + b * sb[:, None] + c * tb[None, :]
+    proj = np.repeat(view[:, :, None], beta.size, axis=2)
+    first = np.zeros(g.nz, np.int64)
+    last = np.full(g.nz, beta.size - 1)
+    ws = (g.ns + 1) / 2 + g.offset_s
+    wt = (g.nt + 1) / 2 + g.offset_t
+    xc = (np.arange(g.nx) - ((g.nx - 1) / 2 + g.offset_x)) * g.dx
+    yc = (np.arange(g.ny) - ((g.ny - 1) / 2 + g.offset_y)) * g.dy
+    zc = (np.arange(g.nz) - ((g.nz - 1) / 2 + g.offset_z)) * g.dz
+    want = np.zeros((g.nx, g.ny, g.nz))
+    for ia, bb in enumerate(beta):
+        u = np.array([math.sin(bb), -math.cos(bb)])
+        e = np.array([math.cos(bb), math.sin(bb)])
+        for ix in range(g.nx):
+            for iy in range(g.ny):
+                V = np.array([xc[ix], yc[iy]])
+                s = V @ e
+                S = s * e - math.sqrt(g.dso ** 2 - s ** 2) * u      # on the circle, behind V
+                phi = math.atan2(-S[0], S[1])
+                dphi = (phi - bb + math.pi) % (2 * math.pi) - math.pi
+                z_src = zs[ia] + dphi * dz_per_rad
+                D = (V - S) @ u
+                for iz in range(g.nz):
+                    t = g.dsd / D * (zc[iz] - z_src)
+                    bs, bt = s / g.ds + ws, t / g.dt + wt
+                    assert 1 < bs < g.ns and 1 < bt < g.nt          # inside: no clamp
+                    want[ix, iy, iz] += a + b * bs + c * bt
+    got = O.back_helix_mat_f64(g, proj, beta, zs, first, last, cone_par=1,
+                               source_dz_per_rad=dz_per_rad)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
