@@ -499,11 +499,6 @@ def run_mat(args, dev):
     return result
 
 
-# launch shape of the helical kernel as built (slices per lane, LDS footprint
-# floats); the committed profile's traffic is used only for this shape
-HELIX_LAUNCH = "slices=24 footprint=768"
-
-
 def helix_update_count(cg, ig, beta, zs, izb, dev, z_range=None):
     """Voxel-view updates of the reference's helical loop (fdk,ts.c:112-245:
     columns whose s bin is in [-1, ns), slices [iz_min, iz_max) inside the
@@ -562,17 +557,17 @@ def helix_windows(cg, ig, na1_half):
 
 def run_helix(args, dev):
     """The helical 'fdk,ts,back' of the fdk-helix kernel on a helical scan
-    (clinical_helix: 3 turns of the clinical_arc detector): views [nt ns na]
-    FP32 and the [nz nx ny] image resident on the device
-    (fdkhip_fdk_ts_back_helix_direct), native helical geometry."""
-    import ctypes as C
+    (clinical_helix: 3 turns of the clinical_arc detector) in native helical
+    geometry: views [nt ns na] FP32 and the [nz nx ny] image resident on the
+    device, a helical plan (fdkhip_plan_create_helix: the axial pipeline with
+    per-view source heights and slice windows, EXACT arithmetic), autotuned,
+    replayed from its HIP graph."""
     import torch
-    from mirt_amd import _lib, geom_of
+    from mirt_amd import FdkPlan
     from mirt_amd.cbct_back import _views
     from mirt_amd.cbct_back_helix import helix_operands
     from mirt_amd.workloads import WORKLOADS, filtered_views
 
-    lib = _lib.load()
     name = args.workload if args.workload != "clinical_arc" else "clinical_helix"
     scan = WORKLOADS[name]()
     cg, ig = scan.cg, scan.ig
@@ -585,34 +580,32 @@ def run_helix(args, dev):
     na1_half, dpr = helix_operands(cg, 1)
     mask = np.ascontiguousarray(np.asarray(ig.mask_or).T).astype(np.uint8)
     img = torch.empty((ig.ny, ig.nx, ig.nz), dtype=torch.float32, device=dev)
-    geom = geom_of(cg, ig)
-    s = torch.cuda.current_stream()
-    kms = []
-
-    def step():
-        ms = C.c_float(0)
-        _lib.check(lib.fdkhip_fdk_ts_back_helix_direct(
-            C.byref(geom), C.c_void_p(tsa.data_ptr()), cg.na,
-            beta.ctypes.data_as(C.c_void_p), zs.ctypes.data_as(C.c_void_p), na1_half, 0, 0,
-            dpr, float(cg.pitch), mask.ctypes.data_as(C.c_void_p), C.c_void_p(img.data_ptr()),
-            C.c_void_p(s.cuda_stream), C.byref(ms)))
-        kms.append(ms.value)
-
-    ms = time_steps(step, args.steps, args.warmup)
-    kernel_ms = float(np.mean(kms[args.warmup:]))
+    plan = FdkPlan.helix_from_geoms(cg, ig)
+    if args.launch:
+        plan.set_config(*args.launch)
+    elif not args.no_autotune:
+        t0 = time.perf_counter()
+        plan.autotune(tsa, out=img, reps=2)
+        log(f"helix: autotune {time.perf_counter() - t0:.1f} s -> {plan.config}")
+    cfg = plan.config
+    launch = f"waves={cfg['waves']} zper={cfg['zper']} vstage={cfg['vstage']}"
+    ms = time_steps(lambda: plan(tsa, out=img), args.steps, args.warmup)
+    prof_ms = plan.profile(tsa, out=img, reps=max(2, args.steps))
+    kernel_ms = prof_ms["back"]
     izb = helix_windows(cg, ig, na1_half)
     U = helix_update_count(cg, ig, beta, zs, izb, dev)
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"helix_kernel_{name}_f32.md")
+    prof = os.path.join(ROOT, "profiles", f"back_kernel_{name}_f32.md")
     if os.path.exists(prof):
         text = open(prof).read()
         m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
-        if m and re.search(r"^launch: " + re.escape(HELIX_LAUNCH) + "$", text, re.M):
+        if m and re.search(r"^launch: " + re.escape(launch) + "$", text, re.M):
             traffic = float(m.group(1))
     U_nom = ig.nx * ig.ny * ig.nz * cg.na
     achieved = FLOP_PER_UPDATE * U / (kernel_ms * 1e-3) / 1e12
-    log(f"helix: {ms:.3f} ms/step, kernel {kernel_ms:.3f} ms, {U:.4e} updates "
-        f"({U / ms * 1e3:.3e}/s)")
+    log(f"helix: {ms:.3f} ms/step, back_kernel {kernel_ms:.3f} ms ({launch}), "
+        f"pad {prof_ms['pad']:.3f} ms, {U:.4e} updates ({U / ms * 1e3:.3e}/s); "
+        f"plan counts {plan.count_updates():.4e}")
     result = {
         "metric": f"voxel-view updates/s (in-window updates), {name}, helical fdk,ts,back",
         "value": U / (ms * 1e-3), "unit": "voxel-view updates/s", "n_gpus": 1,
@@ -626,14 +619,15 @@ def run_helix(args, dev):
                    "nx": ig.nx, "ny": ig.ny, "nz": ig.nz, "na": cg.na, "na1_half": na1_half},
         "updates_in_window": U, "updates_nominal": U_nom,
         "gups_nominal": U_nom / (ms * 1e-3) / 1e9,
-        "kernel_ms": kernel_ms,
+        "kernel_ms": kernel_ms, "launch": launch, "pad_ms": prof_ms["pad"],
+        "path": "helical plan (fdkhip_plan_create_helix), EXACT",
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic,
-                     "valu_issue": (profiled_valu_issue(f"helix_kernel_{name}_f32")
+                     "valu_issue": (profiled_valu_issue(f"back_kernel_{name}_f32")
                                     if traffic is not None else None),
                      "note": f"{FLOP_PER_UPDATE} FP32 flop x {U} in-window updates / "
-                             f"{kernel_ms:.3f} ms helix_kernel (HIP events on its stream)"},
+                             f"{kernel_ms:.3f} ms back_kernel (HIP events on its stream)"},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

@@ -138,7 +138,8 @@ int fdkhip_fdk_st_back(
  * gateway does) and pitch (single).  Computes fdk_ts_back_t of
  * mex/src/fdk-helix/mex/fdk,ts,t.c:107-167 with every branch of fdk_ts_back1
  * (fdk,ts.c:55-251), bit for bit.  w3d table entries past the reference's
- * allocation read 0.  Synchronous.
+ * allocation read 0.  Synchronous.  cone_par = 0 runs on a helical plan
+ * (fdkhip_plan_create_helix); cone_par = 1 on the dedicated helical kernel.
  */
 int fdkhip_fdk_ts_back_helix(
 	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
@@ -233,6 +234,21 @@ typedef struct fdkhip_plan fdkhip_plan;
 int fdkhip_plan_create(fdkhip_plan **plan, const fdkhip_geom *g,
 	int32_t na, const double *beta, const uint8_t *mask_host,
 	int32_t fmt, int32_t math);
+
+/*
+ * A plan for a helical scan in native helical geometry (the cone_par = 0
+ * branch of fdk_ts_back1, mex/src/fdk-helix/mex/fdk,ts.c:55-251): view ia
+ * has source height source_zs[ia] (wzbeta = wz + source_zs/dz, :149-151) and,
+ * when pitch != 0, the slice window between the heights of views
+ * ia -+ na1_half (iz_min_beta, iz_max_beta of fdk,ts,t.c:78-85).  Same
+ * execution, tuning and arithmetic modes as fdkhip_plan_create (EXACT is the
+ * reference bit for bit); each z-group of the launch visits only the views
+ * whose window meets it.  fdkhip_fdk_ts_back_helix and its _direct form run
+ * cone_par = 0 calls on such a plan.
+ */
+int fdkhip_plan_create_helix(fdkhip_plan **plan, const fdkhip_geom *g, int32_t na,
+	const double *beta, const double *source_zs, int32_t na1_half, float pitch,
+	const uint8_t *mask_host, int32_t fmt, int32_t math);
 
 /* Backproject device projections proj_dev ([nt ns na], plan's fmt) into the
  * device image image_dev ([nz nx ny] FP32, fully overwritten).  scale_dev is

@@ -79,6 +79,8 @@ _SIGS = {
                                               _P, C.POINTER(C.c_float)]),
     "fdkhip_plan_create": (C.c_int, [C.POINTER(_P), C.POINTER(Geom), C.c_int32, _P, _P,
                                      C.c_int32, C.c_int32]),
+    "fdkhip_plan_create_helix": (C.c_int, [C.POINTER(_P), C.POINTER(Geom), C.c_int32, _P, _P,
+                                           C.c_int32, C.c_float, _P, C.c_int32, C.c_int32]),
     "fdkhip_plan_execute": (C.c_int, [_P, _P, _P, _P, _P]),
     "fdkhip_plan_execute_direct": (C.c_int, [_P, _P, _P, _P, _P]),
     "fdkhip_plan_config": (C.c_int, [_P, _P]),

@@ -394,7 +394,8 @@ class FdkPlan:
     ``image_xyz`` views it as ``[nx ny nz]`` without copying.
     """
 
-    def __init__(self, geom: Geom, beta, mask=None, *, fmt="f32", math_mode="exact"):
+    def __init__(self, geom: Geom, beta, mask=None, *, fmt="f32", math_mode="exact",
+                 helix=None):
         self.lib = _lib.load()
         self.geom = geom
         self.fmt = _lib.FMT_NAMES[fmt] if isinstance(fmt, str) else int(fmt)
@@ -409,8 +410,16 @@ class FdkPlan:
                 raise ValueError("mask must be [nx ny]")
             self._mask = np.asfortranarray(m != 0).astype(np.uint8, order="F")
         h = C.c_void_p()
-        rc = self.lib.fdkhip_plan_create(C.byref(h), C.byref(geom), self.na,
-                                         _ptr(beta), _ptr(self._mask), self.fmt, self.math)
+        if helix is None:
+            rc = self.lib.fdkhip_plan_create(C.byref(h), C.byref(geom), self.na,
+                                             _ptr(beta), _ptr(self._mask), self.fmt, self.math)
+        else:
+            zs, na1_half, pitch = helix
+            self._zs = np.ascontiguousarray(np.broadcast_to(np.asarray(zs, np.float64),
+                                                            (self.na,)))
+            rc = self.lib.fdkhip_plan_create_helix(
+                C.byref(h), C.byref(geom), self.na, _ptr(beta), _ptr(self._zs),
+                int(na1_half), float(pitch), _ptr(self._mask), self.fmt, self.math)
         _lib.check(rc)
         self.handle = h
 
@@ -421,6 +430,18 @@ class FdkPlan:
         rows)."""
         beta = _views(cg, cg.na)[::ia_skip]
         return cls(geom_of(cg, ig, extrapolate_t), beta, ig.mask_or, **kw)
+
+    @classmethod
+    def helix_from_geoms(cls, cg: CTGeom, ig: ImageGeom, *, ia_skip=1, **kw):
+        """Plan for the helical scan ``cg`` in native helical geometry (views
+        1:ia_skip:na at their source heights, the slice windows of
+        cbct_back_helix's na1_half; fdkhip_plan_create_helix)."""
+        from .cbct_back_helix import helix_operands
+        na1_half, _ = helix_operands(cg, ia_skip)
+        beta = _views(cg, cg.na)[::ia_skip]
+        zs = np.broadcast_to(np.asarray(cg.source_zs, np.float64), (cg.na,))[::ia_skip]
+        return cls(geom_of(cg, ig), beta, ig.mask_or,
+                   helix=(zs, na1_half, float(cg.pitch)), **kw)
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -558,6 +579,18 @@ class FdkBatch:
                                                 mask_arr, self.fmt, self.math))
         self.handle = h
         self.nscan = n
+
+    @classmethod
+    def helix_from_geoms(cls, cg: CTGeom, ig: ImageGeom, *, ia_skip=1, **kw):
+        """Plan for the helical scan ``cg`` in native helical geometry (views
+        1:ia_skip:na at their source heights, the slice windows of
+        cbct_back_helix's na1_half; fdkhip_plan_create_helix)."""
+        from .cbct_back_helix import helix_operands
+        na1_half, _ = helix_operands(cg, ia_skip)
+        beta = _views(cg, cg.na)[::ia_skip]
+        zs = np.broadcast_to(np.asarray(cg.source_zs, np.float64), (cg.na,))[::ia_skip]
+        return cls(geom_of(cg, ig), beta, ig.mask_or,
+                   helix=(zs, na1_half, float(cg.pitch)), **kw)
 
     def __del__(self):
         h = getattr(self, "handle", None)

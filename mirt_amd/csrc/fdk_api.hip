@@ -128,7 +128,9 @@ static std::vector<float2> trig_table(const double *beta, int na)
 // any view's detector rows: t_inc = mag*dz/dt with mag in
 // [dsd/(dso + r), dsd/(dso - r)], r = largest distance of a masked column
 // centre from the rotation axis.  Computed in double with a 2-slice margin.
-static void support_z(const fdkhip_geom *g, const uint8_t *mask, int &lo, int &hi)
+// zsh_lo, zsh_hi: range of the source heights in slices (a helix; 0 axial)
+static void support_z(const fdkhip_geom *g, const uint8_t *mask, int &lo, int &hi,
+	double zsh_lo = 0, double zsh_hi = 0)
 {
 	lo = 0;
 	hi = g->nz;
@@ -166,6 +168,8 @@ static void support_z(const fdkhip_geom *g, const uint8_t *mask, int &lo, int &h
 	}
 	if (!isfinite(zmin) || !isfinite(zmax))
 		return;
+	zmin += std::min(zsh_lo, zsh_hi);
+	zmax += std::max(zsh_lo, zsh_hi);
 	lo = (int) std::max(0.0, floor(zmin) - 2);
 	hi = (int) std::min((double) g->nz, ceil(zmax) + 3);
 	if (lo >= hi) {
@@ -190,13 +194,14 @@ struct fdkhip_plan {
 	void *d_padded = nullptr;     // FP32: zero-padded copy of the views (see launch_pad)
 	// survey results per launch shape (nw, zl): maxarea, fov_any, n_updates
 	struct Survey {
-		int nw, zl, maxarea, fov_any;
+		int nw, zl, zr, maxarea, fov_any;
 		long long n_updates;
 	};
 	std::vector<Survey> surveys;
 	size_t padded_bytes = 0;
 	// launch configuration
 	int nw = 8, zl = 16, vstage = 8, slot = 0;
+	int zr = 1;                 // chunks of zl slices per consumer wave
 	int z0 = 0, ngroups = 0, tiles_x = 0, tiles_y = 0;
 	int zs_lo = 0, zs_hi = 0;
 	long long n_updates = -1;
@@ -204,6 +209,16 @@ struct fdkhip_plan {
 	size_t lds_limit = 65536;   // per workgroup, for the current configuration
 	size_t lds_max = 65536;     // device limit per workgroup
 	hipStream_t survey_stream = nullptr;   // stream of the survey pass (set-up)
+	// helical scan (fdk-helix, cone_par = 0): per-view source heights and
+	// slice windows, and per z-group the views whose window meets it
+	bool helix = false;
+	int na1_half = 0;
+	float pitch = 0.f;
+	std::vector<int2> h_izb;
+	float *d_vzs = nullptr;
+	int2 *d_izb = nullptr;
+	int2 *d_gviews = nullptr;
+	int gviews_cap = 0;
 	// graph cache
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t gexec = nullptr;
@@ -236,6 +251,9 @@ static BackArgs make_args(const fdkhip_plan *p, const void *proj, const float *s
 	a.count = p->d_count;
 	a.maxarea = p->d_maxarea;
 	a.fov_any = p->d_any;
+	a.vzs = p->d_vzs;
+	a.izb = p->d_izb;
+	a.gviews = p->helix ? p->d_gviews : nullptr;
 	return a;
 }
 
@@ -296,24 +314,68 @@ static bool choose_layout(int nw, int fmt, int maxarea, int vstage_req, int slot
 	return true;
 }
 
-static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_cap)
+// zper slices per consumer wave: one chunk of zper slices where that shape is
+// built, else chunks of 16 interleaved over the waves (FP32 walk modes)
+static bool split_zper(int nw, int zper, int fmt, int &zl, int &zr)
 {
-	if (!back_config_supported(nw, zl))
-		return fail(FDKHIP_EUNSUPPORTED, "configuration waves=%d zper=%d not built", nw, zl);
+	if (back_config_supported(nw, zper, 1, fmt)) {
+		zl = zper;
+		zr = 1;
+		return true;
+	}
+	if (zper % 16 == 0 && back_config_supported(nw, 16, zper / 16, fmt)) {
+		zl = 16;
+		zr = zper / 16;
+		return true;
+	}
+	return false;
+}
+
+static int configure(fdkhip_plan *p, int nw, int zper, int vstage_req, int slot_cap)
+{
+	int zl = 0, zr = 0;
+	if (!split_zper(nw, zper, p->fmt, zl, zr))
+		return fail(FDKHIP_EUNSUPPORTED, "configuration waves=%d zper=%d not built", nw, zper);
 	p->nw = nw;
 	p->zl = zl;
+	p->zr = zr;
 	p->tiles_x = (p->geom.nx + 7) / 8;
 	p->tiles_y = (p->geom.ny + 7) / 8;
-	const int gz = nw * zl;
+	const int gz = nw * zl * zr;
 	p->z0 = p->zs_lo;
 	p->ngroups = (p->zs_hi > p->zs_lo) ? (p->zs_hi - p->zs_lo + gz - 1) / gz : 0;
 	drop_graph(p);
+	if (p->helix && p->ngroups > 0) {
+		// views [first, last) whose window [izb0, izb1) meets each z-group
+		if (p->ngroups > p->gviews_cap) {
+			(void) hipFree(p->d_gviews);
+			p->d_gviews = nullptr;
+			p->gviews_cap = 0;
+			HIP_TRY(hipMalloc(&p->d_gviews, sizeof(int2) * p->ngroups));
+			p->gviews_cap = p->ngroups;
+		}
+		std::vector<int2> gv((size_t) p->ngroups, make_int2(0, 0));
+		for (int k = 0; k < p->ngroups; ++k) {
+			const int z0 = p->z0 + k * gz, z1 = std::min(z0 + gz, p->geom.nz);
+			int va = -1, vb = -1;
+			for (int ia = 0; ia < p->na; ++ia)
+				if (p->h_izb[ia].x < z1 && p->h_izb[ia].y > z0) {
+					if (va < 0)
+						va = ia;
+					vb = ia + 1;
+				}
+			if (va >= 0)
+				gv[k] = make_int2(va, vb);
+		}
+		HIP_TRY(hipMemcpy(p->d_gviews, gv.data(), sizeof(int2) * p->ngroups,
+			hipMemcpyHostToDevice));
+	}
 	p->maxarea = 0;
 	p->fov_any = 0;
 	p->n_updates = 0;
 	bool cached = false;
 	for (const auto &sv : p->surveys)
-		if (sv.nw == nw && sv.zl == zl) {
+		if (sv.nw == nw && sv.zl == zl && sv.zr == zr) {
 			p->maxarea = sv.maxarea;
 			p->fov_any = sv.fov_any;
 			p->n_updates = sv.n_updates;
@@ -325,7 +387,7 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 		HIP_TRY(launch_zero_i32(p->d_maxarea, 1, ss));
 		HIP_TRY(launch_zero_i32(p->d_any, 1, ss));
 		BackArgs a = make_args(p, nullptr, nullptr, nullptr);
-		HIP_TRY(launch_survey(a, nw, zl, p->fmt != FDKHIP_F32, ss));
+		HIP_TRY(launch_survey(a, nw * zr, zl, p->fmt != FDKHIP_F32, ss));
 		unsigned long long cnt = 0;
 		HIP_TRY(hipMemcpyAsync(&cnt, p->d_count, sizeof cnt, hipMemcpyDeviceToHost, ss));
 		HIP_TRY(hipMemcpyAsync(&p->maxarea, p->d_maxarea, sizeof(int), hipMemcpyDeviceToHost,
@@ -333,13 +395,14 @@ static int configure(fdkhip_plan *p, int nw, int zl, int vstage_req, int slot_ca
 		HIP_TRY(hipMemcpyAsync(&p->fov_any, p->d_any, sizeof(int), hipMemcpyDeviceToHost, ss));
 		HIP_TRY(hipStreamSynchronize(ss));
 		p->n_updates = (long long) cnt;
-		p->surveys.push_back({nw, zl, p->maxarea, p->fov_any, p->n_updates});
+		p->surveys.push_back({nw, zl, zr, p->maxarea, p->fov_any, p->n_updates});
 	}
 	// views per stage and slot size within the LDS budget: 8 consumer waves
 	// (+8 producers) fill a CU with one workgroup, 4 (+4) fit two
 	p->lds_limit = (nw >= 8) ? p->lds_max : p->lds_max / 2;
 	int vstage = 0, slot = 0;
-	if (!choose_layout(nw, p->fmt, p->maxarea, vstage_req, slot_cap, p->lds_limit, vstage, slot))
+	if (!choose_layout(nw * zr, p->fmt, p->maxarea, vstage_req, slot_cap, p->lds_limit, vstage,
+			slot))
 		return fail(FDKHIP_EUNSUPPORTED, "LDS budget too small for waves=%d", nw);
 	p->vstage = vstage;
 	p->slot = slot;
@@ -359,14 +422,27 @@ static int plan_free(fdkhip_plan *p)
 	(void) hipFree(p->d_maxarea);
 	(void) hipFree(p->d_any);
 	(void) hipFree(p->d_padded);
+	(void) hipFree(p->d_vzs);
+	(void) hipFree(p->d_izb);
+	(void) hipFree(p->d_gviews);
 	delete p;
 	return FDKHIP_OK;
 }
 
 static int apply_tuned(fdkhip_plan *p);
 
-extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
-	int32_t na, const double *beta, const uint8_t *mask_host, int32_t fmt, int32_t math)
+constexpr size_t kPadTail = 256;   // zero bytes after a plan's padded copy
+
+// helical scan of a plan: source heights and the window half-width
+struct HelixPlanInfo {
+	const double *source_zs;
+	int na1_half;
+	float pitch;
+};
+
+static int plan_create_impl(fdkhip_plan **out, const fdkhip_geom *g, int32_t na,
+	const double *beta, const uint8_t *mask_host, int32_t fmt, int32_t math,
+	const HelixPlanInfo *hx)
 {
 	if (!out)
 		return fail(FDKHIP_EARG, "plan output pointer is NULL");
@@ -385,12 +461,43 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	if (g->nz > 65535)
 		return fail(FDKHIP_EUNSUPPORTED, "nz=%d: at most 65535 slices per volume", g->nz);
 
+	if (hx && na > 0 && !hx->source_zs)
+		return fail(FDKHIP_EARG, "source_zs is NULL");
+	if (hx && hx->na1_half < 0)
+		return fail(FDKHIP_EARG, "na1_half must be >= 0");
+
 	fdkhip_plan *p = new fdkhip_plan;
 	p->geom = *g;
 	p->na = na;
 	p->fmt = fmt;
 	p->math = (fmt == FDKHIP_F32) ? math : FDKHIP_MATH_FAST;
 	p->gf = narrow_geom(g, na);
+	double zs_lo = 0, zs_hi = 0;   // source height range in slices
+	std::vector<float> vzs;
+	if (hx) {
+		// fdk,ts,t.c:70-85: heights narrowed to float; the window of view ia
+		// spans the slices between the heights of views ia -+ na1_half
+		p->helix = true;
+		p->na1_half = hx->na1_half;
+		p->pitch = hx->pitch;
+		vzs.resize((size_t) std::max(na, 1), 0.f);
+		p->h_izb.assign((size_t) std::max(na, 1), make_int2(0, g->nz));
+		const float wz = p->gf.wz, dz = p->gf.dz;
+		for (int ia = 0; ia < na; ++ia) {
+			vzs[ia] = (float) hx->source_zs[ia];
+			if (hx->pitch != 0.f) {
+				const int a0 = std::max(ia - hx->na1_half, 0);
+				const int a1 = std::min(ia + hx->na1_half, na - 1);
+				const float c0 = ceilf((float) (wz + hx->source_zs[a0] / dz));
+				const float c1 = ceilf((float) (wz + hx->source_zs[a1] / dz));
+				p->h_izb[ia] = make_int2((int) (c0 > 0 ? c0 : 0),
+					(int) (c1 < g->nz ? c1 : g->nz));
+			}
+			const double zsl = hx->source_zs[ia] / g->dz;
+			zs_lo = ia ? std::min(zs_lo, zsl) : zsl;
+			zs_hi = ia ? std::max(zs_hi, zsl) : zsl;
+		}
+	}
 	if (hipGetDevice(&p->device) != hipSuccess) {
 		delete p;
 		return fail(FDKHIP_EHIP, "no current HIP device");
@@ -414,6 +521,13 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 		std::vector<float2> trig = trig_table(beta, na);
 		PLAN_TRY(hipMalloc(&p->d_trig, sizeof(float2) * na));
 		PLAN_TRY(hipMemcpy(p->d_trig, trig.data(), sizeof(float2) * na, hipMemcpyHostToDevice));
+		if (hx) {
+			PLAN_TRY(hipMalloc(&p->d_vzs, sizeof(float) * na));
+			PLAN_TRY(hipMemcpy(p->d_vzs, vzs.data(), sizeof(float) * na, hipMemcpyHostToDevice));
+			PLAN_TRY(hipMalloc(&p->d_izb, sizeof(int2) * na));
+			PLAN_TRY(hipMemcpy(p->d_izb, p->h_izb.data(), sizeof(int2) * na,
+				hipMemcpyHostToDevice));
+		}
 	}
 	const long nxy = (long) g->nx * g->ny;
 	if (mask_host && nxy > 0) {
@@ -426,14 +540,18 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	{
 		// FP32: [na][ns+2][nt+2] floats; FP16/BF16 views are staged straight
 		// from the caller's buffer and need no copy
+		// plus a zero tail: row nt + 2 of the last padded column (read, as
+		// a zero row, by slices at t_bin >= nt) lies just past the copy
 		p->padded_bytes = (fmt == FDKHIP_F32)
 			? (size_t) na * (g->ns + 2) * (g->nt + 2) * sizeof(float) : 0;
-		if (p->padded_bytes > 0)
-			PLAN_TRY(hipMalloc(&p->d_padded, p->padded_bytes));
+		if (p->padded_bytes > 0) {
+			PLAN_TRY(hipMalloc(&p->d_padded, p->padded_bytes + kPadTail));
+			PLAN_TRY(hipMemset((char *) p->d_padded + p->padded_bytes, 0, kPadTail));
+		}
 	}
 #undef PLAN_TRY
 
-	support_z(g, mask_host, p->zs_lo, p->zs_hi);
+	support_z(g, mask_host, p->zs_lo, p->zs_hi, zs_lo, zs_hi);
 	// default: 8 waves x 16 slices = 128-slice z-groups (4 x 16 when the
 	// support is short)
 	int nw = 8, zl = 16;
@@ -455,6 +573,20 @@ extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
 	return FDKHIP_OK;
 }
 
+extern "C" int fdkhip_plan_create(fdkhip_plan **out, const fdkhip_geom *g,
+	int32_t na, const double *beta, const uint8_t *mask_host, int32_t fmt, int32_t math)
+{
+	return plan_create_impl(out, g, na, beta, mask_host, fmt, math, nullptr);
+}
+
+extern "C" int fdkhip_plan_create_helix(fdkhip_plan **out, const fdkhip_geom *g,
+	int32_t na, const double *beta, const double *source_zs, int32_t na1_half, float pitch,
+	const uint8_t *mask_host, int32_t fmt, int32_t math)
+{
+	const HelixPlanInfo hx{source_zs, na1_half, pitch};
+	return plan_create_impl(out, g, na, beta, mask_host, fmt, math, &hx);
+}
+
 extern "C" int fdkhip_plan_destroy(fdkhip_plan *p)
 {
 	return plan_free(p);
@@ -465,7 +597,7 @@ extern "C" int fdkhip_plan_config(const fdkhip_plan *p, int32_t cfg[6])
 	if (!p || !cfg)
 		return fail(FDKHIP_EARG, "NULL argument");
 	cfg[0] = p->nw;
-	cfg[1] = p->zl;
+	cfg[1] = p->zl * p->zr;
 	cfg[2] = p->vstage;
 	cfg[3] = p->z0;
 	cfg[4] = p->ngroups;
@@ -500,7 +632,7 @@ static int plan_launch(fdkhip_plan *p, const void *proj, const float *scale, flo
 	hipStream_t s, hipEvent_t *ev = nullptr)
 {
 	const long ncol = (long) p->geom.nx * p->geom.ny;
-	const int zend = std::min(p->z0 + p->ngroups * p->nw * p->zl, p->geom.nz);
+	const int zend = std::min(p->z0 + p->ngroups * p->nw * p->zl * p->zr, p->geom.nz);
 	const int zbeg = p->ngroups > 0 ? p->z0 : 0;
 	const bool work = p->ngroups > 0 && ncol > 0 && p->na > 0;
 	if (ev)
@@ -516,7 +648,7 @@ static int plan_launch(fdkhip_plan *p, const void *proj, const float *scale, flo
 		HIP_TRY(hipEventRecord(ev[2], s));
 	if (work) {
 		BackArgs a = make_args(p, proj, scale, image);
-		HIP_TRY(launch_back(a, p->nw, p->zl, p->fmt, p->math == FDKHIP_MATH_FAST, s));
+		HIP_TRY(launch_back(a, p->nw, p->zl, p->zr, p->fmt, p->math == FDKHIP_MATH_FAST, s));
 	}
 	if (ev)
 		HIP_TRY(hipEventRecord(ev[3], s));
@@ -1102,6 +1234,36 @@ static int helix_run(HelixArgs &a, const HelixHost &h, const uint8_t *mask_host,
 	return FDKHIP_OK;
 }
 
+// Native helical geometry (cone_par = 0) runs on the axial pipeline: a
+// helical plan (per-view heights and windows) for this one call.  kernel_ms:
+// the plan's device time (fill, padded copy, backprojection).
+static int helix_plan_run(const fdkhip_geom *g, const float *proj_dev, int na,
+	const double *beta, const double *source_zs, int na1_half, float pitch,
+	const uint8_t *mask, float *image_dev, hipStream_t s, float *kernel_ms)
+{
+	fdkhip_plan *p = nullptr;
+	int rc = fdkhip_plan_create_helix(&p, g, na, beta, source_zs, na1_half, pitch, mask,
+		FDKHIP_F32, FDKHIP_MATH_EXACT);
+	if (rc)
+		return rc;
+	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+	if (kernel_ms)
+		for (auto &e : ev)
+			if (!rc && hipEventCreate(&e) != hipSuccess)
+				rc = fail(FDKHIP_EHIP, "hipEventCreate");
+	if (!rc)
+		rc = plan_launch(p, proj_dev, nullptr, image_dev, s, kernel_ms ? ev : nullptr);
+	if (!rc && hipStreamSynchronize(s) != hipSuccess)
+		rc = fail(FDKHIP_EHIP, "helical backprojection: %s", hipGetErrorString(hipGetLastError()));
+	if (!rc && kernel_ms && hipEventElapsedTime(kernel_ms, ev[0], ev[3]) != hipSuccess)
+		rc = fail(FDKHIP_EHIP, "hipEventElapsedTime");
+	for (auto &e : ev)
+		if (e)
+			(void) hipEventDestroy(e);
+	plan_free(p);
+	return rc;
+}
+
 extern "C" int fdkhip_fdk_ts_back_helix(
 	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
 	const uint8_t *mask, double dso, double dsd, double dfs,
@@ -1139,6 +1301,10 @@ extern "C" int fdkhip_fdk_ts_back_helix(
 	if (nproj == 0) {
 		HIP_TRY(launch_fill(a.image, (long) g.nx * g.ny, g.nz, 0, 0, nullptr));
 		HIP_TRY(hipDeviceSynchronize());
+	} else if (!cone_par) {
+		if ((rc = helix_plan_run(&g, a.proj, na, beta, source_zs, na1_half, pitch, mask,
+				a.image, nullptr, nullptr)))
+			return rc;
 	} else if ((rc = helix_run(a, h, mask, nullptr, nullptr))) {
 		return rc;
 	}
@@ -1176,6 +1342,9 @@ extern "C" int fdkhip_fdk_ts_back_helix_direct(const fdkhip_geom *g, const float
 		HIP_TRY(hipStreamSynchronize(s));
 		return FDKHIP_OK;
 	}
+	if (!cone_par)
+		return helix_plan_run(g, proj_dev, na, beta, source_zs, na1_half, pitch, mask,
+			image_dev, s, kernel_ms);
 	return helix_run(a, h, mask, s, kernel_ms);
 }
 
@@ -1431,13 +1600,17 @@ extern "C" int fdkhip_cbct_back_helix_mat(const fdkhip_geom *g, const void *proj
 
 namespace {
 
-// launch shapes (consumer waves, slices per lane) the library is built with
+// launch shapes (consumer waves, slices per lane) the library is built with;
+// a plan's FP32 walk modes also have shapes of several interleaved 16-slice
+// chunks per wave (4 x 48, 4 x 64, 8 x 32), the batch kernel does not
 const int kShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}};
+const int kPlanShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}, {4, 48}, {4, 64},
+	{8, 32}};
 const int kStages[] = {0, 3, 4, 6, 8, 12, 16};
 const int kTuneViews = 128;   // views per timed launch while tuning
 
 struct Tuned {
-	int nw, zl, vstage;
+	int nw, zl, vstage;   // zl: slices per consumer wave (the configuration's zper)
 };
 
 std::mutex g_tune_mu;
@@ -1447,9 +1620,11 @@ std::string tune_key(const fdkhip_plan *p)
 {
 	char buf[512];
 	const fdkhip_geom &g = p->geom;
-	snprintf(buf, sizeof buf, "%d %d %d %a %a %a %a %a %a %a %a %a %d %d %a %a %a %a %d %d %d",
+	snprintf(buf, sizeof buf,
+		"%d %d %d %a %a %a %a %a %a %a %a %a %d %d %a %a %a %a %d %d %d helix %d %d %a",
 		g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.offset_x, g.offset_y, g.offset_z, g.dso, g.dsd,
-		g.dfs, g.ns, g.nt, g.ds, g.dt, g.offset_s, g.offset_t, p->na, p->fmt, p->math);
+		g.dfs, g.ns, g.nt, g.ds, g.dt, g.offset_s, g.offset_t, p->na, p->fmt, p->math,
+		(int) p->helix, p->na1_half, (double) p->pitch);
 	return buf;
 }
 
@@ -1516,14 +1691,14 @@ extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const floa
 	if (reps < 1)
 		return fail(FDKHIP_EARG, "reps must be >= 1");
 	hipStream_t s = (hipStream_t) stream;
-	Tuned best{p->nw, p->zl, p->vstage};
+	Tuned best{p->nw, p->zl * p->zr, p->vstage};
 	float best_t = 3.4e38f;
 	// the padded views once; then the backprojection kernel alone over the
 	// first kTuneViews views (its time is proportional to the view count)
 	rc = plan_launch(p, proj, scale, image, s);
 	if (rc)
 		return rc;
-	for (const auto &sh : kShapes) {
+	for (const auto &sh : kPlanShapes) {
 		int last_vstage = -1;
 		for (int vs : kStages) {
 			if (configure(p, sh[0], sh[1], vs, 0) != FDKHIP_OK)
@@ -1537,7 +1712,8 @@ extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const floa
 			rc = time_launches([&](hipStream_t st) {
 				BackArgs a = make_args(p, proj, scale, image);
 				a.g.na = std::min(p->na, kTuneViews);
-				HIP_TRY(launch_back(a, p->nw, p->zl, p->fmt, p->math == FDKHIP_MATH_FAST, st));
+				HIP_TRY(launch_back(a, p->nw, p->zl, p->zr, p->fmt, p->math == FDKHIP_MATH_FAST,
+					st));
 				return FDKHIP_OK;
 			}, s, reps, &t);
 			if (rc)
@@ -1680,7 +1856,7 @@ static int batch_half_zper(int zl)
 // chosen again for the heights kept.
 static int batch_configure(fdkhip_batch *b, int nw, int zl, int vstage_req)
 {
-	if (!back_config_supported(nw, zl))
+	if (!back_config_supported(nw, zl, 1, b->fmt))
 		return fail(FDKHIP_EUNSUPPORTED, "configuration waves=%d zper=%d not built", nw, zl);
 	const int zh = batch_half_zper(zl);
 	if (!b->zper_req.empty())
@@ -1893,7 +2069,7 @@ static int batch_launch(fdkhip_batch *b, const void *const *proj, const float *c
 		if (ncol == 0 || p->geom.nz == 0)
 			continue;
 		const bool work = p->ngroups > 0 && p->na > 0;
-		const int zend = std::min(p->z0 + p->ngroups * p->nw * p->zl, p->geom.nz);
+		const int zend = std::min(p->z0 + p->ngroups * p->nw * p->zl * p->zr, p->geom.nz);
 		HIP_TRY(launch_fill(image[i], ncol, p->geom.nz, work ? p->z0 : 0, work ? zend : 0, s));
 		if (work && p->fmt == FDKHIP_F32)
 			HIP_TRY(launch_pad(static_cast<const float *>(proj[i]),

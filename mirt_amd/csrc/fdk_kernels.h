@@ -36,6 +36,10 @@ struct BackArgs {
 	unsigned long long *count;   // survey: reference loop iterations
 	int *maxarea;            // survey: largest staged footprint (floats)
 	int *fov_any;            // survey: some column meets some view
+	// helical scans (fdk-helix, cone_par = 0); NULL for an axial scan
+	const float *vzs;        // [na] source_zs narrowed to float
+	const int2 *izb;         // [na] iz_min_beta, iz_max_beta (fdk,ts,t.c:78-85)
+	const int2 *gviews;      // [ngroups] views [first, last) whose window meets the z-group
 };
 
 // a batch of scans for the persistent kernel: scans[i] holds scan i's
@@ -104,9 +108,12 @@ struct MatArgs {
 };
 
 size_t back_lds_bytes(int nw, int vstage, int slot, int fmt);
-bool back_config_supported(int nw, int zl);
-hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s);
-hipError_t launch_survey(const BackArgs &a, int nw, int zl, bool direct, hipStream_t s);
+// nw consumer waves, zl slices per chunk, zr chunks per wave (interleaved;
+// zr > 1: FP32 walk modes only); the survey takes the z-group's chunk count
+bool back_config_supported(int nw, int zl, int zr, int fmt);
+hipError_t launch_back(const BackArgs &a, int nw, int zl, int zr, int fmt, bool fast,
+	hipStream_t s);
+hipError_t launch_survey(const BackArgs &a, int nchunk, int zl, bool direct, hipStream_t s);
 hipError_t launch_batch(const BatchArgs &b, int nwg, int nw, int zl, int vstage, int slot,
 	int fmt, bool fast, hipStream_t s);
 // resident persistent workgroups per CU for this shape and LDS layout

@@ -80,9 +80,32 @@ struct ColSetup {
 // SETUP_IDLE when it meets it but reads no sample, SETUP_ACTIVE otherwise.
 enum { SETUP_OUT = 0, SETUP_ACTIVE = 1, SETUP_IDLE = 2 };
 
-__device__ __forceinline__ int column_setup(const GeomF &g, float xx, float yy,
-	float cosb, float sinb, ColSetup &c)
+// A view's geometry: the trig of its angle and, on a helix, its source
+// height as wzbeta = wz + source_zs / dz (fdk,ts.c:149-151, cone_par = 0) and
+// its slice window [izb0, izb1) (fdk,ts,t.c:78-85); an axial view has wz and
+// the whole volume
+struct ViewGeo {
+	float cosb, sinb, wzb;
+	int izb0, izb1;
+};
+
+__device__ __forceinline__ ViewGeo view_geo(const BackArgs &a, int v)
 {
+	const float2 cs = a.trig[v];
+	ViewGeo vg{cs.x, cs.y, a.g.wz, 0, a.g.nz};
+	if (a.vzs) {
+		vg.wzb = a.g.wz + a.vzs[v] / a.g.dz;
+		const int2 w = a.izb[v];
+		vg.izb0 = w.x;
+		vg.izb1 = w.y;
+	}
+	return vg;
+}
+
+__device__ __forceinline__ int column_setup(const GeomF &g, float xx, float yy,
+	const ViewGeo &vg, ColSetup &c)
+{
+	const float cosb = vg.cosb, sinb = vg.sinb, wzb = vg.wzb;
 	const float xbeta = xx * cosb + yy * sinb;
 	const float ybetas = g.dso - (-xx * sinb + yy * cosb);
 	const float mag = (g.kind == KIND_PAR) ? 1.f : g.dsd / ybetas;
@@ -104,11 +127,11 @@ __device__ __forceinline__ int column_setup(const GeomF &g, float xx, float yy,
 	// footprint bounds below rely on t_bin increasing along z.
 	if (!(c.t_inc > 0.f) || c.t_inc == INFINITY)
 		return SETUP_IDLE;
-	int iz_min = to_int_x86(floorf(g.wz - (g.wt + 1.f) / c.t_inc) + 1.f);
-	int iz_max = to_int_x86(ceilf(g.wz + ((float) g.nt - g.wt) / c.t_inc) - 1.f + 1.f);
-	iz_min = max(iz_min, 0);
-	iz_max = min(iz_max, g.nz);
-	float t_bin = c.t_inc * ((float) iz_min - g.wz) + g.wt;
+	int iz_min = to_int_x86(floorf(wzb - (g.wt + 1.f) / c.t_inc) + 1.f);
+	int iz_max = to_int_x86(ceilf(wzb + ((float) g.nt - g.wt) / c.t_inc) - 1.f + 1.f);
+	iz_min = max(max(iz_min, 0), vg.izb0);
+	iz_max = min(min(iz_max, g.nz), vg.izb1);
+	float t_bin = c.t_inc * ((float) iz_min - wzb) + g.wt;
 	if (t_bin == -1.f) {
 		++iz_min;
 		t_bin += c.t_inc;
@@ -127,8 +150,9 @@ __device__ __forceinline__ int column_setup(const GeomF &g, float xx, float yy,
 // of divisions, float atan2 for the arc detector, FMA.  iz_min/iz_max are
 // only a first guess here; setup_direct settles the slice range exactly.
 __device__ __forceinline__ int column_setup_fast(const GeomF &g, float xx, float yy,
-	float cosb, float sinb, ColSetup &c)
+	const ViewGeo &vg, ColSetup &c)
 {
+	const float cosb = vg.cosb, sinb = vg.sinb, wzb = vg.wzb;
 	const float xbeta = fmaf(xx, cosb, yy * sinb);
 	const float ybetas = g.dso - fmaf(-xx, sinb, yy * cosb);
 	const float rb = __builtin_amdgcn_rcpf(ybetas);
@@ -145,11 +169,11 @@ __device__ __forceinline__ int column_setup_fast(const GeomF &g, float xx, float
 	if (!(c.t_inc > 0.f) || c.t_inc == INFINITY)
 		return SETUP_IDLE;
 	const float rti = __builtin_amdgcn_rcpf(c.t_inc);
-	const float zmin = floorf(fmaf(-(g.wt + 1.f), rti, g.wz)) + 1.f;
-	const float zmax = ceilf(fmaf((float) g.nt - g.wt, rti, g.wz));
-	c.iz_min = (int) fminf(fmaxf(zmin, -1.f), (float) g.nz + 1.f);
-	c.iz_max = (int) fminf(fmaxf(zmax, -1.f), (float) g.nz + 1.f);
-	c.t_bin = fmaf(c.t_inc, (float) c.iz_min - g.wz, g.wt);
+	const float zmin = floorf(fmaf(-(g.wt + 1.f), rti, wzb)) + 1.f;
+	const float zmax = ceilf(fmaf((float) g.nt - g.wt, rti, wzb));
+	c.iz_min = max((int) fminf(fmaxf(zmin, -1.f), (float) g.nz + 1.f), vg.izb0);
+	c.iz_max = min((int) fminf(fmaxf(zmax, -1.f), (float) g.nz + 1.f), vg.izb1);
+	c.t_bin = fmaf(c.t_inc, (float) c.iz_min - wzb, g.wt);
 	if (g.kind == KIND_ARC)
 		c.w2 = (g.dsd * g.dsd) * __builtin_amdgcn_rcpf(fmaf(ybetas, ybetas, xbeta * xbeta));
 	else
@@ -163,8 +187,9 @@ __device__ __forceinline__ int column_setup_fast(const GeomF &g, float xx, float
 // sequence of additions.  Records, for each of the NW chunks of ZL slices,
 // t_bin at the chunk's first active slice, and returns the effective range
 // [zlo, zhi) with t_bin at zlo (exact) and an upper bound of t_bin at zhi-1.
-// Slices whose t_bin is <= -1 or >= nt are dropped: every sample they would
-// read is a zero pad, so the reference adds exactly 0 there.
+// Leading slices whose t_bin is <= -1 are dropped: every sample they would
+// read is a zero pad, so the reference adds exactly 0 there; trailing ones
+// at t_bin >= nt are kept and read zero rows (see the bound below).
 template <int NW, int ZL>
 __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 	int zg0, int zg1, float (&Tk)[NW], int &zlo, int &zhi,
@@ -173,9 +198,22 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 	const int lo = max(c.iz_min, zg0), hi = min(c.iz_max, zg1);
 	const float ti = c.t_inc;
 	float t = c.t_bin;
-	int z = c.iz_min;
-	for (; z < lo; ++z)           // z-groups above the first: walk up to zg0
-		t += ti;
+	// z-groups above the first: walk up to zg0, eight additions a step --
+	// plain while every lane has eight left, the last steps masked per lane
+	int n = lo - c.iz_min;
+	while (__all(n >= 8)) {
+#pragma unroll
+		for (int j = 0; j < 8; ++j)
+			t += ti;
+		n -= 8;
+	}
+	while (__any(n > 0)) {
+#pragma unroll
+		for (int j = 0; j < 8; ++j)
+			t = (j < n) ? t + ti : t;
+		n -= 8;
+	}
+	int z = lo;
 	while (z < hi && t <= -1.f) { // rare: leading zero-contribution slices
 		t += ti;
 		++z;
@@ -190,47 +228,43 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 		t_last = t;
 		return false;
 	}
-	// chunk holding zlo, and the additions up to the next chunk boundary
-	const int k0 = (zlo - zg0) / ZL;
-	const int n0 = zg0 + (k0 + 1) * ZL - zlo;     // 1..ZL
+	// Tk[k] = t_bin at s_k = max(zlo, zg0 + k ZL), the lane's first slice in
+	// chunk k: s_k - s_(k-1) more additions (0..ZL) from chunk k-1's value.
+	// Uniform control flow: ZL plain additions where every lane needs them,
+	// masked ones where some do (adding nothing leaves the value exact);
+	// lanes past their range add harmlessly.
 	float tt = t;
-#pragma unroll
-	for (int k = 0; k < NW; ++k)
-		if (k == k0)
-			Tk[k] = t;
-#pragma unroll
-	for (int j = 0; j < ZL; ++j)
-		tt = (j < n0) ? tt + ti : tt;              // adding nothing leaves tt exact
-	// later chunks: ZL additions each (lanes past their range add harmlessly)
+	int sp = zlo;
+	Tk[0] = t;
 #pragma unroll
 	for (int k = 1; k < NW; ++k) {
-		if (__all(zhi <= zg0 + k * ZL))
+		const int ck = zg0 + k * ZL;
+		if (__all(zhi <= ck))
 			break;                                 // no active lane reaches chunk k
-		if (k > k0) {
-			Tk[k] = tt;
+		const int sk = max(zlo, ck);
+		const int nk = sk - sp;
+		if (__all(nk == ZL)) {
 #pragma unroll
 			for (int j = 0; j < ZL; ++j)
 				tt += ti;
+		} else if (__any(nk > 0)) {
+#pragma unroll
+			for (int j = 0; j < ZL; ++j)
+				tt = (j < nk) ? tt + ti : tt;
 		}
+		Tk[k] = tt;
+		sp = sk;
 	}
 	// upper bound of t_bin at zhi - 1: m additions of ti each round to within
-	// half an ulp of the running value (|t| < 2^24 here)
+	// half an ulp of the running value (|t| < 2^24 here).  The reference's
+	// slice range keeps t < nt up to that rounding, so t_last < nt + 1: a
+	// trailing slice at t >= nt reads padded rows nt + 1 and nt + 2, both
+	// zero (row nt + 2 of a padded column is row 0 of the next, and the
+	// padded copy ends in zeros), and adds exactly 0, as in the reference.
 	const float m = (float) (zhi - 1 - zlo);
 	const float est = fmaf(m, ti, t_first);
 	const float mag = fmaxf(fmaxf(fabsf(t_first), fabsf(est)), 1.f);
 	t_last = est + (m + 1.f) * mag * 1.2e-7f + 1e-6f;
-	if (t_last >= (float) g.nt) {
-		// maybe a trailing zero-contribution slice: exact walk to trim it
-		float t2 = t_first, tl = t_first;
-		int z2 = zlo;
-		while (z2 < zhi && t2 < (float) g.nt) {
-			tl = t2;
-			t2 += ti;
-			++z2;
-		}
-		zhi = z2;
-		t_last = tl;
-	}
 	return zlo < zhi;
 }
 
@@ -289,14 +323,14 @@ struct ColRange {
 };
 
 template <int NW, int ZL, bool DIRECT>
-__device__ __forceinline__ bool col_range(const GeomF &g, float xx, float yy, float2 cs,
+__device__ __forceinline__ bool col_range(const GeomF &g, float xx, float yy, const ViewGeo &vg,
 	int zg0, int zg1, ColRange &r, float (&Tk)[NW])
 {
 	r.zlo = r.zhi = 0;
 	r.first = r.last = 0.f;
 	r.u_hi = 0.f;
 	if constexpr (DIRECT) {
-		if (column_setup_fast(g, xx, yy, cs.x, cs.y, r.c) != SETUP_ACTIVE)
+		if (column_setup_fast(g, xx, yy, vg, r.c) != SETUP_ACTIVE)
 			return false;
 		if (!setup_direct<NW, ZL>(g, r.c, zg0, zg1, Tk, r.zlo, r.zhi, r.first, r.last))
 			return false;
@@ -307,7 +341,7 @@ __device__ __forceinline__ bool col_range(const GeomF &g, float xx, float yy, fl
 		r.row_lo = (r.zlo > zg0) ? 0 : (int) fminf(fmaxf(r.first, 0.f), r.u_hi);
 		r.row_hi = (int) r.u_hi + 1;
 	} else {
-		if (column_setup(g, xx, yy, cs.x, cs.y, r.c) != SETUP_ACTIVE)
+		if (column_setup(g, xx, yy, vg, r.c) != SETUP_ACTIVE)
 			return false;
 		if (!setup_walk<NW, ZL>(g, r.c, zg0, zg1, Tk, r.zlo, r.zhi, r.first, r.last))
 			return false;
@@ -408,7 +442,7 @@ __device__ __forceinline__ Footprint view_footprint(const GeomF &g, bool act, co
 	f.s_lo = max(wave_min(act ? cr.c.is + 1 : 0x7fffffff), 0);
 	f.s_hi = min(wave_max(act ? cr.c.is + 2 : (int) 0x80000000), g.ns + 1);
 	f.t_lo = max(wave_min(act ? cr.row_lo : 0x7fffffff), 0);
-	f.t_hi = min(wave_max(act ? cr.row_hi : (int) 0x80000000), DIRECT ? g.nt + 2 : g.nt + 1);
+	f.t_hi = min(wave_max(act ? cr.row_hi : (int) 0x80000000), g.nt + 2);
 	return f;
 }
 
@@ -493,7 +527,7 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 		ColRange cr{};
 		float Tk[NW] = {};
 		const bool act = colvalid
-			&& col_range<NW, ZL, DIRECT>(g, xx, yy, a.trig[v], zg0, zg1, cr, Tk);
+			&& col_range<NW, ZL, DIRECT>(g, xx, yy, view_geo(a, v), zg0, zg1, cr, Tk);
 		const ColSetup &c = cr.c;
 		const Footprint f = view_footprint<DIRECT>(g, act, cr);
 		int mode = MODE_NONE, tw = 0, sw = 0;
@@ -693,25 +727,39 @@ __device__ __forceinline__ uint32_t padded16(const uint16_t *view, const GeomF &
 // [zc, zc + ZL) of its column), in view order (EXACT reproduces the
 // reference's sum order).  The next view's record is read while the current
 // one is added, so the per-view LDS round trips overlap the slice arithmetic.
-template <int NW, int ZL, int FMT, bool FAST>
-__device__ __forceinline__ void consume_walk(const BackArgs &a, int bo, const Lds<NW, true> &L,
-	int v0, int nv, int wave, int lane, int zc, float (&acc)[ZL])
+// R chunks per consumer wave: chunk c of the z-group (ZL slices from
+// zg0 + c ZL) belongs to wave c % NW, as its accumulators acc[c / NW]; the
+// chunks of a wave are NW ZL slices apart, so that a helical view, which
+// covers only part of the group, still gives every wave work.
+template <int NW, int R, int ZL, int FMT, bool FAST>
+__device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
+	const Lds<NW * R, true> &L, int v0, int nv, int wave, int lane, int zg0,
+	float (&accs)[R][ZL])
 {
+	constexpr int NC = NW * R;
 	const GeomF &g = a.g;
 	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
 	const float *Tw = fdk_lds + bo + L.T + wave * 64 + lane;
 	const int2 *VI = reinterpret_cast<const int2 *>(fdk_lds + bo + L.vinfo + 2);
 	float4 np0 = P4[0], np1 = P4[1];
-	float nt0 = Tw[0];
+	float nts[R];
+#pragma unroll
+	for (int r = 0; r < R; ++r)
+		nts[r] = Tw[r * NW * 64];
 	int2 nvi = VI[0];
 	for (int vv = 0; vv < nv; ++vv) {
 		const float4 p0 = np0, p1 = np1;
 		const int2 vi = nvi;
-		const float t0 = nt0;      // t_bin at the lane's first active slice of this chunk
+		float ts[R];               // t_bin at the lane's first active slice of each chunk
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			ts[r] = nts[r];
 		if (vv + 1 < nv) {
 			np0 = P4[(vv + 1) * 128];
 			np1 = P4[(vv + 1) * 128 + 1];
-			nt0 = Tw[(vv + 1) * NW * 64];
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				nts[r] = Tw[((vv + 1) * NC + r * NW) * 64];
 			nvi = VI[(vv + 1) * 2];
 		}
 		const int mode = __builtin_amdgcn_readfirstlane(vi.y) >> 16;
@@ -719,83 +767,100 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo, const Ld
 			continue;
 		const int zlohi = __float_as_int(p1.y);
 		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
-		const int jlo = max(zlo - zc, 0);
-		const int jhi = min(zhi - zc, ZL);
-		if (!__any(jlo < jhi))
-			continue;                  // no lane of this wave meets the view
-		const int cb = __float_as_int(p0.x);
-		const float wl = p0.y, wr = p0.z, w2 = p0.w;
-		const float ti = p1.x;
-		const unsigned nj = (unsigned) max(jhi - jlo, 0);
-		if (mode == MODE_LDS) {
-			const int c1 = bo + cb;
-			const int tw = __builtin_amdgcn_readfirstlane(vi.x);
-			const int c2 = c1 + tw;
-			if (__all(jlo == 0 && jhi == ZL)) {
-				// every lane covers the whole chunk: straight-line slices
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			float (&acc)[ZL] = accs[r];
+			const float t0 = ts[r];
+			const int zc = zg0 + (wave + r * NW) * ZL;
+			const int jlo = max(zlo - zc, 0);
+			const int jhi = min(zhi - zc, ZL);
+			if (!__any(jlo < jhi))
+				continue;                  // no lane of this wave meets the view here
+			const int cb = __float_as_int(p0.x);
+			const float wl = p0.y, wr = p0.z, w2 = p0.w;
+			const float ti = p1.x;
+			const unsigned nj = (unsigned) max(jhi - jlo, 0);
+			if (mode == MODE_LDS) {
+				const int c1 = bo + cb;
+				const int tw = __builtin_amdgcn_readfirstlane(vi.x);
+				const int c2 = c1 + tw;
+				if (__all(jlo == 0 && jhi == ZL)) {
+					// every lane covers the whole chunk: straight-line slices
+					float t = t0;
+#pragma unroll
+					for (int j = 0; j < ZL; ++j) {
+						acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
+						t += ti;
+					}
+				} else {
+					// partial chunk, in groups of GS slices: a group every lane
+					// adds runs straight (t of each lane is then at the group's
+					// first slice); in a group some lanes add, slices outside
+					// [jlo, jhi) read inside the staged footprint and are
+					// discarded by a select; a group no lane adds is skipped (t
+					// advances only on added slices).  Several chunks per wave
+					// (helical windows cut them often) take finer groups.
+					constexpr int GS = R > 1 ? 4 : 8;
+					float t = t0;
+#pragma unroll
+					for (int g0 = 0; g0 < ZL; g0 += GS) {
+						if (!__any(jlo < g0 + GS && jhi > g0 && jlo < jhi))
+							continue;
+						if (R > 1 && __all(jlo <= g0 && jhi >= g0 + GS)) {
+#pragma unroll
+							for (int j = g0; j < g0 + GS; ++j) {
+								acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
+								t += ti;
+							}
+							continue;
+						}
+#pragma unroll
+						for (int j = g0; j < g0 + GS; ++j) {
+							const bool rr = (unsigned) (j - jlo) < nj;
+							const float tv = rr ? t : t0;
+							const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
+							acc[j] = rr ? up : acc[j];
+							t = rr ? t + ti : t;
+						}
+					}
+				}
+			} else {
+				// footprint larger than the LDS slot: gather from the padded copy
+				// in HBM, every slice computed (no per-lane branches, so the
+				// accumulators stay in place), the ones outside [jlo, jhi) read at
+				// t0, inside the padded view, and discarded by a select
+				const int is = cb;                    // the record holds is for these views
+				const int pnt = g.nt + 2;
+				const size_t col = ((size_t) (v0 + vv) * (g.ns + 2) + is + 1) * pnt;
+				const size_t c1 = col + 1, c2 = c1 + pnt;  // (s = is, t = 0), (is+1, 0)
 				float t = t0;
 #pragma unroll
 				for (int j = 0; j < ZL; ++j) {
-					acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
-					t += ti;
-				}
-			} else {
-				// partial chunk: slices outside [jlo, jhi) read inside the
-				// staged footprint and are discarded by a select; a lane whose
-				// range misses this chunk (jhi <= jlo) adds nothing.  Groups
-				// of 8 slices that no lane of the wave adds are skipped (t
-				// advances only on added slices)
-				float t = t0;
-#pragma unroll
-				for (int g0 = 0; g0 < ZL; g0 += 8) {
-					if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
-						continue;
-#pragma unroll
-					for (int j = g0; j < g0 + 8; ++j) {
-						const bool r = (unsigned) (j - jlo) < nj;
-						const float tv = r ? t : t0;
-						const float up = slice_update<FMT, FAST>(acc[j], tv, c1, c2, wl, wr, w2);
-						acc[j] = r ? up : acc[j];
-						t = r ? t + ti : t;
+					const bool rr = (unsigned) (j - jlo) < nj;
+					const float tv = rr ? t : t0;
+					const int it = (int) floorf(tv);
+					const float wu = tv - (float) it;
+					const float wd = 1.f - wu;
+					const float a0 = load_sample(a.padded, c1 + it);
+					const float a1 = load_sample(a.padded, c1 + it + 1);
+					const float b0 = load_sample(a.padded, c2 + it);
+					const float b1 = load_sample(a.padded, c2 + it + 1);
+					float up;
+					if (FAST) {
+						const float q1 = fmaf(wr, b0, wl * a0);
+						const float q2 = fmaf(wr, b1, wl * a1);
+						up = fmaf(wu, q2, fmaf(wd, q1, acc[j]));
+					} else {
+						const float q1 = wl * a0 + wr * b0;
+						const float q2 = wl * a1 + wr * b1;
+						const float p0v = wd * q1 + wu * q2;
+						up = acc[j] + w2 * p0v;
 					}
+					acc[j] = rr ? up : acc[j];
+					t = rr ? t + ti : t;
+					// one slice's gathers at a time (register budget)
+					__builtin_amdgcn_sched_barrier(0);
 				}
-			}
-		} else {
-			// footprint larger than the LDS slot: gather from the padded copy
-			// in HBM, every slice computed (no per-lane branches, so the
-			// accumulators stay in place), the ones outside [jlo, jhi) read at
-			// t0, inside the padded view, and discarded by a select
-			const int is = cb;                    // the record holds is for these views
-			const int pnt = g.nt + 2;
-			const size_t col = ((size_t) (v0 + vv) * (g.ns + 2) + is + 1) * pnt;
-			const size_t c1 = col + 1, c2 = c1 + pnt;  // (s = is, t = 0), (is+1, 0)
-			float t = t0;
-#pragma unroll
-			for (int j = 0; j < ZL; ++j) {
-				const bool r = (unsigned) (j - jlo) < nj;
-				const float tv = r ? t : t0;
-				const int it = (int) floorf(tv);
-				const float wu = tv - (float) it;
-				const float wd = 1.f - wu;
-				const float a0 = load_sample(a.padded, c1 + it);
-				const float a1 = load_sample(a.padded, c1 + it + 1);
-				const float b0 = load_sample(a.padded, c2 + it);
-				const float b1 = load_sample(a.padded, c2 + it + 1);
-				float up;
-				if (FAST) {
-					const float q1 = fmaf(wr, b0, wl * a0);
-					const float q2 = fmaf(wr, b1, wl * a1);
-					up = fmaf(wu, q2, fmaf(wd, q1, acc[j]));
-				} else {
-					const float q1 = wl * a0 + wr * b0;
-					const float q2 = wl * a1 + wr * b1;
-					const float p0v = wd * q1 + wu * q2;
-					up = acc[j] + w2 * p0v;
-				}
-				acc[j] = r ? up : acc[j];
-				t = r ? t + ti : t;
-				// one slice's gathers at a time (register budget)
-				__builtin_amdgcn_sched_barrier(0);
 			}
 		}
 	}
@@ -898,21 +963,31 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 // NP producer waves.  Views go in stages of `vstage` through two LDS
 // buffers: while the consumers add stage s from one buffer, the producers
 // set up and stage s+1 into the other; one barrier per stage.
-template <int NW, int NP, int ZL, int FMT, bool FAST>
+template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1>
 __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int group)
 {
+	static_assert(R == 1 || FMT == FMT_F32, "several chunks per wave: FP32 walk modes only");
+	constexpr int NC = NW * R;   // ZL-slice chunks of the z-group
 	const GeomF &g = a.g;
 	const int tid = threadIdx.x;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int lane = tid & 63;
 	const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-	const int zg0 = a.z0 + group * (NW * ZL);
-	const int zg1 = min(zg0 + NW * ZL, g.nz);
+	const int zg0 = a.z0 + group * (NC * ZL);
+	const int zg1 = min(zg0 + NC * ZL, g.nz);
 	const int ix = tx * 8 + (lane & 7);
 	const int iy = ty * 8 + (lane >> 3);
 
-	const Lds<NW, FMT == FMT_F32> L(a.vstage, a.slot);
-	const int nstage = (g.na + a.vstage - 1) / a.vstage;
+	const Lds<NC, FMT == FMT_F32> L(a.vstage, a.slot);
+	// the views whose slice window meets this z-group (a helix), else all
+	int va = 0, vb = g.na;
+	if (a.gviews) {
+		// (at most g.na of them: the autotuner times a prefix of each range)
+		const int2 r = a.gviews[group];
+		va = __builtin_amdgcn_readfirstlane(r.x);
+		vb = min(__builtin_amdgcn_readfirstlane(r.y), va + g.na);
+	}
+	const int nstage = (vb - va + a.vstage - 1) / a.vstage;
 
 	if (wave >= NW) {
 		// ------------------------------------------------ producer waves
@@ -923,45 +998,52 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 		const float xx = g.dx * ((float) ix - g.wx);
 		if (FMT != FMT_F32 && pw == 0 && lane < 2)
 			fdk_lds[L.zero + lane] = 0.f;   // read by DIRECT lanes without work
-		produce<NW, NP, ZL, FMT, FAST>(a, 0, L, 0, min(a.vstage, g.na), pw, lane,
-			colvalid, xx, yy, zg0, zg1);
+		if (nstage > 0)
+			produce<NC, NP, ZL, FMT, FAST>(a, 0, L, va, min(a.vstage, vb - va), pw, lane,
+				colvalid, xx, yy, zg0, zg1);
 		__builtin_amdgcn_s_waitcnt(0);   // staged footprints have landed in LDS
 		__syncthreads();
 		for (int s = 0; s < nstage; ++s) {
 			if (s + 1 < nstage) {
-				const int v0 = (s + 1) * a.vstage;
-				produce<NW, NP, ZL, FMT, FAST>(a, ((s + 1) & 1) * L.total, L, v0,
-					min(a.vstage, g.na - v0), pw, lane, colvalid, xx, yy, zg0, zg1);
+				const int v0 = va + (s + 1) * a.vstage;
+				produce<NC, NP, ZL, FMT, FAST>(a, ((s + 1) & 1) * L.total, L, v0,
+					min(a.vstage, vb - v0), pw, lane, colvalid, xx, yy, zg0, zg1);
 			}
 			__builtin_amdgcn_s_waitcnt(0);
 			__syncthreads();
 		}
 	} else {
 		// ------------------------------------------------ consumer waves
-		const int zc = zg0 + wave * ZL;
-		float acc[ZL];
+		// chunks wave, wave + NW, ... (R of them) of the z-group
+		float accs[R][ZL];
 #pragma unroll
-		for (int j = 0; j < ZL; ++j)
-			acc[j] = 0.f;
+		for (int r = 0; r < R; ++r)
+#pragma unroll
+			for (int j = 0; j < ZL; ++j)
+				accs[r][j] = 0.f;
 		__syncthreads();
 		for (int s = 0; s < nstage; ++s) {
-			const int v0 = s * a.vstage;
+			const int v0 = va + s * a.vstage;
 			if constexpr (FMT == FMT_F32)
-				consume_walk<NW, ZL, FMT, FAST>(a, (s & 1) * L.total, L, v0,
-					min(a.vstage, g.na - v0), wave, lane, zc, acc);
+				consume_walk<NW, R, ZL, FMT, FAST>(a, (s & 1) * L.total, L, v0,
+					min(a.vstage, vb - v0), wave, lane, zg0, accs);
 			else
 				consume_direct<NW, ZL, FMT>(a, (s & 1) * L.total, L, v0,
-					min(a.vstage, g.na - v0), wave, lane, zc, acc);
+					min(a.vstage, vb - v0), wave, lane, zg0 + wave * ZL, accs[0]);
 			__syncthreads();
 		}
 		// one write per voxel, [nz nx ny]
 		if (ix < g.nx && iy < g.ny) {
 			float *col = a.image + ((long) iy * g.nx + ix) * g.nz;
 #pragma unroll
-			for (int j = 0; j < ZL; ++j) {
-				const int z = zc + j;
-				if (z < zg1)
-					col[z] = acc[j];
+			for (int r = 0; r < R; ++r) {
+				const int zc = zg0 + (wave + r * NW) * ZL;
+#pragma unroll
+				for (int j = 0; j < ZL; ++j) {
+					const int z = zc + j;
+					if (z < zg1)
+						col[z] = accs[r][j];
+				}
 			}
 		}
 	}
@@ -970,14 +1052,14 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 // 4 consumer + 4 producer waves: two workgroups share a CU; 8 + 8: one.
 // Either way 4 waves per SIMD, so at most 128 VGPRs (the second launch
 // bound is the minimum waves per SIMD).
-template <int NW, int NP, int ZL, int FMT, bool FAST>
+template <int NW, int NP, int ZL, int FMT, bool FAST, int R>
 __global__ void __launch_bounds__((NW + NP) * 64, 4)
 back_kernel(const BackArgs a)
 {
 	const int nblocks = a.ntiles * a.ngroups;
 	const int lb = xcd_block(blockIdx.x, nblocks);
 	const int tile = lb / a.ngroups;
-	tile_pipeline<NW, NP, ZL, FMT, FAST>(a, tile, lb - tile * a.ngroups);
+	tile_pipeline<NW, NP, ZL, FMT, FAST, R>(a, tile, lb - tile * a.ngroups);
 }
 
 // Persistent form for a batch of scans of uneven sizes: every workgroup takes
@@ -1070,9 +1152,9 @@ survey_kernel(const BackArgs a)
 	unsigned long long cnt = 0;
 	int maxarea = 0, any = 0;
 	for (int v = 0; v < g.na; ++v) {
-		const float2 cs = a.trig[v];
+		const ViewGeo vg = view_geo(a, v);
 		ColSetup c;
-		const int code = colvalid ? column_setup(g, xx, yy, cs.x, cs.y, c) : SETUP_OUT;
+		const int code = colvalid ? column_setup(g, xx, yy, vg, c) : SETUP_OUT;
 		any |= (code != SETUP_OUT) ? 1 : 0;
 		// reference loop iterations of this column-view (whole range, once)
 		if (code == SETUP_ACTIVE && group == 0)
@@ -1081,7 +1163,7 @@ survey_kernel(const BackArgs a)
 		ColRange cr{};
 		float Tk[NW];
 		const bool act = colvalid
-			&& col_range<NW, ZL, DIRECT>(g, xx, yy, cs, zg0, zg1, cr, Tk);
+			&& col_range<NW, ZL, DIRECT>(g, xx, yy, vg, zg0, zg1, cr, Tk);
 		const Footprint f = view_footprint<DIRECT>(g, act, cr);
 		if (f.s_lo <= f.s_hi && f.t_lo <= f.t_hi) {
 			const int rows = f.t_hi - f.t_lo + 1;
@@ -1325,11 +1407,11 @@ view_quantize_kernel(const float *proj, long view_elems, int fmt,
 // ---------------------------------------------------------------------------
 // launchers (host side)
 
-template <int NW, int NP, int ZL, int FMT, bool FAST>
+template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1>
 static hipError_t launch_back_t(const BackArgs &a, size_t lds_bytes, hipStream_t s)
 {
 	const int nblocks = a.ntiles * a.ngroups;
-	hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST>), dim3(nblocks),
+	hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST, R>), dim3(nblocks),
 		dim3((NW + NP) * 64), lds_bytes, s, a);
 	return hipGetLastError();
 }
@@ -1352,9 +1434,26 @@ size_t back_lds_bytes(int nw, int vstage, int slot, int fmt)
 		if (fmt == FMT_BF16) return launch_back_t<NW, NP_, ZL, FMT_BF16, true>(a, lds, s);        \
 	} while (0)
 
-hipError_t launch_back(const BackArgs &a, int nw, int zl, int fmt, bool fast, hipStream_t s)
+hipError_t launch_back(const BackArgs &a, int nw, int zl, int zr, int fmt, bool fast,
+	hipStream_t s)
 {
-	const size_t lds = back_lds_bytes(nw, a.vstage, a.slot, fmt);
+	const size_t lds = back_lds_bytes(nw * zr, a.vstage, a.slot, fmt);
+	if (zr > 1) {
+		// FP32 walk modes, chunks interleaved over the waves
+		if (fmt != FMT_F32)
+			return hipErrorInvalidValue;
+#define FDK_DISPATCH_R(NW, ZL, R)                                                            \
+		if (nw == NW && zl == ZL && zr == R) {                                               \
+			constexpr int NP_ = (NW) >= 8 ? 8 : 4;                                           \
+			return fast ? launch_back_t<NW, NP_, ZL, FMT_F32, true, R>(a, lds, s)            \
+				: launch_back_t<NW, NP_, ZL, FMT_F32, false, R>(a, lds, s);                  \
+		}
+		FDK_DISPATCH_R(4, 16, 3)
+		FDK_DISPATCH_R(4, 16, 4)
+		FDK_DISPATCH_R(8, 16, 2)
+#undef FDK_DISPATCH_R
+		return hipErrorInvalidValue;
+	}
 	if (zl == 16) {
 		if (nw == 4) FDK_DISPATCH_FMT(4, 16);
 		if (nw == 8) FDK_DISPATCH_FMT(8, 16);
@@ -1435,10 +1534,13 @@ int batch_blocks_per_cu(int nw, int zl, int vstage, int slot, int fmt, bool fast
 	return 1;
 }
 
-bool back_config_supported(int nw, int zl)
+bool back_config_supported(int nw, int zl, int zr, int fmt)
 {
-	return (zl == 16 && (nw == 4 || nw == 8)) || (zl == 8 && (nw == 4 || nw == 8))
-		|| (zl == 32 && nw == 4);
+	if (zr > 1)
+		return fmt == FMT_F32 && zl == 16 && ((nw == 4 && (zr == 3 || zr == 4))
+			|| (nw == 8 && zr == 2));
+	return zr == 1 && ((zl == 16 && (nw == 4 || nw == 8)) || (zl == 8 && (nw == 4 || nw == 8))
+		|| (zl == 32 && nw == 4));
 }
 
 hipError_t launch_survey(const BackArgs &a, int nw, int zl, bool direct, hipStream_t s)
@@ -1456,6 +1558,8 @@ hipError_t launch_survey(const BackArgs &a, int nw, int zl, bool direct, hipStre
 	FDK_SURVEY(4, 8)
 	FDK_SURVEY(8, 8)
 	FDK_SURVEY(4, 32)
+	FDK_SURVEY(12, 16)   // 4 waves x 3 chunks
+	FDK_SURVEY(16, 16)   // 4 x 4 and 8 x 2
 #undef FDK_SURVEY
 	return hipErrorInvalidValue;
 }

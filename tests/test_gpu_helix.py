@@ -139,3 +139,71 @@ def test_cbct_back_helix_gpu_route(device):
         with pytest.warns(UserWarning):
             want = cbct_back_helix(proj, cg, ig, back_call=helix_back_call, **kw)
         assert np.array_equal(got, want)
+
+
+def _helix_plan(g, beta, zs, mask, ops, fmt="f32", math_mode="exact"):
+    from mirt_amd import FdkPlan
+    from mirt_amd._lib import Geom
+    gg = Geom(*[getattr(g, f[0]) for f in Geom._fields_])
+    m = None if mask is None else np.asarray(mask, np.uint8).reshape(g.ny, g.nx).T
+    return FdkPlan(gg, beta, m, fmt=fmt, math_mode=math_mode,
+                   helix=(zs, ops["na1_half"], ops["pitch"]))
+
+
+@pytest.mark.parametrize("dfs", [0.0, np.inf])
+def test_helix_plan_every_launch_shape(device, dfs):
+    """Native helical geometry on the axial pipeline (fdkhip_plan_create_helix):
+    per-view source heights and slice windows, each z-group visiting only the
+    views whose window meets it.  Every launch shape and stage depth is the
+    reference bit for bit (restatement; reference C when built), and the
+    direct entry's cone_par = 0 route equals it too."""
+    import torch
+    from mirt_amd import _lib
+    g, proj, beta, zs, mask, ops = _seeded(dfs, pitch=1.0, cone_par=0, w3d=0, seed=41, nz=70)
+    want = O.back_ts_helix_f32(g, proj, beta, zs, mask, **ops)
+    if O.have_reference():
+        assert np.array_equal(want, O.ref_back_ts_helix(g, proj, beta, zs, mask, nthread=4,
+                                                        **ops))
+    plan = _helix_plan(g, beta, zs, mask, ops)
+    pt = torch.from_numpy(np.ascontiguousarray(proj)).to(device)
+    for waves, zper in [(8, 16), (8, 8), (4, 16), (4, 32), (4, 8), (4, 48), (4, 64), (8, 32)]:
+        for vstage in (0, 3, 8):
+            print(f"helix plan dfs={dfs} waves={waves} zper={zper} vstage={vstage}", flush=True)
+            plan.set_config(waves, zper, vstage)
+            got = plan(pt).cpu().numpy()
+            assert np.array_equal(got, want), (waves, zper, vstage,
+                                               float(np.abs(got - want).max()))
+    assert plan.count_updates() > 0
+    # the direct entry (cone_par = 0) runs on such a plan
+    from mirt_amd._lib import Geom
+    gg = Geom(*[getattr(g, f[0]) for f in Geom._fields_])
+    img = torch.empty((g.ny, g.nx, g.nz), dtype=torch.float32, device=device)
+    P = lambda a: C.c_void_p(a.ctypes.data)
+    ms = C.c_float(-1)
+    m = np.ascontiguousarray(mask, np.uint8)
+    b = np.ascontiguousarray(beta, np.float64)
+    z = np.ascontiguousarray(zs, np.float64)
+    _lib.check(_lib.load().fdkhip_fdk_ts_back_helix_direct(
+        C.byref(gg), C.c_void_p(pt.data_ptr()), proj.shape[0], P(b), P(z), ops["na1_half"],
+        0, 0, ops["source_dz_per_rad"], ops["pitch"], P(m), C.c_void_p(img.data_ptr()),
+        C.c_void_p(torch.cuda.current_stream().cuda_stream), C.byref(ms)))
+    assert np.array_equal(img.cpu().numpy(), want) and ms.value > 0
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_helix_plan_half_storage(device, fmt):
+    """FP16/BF16-stored views on a helical plan (FAST arithmetic, closed-form
+    slice coordinate): within the storage bar of the exact FP32 result
+    (relative L2 1e-3 for FP16, 4e-3 for BF16, the formats' unit roundoffs
+    2^-11 and 2^-8 with margin)."""
+    import torch
+    from mirt_amd.cbct_back import quantize_views
+    g, proj, beta, zs, mask, ops = _seeded(0.0, pitch=1.0, cone_par=0, w3d=0, seed=43, nz=70)
+    want = O.back_ts_helix_f32(g, proj, beta, zs, mask, **ops)
+    plan = _helix_plan(g, beta, zs, mask, ops, fmt=fmt)
+    pt = torch.from_numpy(np.ascontiguousarray(proj)).to(device)
+    q, sc = quantize_views(pt, fmt)
+    got = plan(q, sc).cpu().numpy()
+    err = O.rel_l2(got, want)
+    print(f"helix plan {fmt}: relL2 {err:.2e}", flush=True)
+    assert err <= {"f16": 1e-3, "bf16": 4e-3}[fmt], err

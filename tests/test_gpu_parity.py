@@ -179,15 +179,25 @@ def test_lds_overflow_path(device):
     assert np.array_equal(img, want)
 
 
-@pytest.mark.parametrize("waves,zper", [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32)])
+@pytest.mark.parametrize("waves,zper", [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32), (4, 48),
+                                        (4, 64), (8, 32)])
 def test_all_configs(device, waves, zper):
+    """EXACT at every built launch shape, including several interleaved
+    16-slice chunks per consumer wave (4 x 48, 4 x 64, 8 x 32), FP32 EXACT
+    and FAST: bit for bit with the reference C (EXACT), within FP32 rounding
+    of it (FAST)."""
     import torch
     g, proj, beta, mask = make_case("arc_tall", seed=9, data="noise")
+    want = reference("fixed", g, proj, beta, mask)
     plan = _plan(g, beta, mask, "f32", "exact")
     plan.set_config(waves, zper, 0, 0)
-    want = reference("fixed", g, proj, beta, mask)
+    assert plan.config["waves"] == waves and plan.config["zper"] == zper
     img = plan(torch.from_numpy(proj).to(device)).cpu().numpy()
     assert np.array_equal(img, want)
+    fast = _plan(g, beta, mask, "f32", "fast")
+    fast.set_config(waves, zper, 0, 0)
+    img = fast(torch.from_numpy(proj).to(device)).cpu().numpy()
+    assert O.rel_l2(img, want) <= 1e-5
 
 
 @pytest.mark.parametrize("waves,zper", [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32)])
