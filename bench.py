@@ -873,6 +873,9 @@ def main():
                     help="fixed launch shape (no autotune), e.g. 4,32,4 (profiling runs)")
     ap.add_argument("--fmt-only", action="store_true",
                     help="time only the headline path (profiling runs)")
+    ap.add_argument("--filter-only", action="store_true",
+                    help="time only the FDK weight + filter kernel on the workload's "
+                         "views (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -890,6 +893,12 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 4,
                           "bytes_written": a.numel() * 4, "repeats": 3}), flush=True)
+        return
+    if args.filter_only:
+        from mirt_amd.workloads import WORKLOADS
+        cg = WORKLOADS[args.workload]().cg
+        print(json.dumps({"workload": args.workload,
+                          "fdk_filter": filter_times(args, cg, dev)}), flush=True)
         return
     if args.path == "mat":
         result = run_mat(args, dev)

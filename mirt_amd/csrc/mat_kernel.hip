@@ -58,7 +58,9 @@ __device__ __forceinline__ double mat_sample(const double *p, long i)
 	return p[i];
 }
 
-template <typename TP>
+// HELIX: the helical backprojector's per-slice view ranges and scales and
+// cone-parallel geometry (compiled out of the axial route)
+template <typename TP, bool HELIX>
 __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 {
 	const int tx = (a.nx + 7) >> 3, ty = (a.ny + 7) >> 3;
@@ -96,7 +98,7 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 	// the views each slice takes (helical: its own range, cbct_back_helix.m:224-236)
 	int vlo = 0, vhi = a.nv - 1;
 	int lo[MAT_ZC], hi[MAT_ZC];
-	if (a.zrange) {
+	if (HELIX) {
 		vlo = a.nv;
 		vhi = -1;
 #pragma unroll
@@ -114,11 +116,11 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 		bool use[MAT_ZC], any = false;
 #pragma unroll
 		for (int k = 0; k < MAT_ZC; ++k) {
-			use[k] = k < nk && (!a.zrange || (v >= lo[k] && v <= hi[k] &&
+			use[k] = k < nk && (!HELIX || (v >= lo[k] && v <= hi[k] &&
 				(v - lo[k]) % a.vstep == 0));
 			any = any || use[k];
 		}
-		if (!any)
+		if (HELIX && !any)
 			continue;
 		const double4 vw = a.views[v];              // {cos beta, sin beta, source_zs}
 		const TP *pv = (const TP *) a.proj + (long) v * a.view_stride;
@@ -127,7 +129,7 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 		const double y_betas = a.dso - (-xc * vw.y + yc * vw.x);
 		// cbct_back.m:208-220, cbct_back_helix.m:245-265
 		double mag, sprime, r_loop = 0.0, zoff = 0.0;
-		if (a.cone_par) {
+		if (HELIX && a.cone_par) {
 			mag = a.dsd / (sqrt(a.dso * a.dso - x_beta * x_beta) - (a.dso - y_betas));
 			sprime = x_beta;
 			zoff = asin(-x_beta / a.dso) * a.dz_per_rad;
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 		for (int k = 0; k < MAT_ZC; ++k) {
 			if (use[k]) {
 				// cbct_back.m:219-233 for t: clamp to [1, nt], bt = nt uses rows nt-1, nt
-				const double tprime = a.cone_par ? mag * (zc[k] - (vw.z + zoff))
+				const double tprime = (HELIX && a.cone_par) ? mag * (zc[k] - (vw.z + zoff))
 					: mag * (zc[k] - vw.z);
 				double bt = tprime / a.dt + a.wt;
 				bt = fmax(bt, 1.0);                  // NaN -> 1, as MATLAB's max
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 				const double p1 = wl * a00 + wr * a10;
 				const double p2 = wl * a01 + wr * a11;
 				double p0 = wd * p1 + wu * p2;
-				if (!a.cone_par) {                   // cone_par: no weight without w3d
+				if (!(HELIX && a.cone_par)) {        // cone_par: no weight without w3d
 					if (a.kind == KIND_ARC)
 						p0 = p0 * dsd2 / wden;
 					else
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(64) mat_kernel(const MatArgs a)
 #pragma unroll
 	for (int k = 0; k < MAT_ZC; ++k)
 		if (k < nk)
-			out[(long) k * nxy] = (a.zscale ? a.zscale[z0 + k] : a.scale) * acc[k];
+			out[(long) k * nxy] = ((HELIX && a.zscale) ? a.zscale[z0 + k] : a.scale) * acc[k];
 }
 
 } // namespace
@@ -209,10 +211,15 @@ hipError_t launch_mat(const MatArgs &a, bool proj_fp64, hipStream_t s)
 		return hipSuccess;
 	if (tiles > 0x7fffffffL)
 		return hipErrorInvalidValue;
-	if (proj_fp64)
-		hipLaunchKernelGGL(mat_kernel<double>, dim3((unsigned) tiles), dim3(64), 0, s, a);
+	const bool hx = a.zrange != nullptr;
+	if (proj_fp64 && hx)
+		hipLaunchKernelGGL((mat_kernel<double, true>), dim3((unsigned) tiles), dim3(64), 0, s, a);
+	else if (proj_fp64)
+		hipLaunchKernelGGL((mat_kernel<double, false>), dim3((unsigned) tiles), dim3(64), 0, s, a);
+	else if (hx)
+		hipLaunchKernelGGL((mat_kernel<float, true>), dim3((unsigned) tiles), dim3(64), 0, s, a);
 	else
-		hipLaunchKernelGGL(mat_kernel<float>, dim3((unsigned) tiles), dim3(64), 0, s, a);
+		hipLaunchKernelGGL((mat_kernel<float, false>), dim3((unsigned) tiles), dim3(64), 0, s, a);
 	return hipGetLastError();
 }
 
