@@ -15,10 +15,10 @@ views and the tiled kernel), projections and image resident in HBM.
 value = nx*ny*nz*na / t.  The roofline entry is for the backprojection kernel
 alone, timed with HIP events on its own stream (fdkhip_plan_profile).
 
-``--workload uneven_scans`` times 32 seeded scans of uneven size in one
-persistent launch (FdkBatch, autotuned, replayed from a HIP graph) against
-the same scans run one plan after another; other workloads are listed in
-mirt_amd/workloads.py.
+``--workload uneven_scans`` times 32 seeded scans of uneven size in one batch
+execute (FdkBatch, autotuned: the persistent kernel or one launch per scan,
+replayed from a HIP graph) against the same scans run one untuned plan after
+another; other workloads are listed in mirt_amd/workloads.py.
 
 Prints ONE JSON line.  The CPU baseline is the reference C (oracle/_ref,
 fdk-helix axial path, pthreads) timed on this host's cores on a stated sample
@@ -90,6 +90,9 @@ def profiled_traffic(workload, fmt, cfg, kernel="back_kernel"):
         return None
     text = open(path).read()
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    if m and cfg["waves"] == 0:
+        # a batch in one launch per scan: the summary covers one execute
+        return float(m.group(1)) if re.search(r"^launch:\s*per_scan", text, re.M) else None
     shape = re.search(r"^launch:\s*waves=(\d+) zper=(\d+) vstage=(\d+)", text, re.M)
     if not m or not shape:
         return None
@@ -770,7 +773,12 @@ def run_batch(args, dev, errors_without_baseline=False):
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned:.3f} ms, "
             f"config {batch.config}")
     launch = dict(batch.config)
-    launch["half_height_scans"] = sum(z < launch["zper"] for z in batch.scan_zper)
+    if batch.per_scan:
+        launch = {"mode": "one launch per scan", "workgroups": launch["workgroups"],
+                  "slices_per_lane": batch.scan_zper}
+    else:
+        launch["mode"] = "persistent"
+        launch["half_height_scans"] = sum(z < launch["zper"] for z in batch.scan_zper)
     log(f"per-scan slices per lane: {batch.scan_zper}")
     ms = time_steps(lambda: batch(projs, scales, outs), args.steps, args.warmup)
     prof = batch.profile(projs, scales, outs, reps=max(2, args.steps))
@@ -795,7 +803,8 @@ def run_batch(args, dev, errors_without_baseline=False):
 
     kms = prof["back"]
     achieved = FLOP_PER_UPDATE * U_eff / (kms * 1e-3) / 1e12
-    btraffic = profiled_traffic("uneven_scans", head, batch.config, "batch_kernel")
+    bkernel = "back_kernel" if batch.per_scan else "batch_kernel"
+    btraffic = profiled_traffic("uneven_scans", head, batch.config, bkernel)
     result = {
         "metric": "voxel-view updates/s (GUPS), uneven_scans batch",
         "value": U / (ms * 1e-3),
@@ -805,9 +814,9 @@ def run_batch(args, dev, errors_without_baseline=False):
         "data": f"synthetic: {len(scans)} seeded uneven scans, 10-30 ellipsoids each, "
                 f"analytic flat cone-beam line integrals, FDK weight + ramp filter, {head} "
                 "storage",
-        "config": {"workload": f"uneven_scans: {len(scans)} scans in one persistent launch "
-                               "(edge {128,192,256,384}, nz=edge/2, 180-720 views, flat "
-                               "256-768 x 128-512 det)",
+        "config": {"workload": f"uneven_scans: {len(scans)} scans in one batch execute (one "
+                               "graph replay; edge {128,192,256,384}, nz=edge/2, 180-720 "
+                               "views, flat 256-768 x 128-512 det)",
                    "storage": head, "accumulate": "f32", "launch": launch},
         "updates_nominal": U, "updates_effective": U_eff,
         "sequential_plans_ms_per_step": ms_seq,
@@ -815,11 +824,13 @@ def run_batch(args, dev, errors_without_baseline=False):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": btraffic,
-                     "valu_issue": (profiled_valu_issue(f"batch_kernel_uneven_scans_{head}")
+                     "valu_issue": (profiled_valu_issue(f"{bkernel}_uneven_scans_{head}")
                                     if btraffic is not None else None),
                      "note": f"{FLOP_PER_UPDATE} FP32 flop x {U_eff} in-support updates / "
-                             f"{kms:.3f} ms batch_kernel (HIP events); traffic and VALU issue "
-                             "from the rocprofv3 summary in profiles/"},
+                             f"{kms:.3f} ms of {bkernel} "
+                             f"({'one launch per scan' if batch.per_scan else 'persistent'}; "
+                             "HIP events); traffic and VALU issue per batch execute from the "
+                             "rocprofv3 summary in profiles/"},
         "cpu_baseline": None,
     }
     if (not args.no_cpu_baseline or errors_without_baseline) and hosts:

@@ -308,7 +308,9 @@ int fdkhip_plan_profile(fdkhip_plan *plan, const void *proj_dev,
  * Batches of scans: many 'fdk,ts,back' calls -- different geometries, view
  * counts and masks, one storage format -- backprojected by one persistent
  * launch whose workgroups take (scan, column tile, z group) items from a
- * device work queue, so scans of uneven size keep every CU busy.  Each scan
+ * device work queue, so scans of uneven size keep every CU busy, or by one
+ * launch per scan in turn (fdkhip_batch_set_per_scan), each in its own launch
+ * shape; either way one graph replay per execute.  Each scan
  * i is what fdkhip_plan_execute would compute for geoms[i], na[i], beta[i]
  * and mask[i] (mask may be NULL, or mask[i] NULL for all ones), with the same
  * result bits for the same math mode.  The reference backprojects one scan
@@ -332,8 +334,20 @@ int fdkhip_batch_execute_direct(fdkhip_batch *batch, const void *const *proj,
 
 /* cfg[0] = waves per workgroup, cfg[1] = slices per lane, cfg[2] = views per
  * LDS stage, cfg[3] = floats per view slot, cfg[4] = persistent workgroups,
- * cfg[5] = work items. */
+ * cfg[5] = work items.  One launch per scan: cfg[0..3] = 0 and cfg[4] =
+ * cfg[5] = the workgroups of all the scans' launches. */
 int fdkhip_batch_config(const fdkhip_batch *batch, int32_t cfg[6]);
+
+/* One launch per scan in turn instead of the persistent kernel, each scan in
+ * its plan's own launch shape (the default of fdkhip_plan_create, or what
+ * fdkhip_plan_autotune chose earlier in this process for that geometry).
+ * Every workgroup of a launch starts together and walks the views in step
+ * with its neighbouring tiles, which then share their detector footprints
+ * in L2; the persistent kernel's workgroups drift apart from item to item.
+ * fdkhip_batch_set_config and fdkhip_batch_set_zper return to the
+ * persistent kernel.  Results: the same bits as the persistent kernel for
+ * EXACT.  No reference counterpart (a launch parameter of the batch). */
+int fdkhip_batch_set_per_scan(fdkhip_batch *batch);
 
 /* Launch shape for the whole batch (a pair the build supports) and an upper
  * bound on views per LDS stage (0 = largest that fits).  Each scan's slices
@@ -355,7 +369,8 @@ int fdkhip_batch_scan_zper(const fdkhip_batch *batch, int32_t *zper);
 /* As fdkhip_plan_autotune, over the batch's launch shapes and stage depths
  * (the persistent kernel over the first 128 views of every scan), each shape
  * with the automatic per-scan slices per lane and, where those differ from
- * the shape's, with the shape's for every scan. */
+ * the shape's, with the shape's for every scan; then one launch per scan,
+ * each scan's plan tuned alone by fdkhip_plan_autotune.  Keeps the fastest. */
 int fdkhip_batch_autotune(fdkhip_batch *batch, const void *const *proj,
 	const float *const *scale, float *const *image, void *stream, int32_t reps,
 	float *best_ms);

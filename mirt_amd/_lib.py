@@ -97,6 +97,7 @@ _SIGS = {
     "fdkhip_batch_execute_direct": (C.c_int, [_P, _P, _P, _P, _P]),
     "fdkhip_batch_config": (C.c_int, [_P, _P]),
     "fdkhip_batch_set_config": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    "fdkhip_batch_set_per_scan": (C.c_int, [_P]),
     "fdkhip_batch_set_zper": (C.c_int, [_P, _P]),
     "fdkhip_fdk_filter_views": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P,
                                           C.c_int32, _P, _P]),

@@ -580,18 +580,6 @@ class FdkBatch:
         self.handle = h
         self.nscan = n
 
-    @classmethod
-    def helix_from_geoms(cls, cg: CTGeom, ig: ImageGeom, *, ia_skip=1, **kw):
-        """Plan for the helical scan ``cg`` in native helical geometry (views
-        1:ia_skip:na at their source heights, the slice windows of
-        cbct_back_helix's na1_half; fdkhip_plan_create_helix)."""
-        from .cbct_back_helix import helix_operands
-        na1_half, _ = helix_operands(cg, ia_skip)
-        beta = _views(cg, cg.na)[::ia_skip]
-        zs = np.broadcast_to(np.asarray(cg.source_zs, np.float64), (cg.na,))[::ia_skip]
-        return cls(geom_of(cg, ig), beta, ig.mask_or,
-                   helix=(zs, na1_half, float(cg.pitch)), **kw)
-
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
@@ -600,13 +588,28 @@ class FdkBatch:
 
     @property
     def config(self):
+        """Launch shape of the persistent kernel; with one launch per scan,
+        waves = zper = vstage = slot = 0 and workgroups = items = the
+        workgroups of all the scans' launches."""
         cfg = (C.c_int32 * 6)()
         _lib.check(self.lib.fdkhip_batch_config(self.handle, cfg))
         keys = ("waves", "zper", "vstage", "slot", "workgroups", "items")
         return dict(zip(keys, list(cfg)))
 
     def set_config(self, waves, zper, vstage=0):
+        """Persistent kernel in this launch shape (fdkhip_batch_set_config)."""
         _lib.check(self.lib.fdkhip_batch_set_config(self.handle, waves, zper, vstage))
+
+    def set_per_scan(self):
+        """One launch per scan in turn, each in its plan's own launch shape
+        (fdkhip_batch_set_per_scan); ``set_config`` returns to the persistent
+        kernel."""
+        _lib.check(self.lib.fdkhip_batch_set_per_scan(self.handle))
+
+    @property
+    def per_scan(self) -> bool:
+        """True when the batch runs one launch per scan."""
+        return self.config["waves"] == 0
 
     @property
     def scan_zper(self):
@@ -677,9 +680,10 @@ class FdkBatch:
         return {"prep": ms[0], "back": ms[1]}
 
     def autotune(self, projs, scales=None, outs=None, *, reps=3, stream=None):
-        """fdkhip_batch_autotune on these buffers (the persistent kernel over
-        the first 128 views of every scan); returns the best average ms of
-        such a launch.  ``outs`` are overwritten."""
+        """fdkhip_batch_autotune on these buffers (the persistent kernel's
+        shapes, then one launch per scan with every plan tuned alone, each
+        timed over the first 128 views of every scan); keeps the fastest and
+        returns its average ms.  ``outs`` are overwritten."""
         import torch
         outs, pa, sa, oa = self._arrays(projs, scales, outs)
         s = stream if stream is not None else torch.cuda.current_stream(projs[0].device)

@@ -430,6 +430,7 @@ static int plan_free(fdkhip_plan *p)
 }
 
 static int apply_tuned(fdkhip_plan *p);
+static int plan_default_config(fdkhip_plan *p);
 
 constexpr size_t kPadTail = 256;   // zero bytes after a plan's padded copy
 
@@ -552,14 +553,7 @@ static int plan_create_impl(fdkhip_plan **out, const fdkhip_geom *g, int32_t na,
 #undef PLAN_TRY
 
 	support_z(g, mask_host, p->zs_lo, p->zs_hi, zs_lo, zs_hi);
-	// default: 8 waves x 16 slices = 128-slice z-groups (4 x 16 when the
-	// support is short)
-	int nw = 8, zl = 16;
-	if (p->zs_hi - p->zs_lo <= 64)
-		nw = 4;
-	rc = configure(p, nw, zl, 0, 0);
-	if (rc == FDKHIP_OK)
-		rc = apply_tuned(p);
+	rc = plan_default_config(p);
 	if (rc) {
 		plan_free(p);
 		return rc;
@@ -1769,6 +1763,7 @@ extern "C" int fdkhip_plan_profile(fdkhip_plan *p, const void *proj, const float
 	return rc;
 }
 
+
 // apply a choice made earlier in this process for the same geometry
 static int apply_tuned(fdkhip_plan *p)
 {
@@ -1778,12 +1773,26 @@ static int apply_tuned(fdkhip_plan *p)
 	return configure(p, t.nw, t.zl, t.vstage, 0);
 }
 
+// a plan's own launch shape: 8 waves x 16 slices = 128-slice z-groups (4 x 16
+// when the support is short), or the autotuner's choice made earlier in this
+// process for the same geometry
+static int plan_default_config(fdkhip_plan *p)
+{
+	const int nw = (p->zs_hi - p->zs_lo <= 64) ? 4 : 8;
+	const int rc = configure(p, nw, 16, 0, 0);
+	return rc ? rc : apply_tuned(p);
+}
+
 // ---------------------------------------------------------------------------
-// batches of scans: one persistent launch driven by a device work queue
+// batches of scans: one persistent launch driven by a device work queue, or
+// one launch per scan
 
 struct fdkhip_batch {
 	int nscan = 0, fmt = FMT_F32, math = FDKHIP_MATH_FAST;
 	std::vector<fdkhip_plan *> plans;
+	// one back_kernel launch per scan in turn, each plan with its own launch
+	// shape, instead of the persistent kernel (fdkhip_batch_set_per_scan)
+	bool per_scan = false;
 	int nw = 8, zl = 16, vstage = 8, slot = 64, nwg = 256;
 	int vstage_req = 0;   // requested views per stage (0: largest that fits)
 	// per-scan slices per consumer wave: chosen by batch_configure
@@ -1925,6 +1934,7 @@ static int batch_configure(fdkhip_batch *b, int nw, int zl, int vstage_req)
 		cus = 256;
 	b->nwg = cus * batch_blocks_per_cu(nw, zl, vstage, slot, b->fmt,
 		b->math == FDKHIP_MATH_FAST);
+	b->per_scan = false;
 	(void) hipFree(b->d_item_begin);
 	b->d_item_begin = nullptr;
 	HIP_TRY(hipMalloc(&b->d_item_begin, sizeof(long long) * b->item_begin.size()));
@@ -1982,12 +1992,34 @@ extern "C" int fdkhip_batch_config(const fdkhip_batch *b, int32_t cfg[6])
 {
 	if (!b || !cfg)
 		return fail(FDKHIP_EARG, "NULL argument");
+	if (b->per_scan) {
+		long long wg = 0;
+		for (const auto *p : b->plans)
+			wg += (p->na > 0) ? (long long) p->tiles_x * p->tiles_y * p->ngroups : 0;
+		cfg[0] = cfg[1] = cfg[2] = cfg[3] = 0;
+		cfg[4] = cfg[5] = (int32_t) std::min(wg, (long long) 0x7fffffff);
+		return FDKHIP_OK;
+	}
 	cfg[0] = b->nw;
 	cfg[1] = b->zl;
 	cfg[2] = b->vstage;
 	cfg[3] = b->slot;
 	cfg[4] = b->nwg;
 	cfg[5] = (int32_t) std::min(b->nitems, (long long) 0x7fffffff);
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_batch_set_per_scan(fdkhip_batch *b)
+{
+	if (!b)
+		return fail(FDKHIP_EARG, "batch is NULL");
+	for (auto *p : b->plans) {
+		const int rc = plan_default_config(p);
+		if (rc)
+			return rc;
+	}
+	b->per_scan = true;
+	batch_drop_graph(b);
 	return FDKHIP_OK;
 }
 
@@ -2024,7 +2056,7 @@ extern "C" int fdkhip_batch_scan_zper(const fdkhip_batch *b, int32_t *zper)
 	if (!b || !zper)
 		return fail(FDKHIP_EARG, "NULL argument");
 	for (int i = 0; i < b->nscan; ++i)
-		zper[i] = b->plans[i]->zl;
+		zper[i] = b->plans[i]->zl * b->plans[i]->zr;
 	return FDKHIP_OK;
 }
 
@@ -2039,12 +2071,20 @@ extern "C" int fdkhip_batch_count_updates(fdkhip_batch *b, int64_t *n)
 	return FDKHIP_OK;
 }
 
-// Per-scan zero fills and padded copies, then the persistent kernel (and its
-// queue reset).  With `ev`, ev[0..2] are recorded before the per-scan steps,
-// between them and the kernel, and after it.
-// the persistent kernel alone (with its queue reset), on the uploaded arguments
+// the backprojection alone on the uploaded arguments: the persistent kernel
+// (with its queue reset), or each scan's own launch in turn
 static int batch_kernel_launch(fdkhip_batch *b, hipStream_t s)
 {
+	if (b->per_scan) {
+		for (int i = 0; i < b->nscan; ++i) {
+			const fdkhip_plan *p = b->plans[i];
+			if (p->ngroups == 0 || p->na == 0)
+				continue;
+			HIP_TRY(launch_back(b->host_args[i], p->nw, p->zl, p->zr, p->fmt,
+				p->math == FDKHIP_MATH_FAST, s));
+		}
+		return FDKHIP_OK;
+	}
 	if (b->nitems == 0)
 		return FDKHIP_OK;
 	BatchArgs ba;
@@ -2058,6 +2098,9 @@ static int batch_kernel_launch(fdkhip_batch *b, hipStream_t s)
 	return FDKHIP_OK;
 }
 
+// Per-scan zero fills and padded copies, then the backprojection
+// (batch_kernel_launch).  With `ev`, ev[0..2] are recorded before the
+// per-scan steps, between them and the backprojection, and after it.
 static int batch_launch(fdkhip_batch *b, const void *const *proj, const float *const *scale,
 	float *const *image, hipStream_t s, hipEvent_t *ev = nullptr)
 {
@@ -2226,10 +2269,32 @@ extern "C" int fdkhip_batch_autotune(fdkhip_batch *b, const void *const *proj,
 			}
 		}
 	}
-	b->zper_req = breq;
-	rc = batch_configure(b, bn, bz, bv);
+	// one launch per scan, each plan tuned alone on its own buffers
+	for (int i = 0; i < b->nscan; ++i) {
+		rc = fdkhip_plan_autotune(b->plans[i], proj[i], scale ? scale[i] : nullptr, image[i],
+			stream, reps, nullptr);
+		if (rc)
+			return rc;
+	}
+	b->per_scan = true;
+	rc = batch_upload_args(b, proj, scale, image, s, kTuneViews);
 	if (rc)
 		return rc;
+	float t_scan = 0.f;
+	rc = time_launches([&](hipStream_t st) {
+		return batch_kernel_launch(b, st);
+	}, s, reps, &t_scan);
+	if (rc)
+		return rc;
+	if (t_scan < best_t) {
+		best_t = t_scan;
+		batch_drop_graph(b);
+	} else {
+		b->zper_req = breq;
+		rc = batch_configure(b, bn, bz, bv);
+		if (rc)
+			return rc;
+	}
 	if (best_ms)
 		*best_ms = best_t;
 	return FDKHIP_OK;

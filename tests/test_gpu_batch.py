@@ -9,6 +9,8 @@ Bars:
 * graph replays with new projection data match direct launches bit for bit;
 * per-scan z-group heights (the launch's slices per lane or half of it, in
   one launch) leave EXACT results bit for bit unchanged;
+* one launch per scan (each plan in its own launch shape) gives the same
+  EXACT bits, FP16/BF16 within 1e-6 of the persistent kernel;
 * the autotuner leaves results unchanged (EXACT: bit for bit).
 """
 from __future__ import annotations
@@ -185,6 +187,62 @@ def test_batch_f16_heights_vs_fp64(device):
         assert err <= 1e-3, (i, err)
 
 
+def test_batch_per_scan_mode(device):
+    """One launch per scan: EXACT bit for bit equal to the oracle (and so to
+    the persistent kernel), with each plan in its own launch shape, also
+    after the plans' shapes change; graph replays and direct launches agree;
+    set_config returns to the persistent kernel."""
+    import torch
+    cases = _cases() + [_off_fov_case()]
+    b = _batch(cases)
+    projs = [torch.from_numpy(p).to(device) for _, p, _, _ in cases]
+    want = [O.back_ts_f32(g, proj, beta, mask) for g, proj, beta, mask in cases]
+    b.set_per_scan()
+    cfg = b.config
+    print(f"batch per-scan: config {cfg}, slices per lane {b.scan_zper}", flush=True)
+    assert b.per_scan and cfg["waves"] == 0 and cfg["workgroups"] > 0
+    for graph in (True, False, True):
+        outs = [o.cpu().numpy() for o in b(projs, graph=graph)]
+        for i in range(len(cases)):
+            assert np.array_equal(outs[i], want[i]), (graph, i)
+    # the plans of these geometries tuned elsewhere in the process: a batch
+    # set to per-scan launches afterwards takes their tuned shapes
+    for (g, proj, beta, mask), t in zip(cases[:3], projs):
+        _plan(g, beta, mask).autotune(t, reps=1)
+    b2 = _batch(cases)
+    b2.set_per_scan()
+    outs = [o.cpu().numpy() for o in b2(projs)]
+    for i in range(len(cases)):
+        assert np.array_equal(outs[i], want[i]), ("tuned plans", i)
+    b.set_config(4, 16, 0)
+    assert not b.per_scan and b.config["zper"] == 16
+    outs = [o.cpu().numpy() for o in b(projs)]
+    for i in range(len(cases)):
+        assert np.array_equal(outs[i], want[i]), ("persistent again", i)
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_batch_per_scan_mode_16bit(device, fmt):
+    """One launch per scan with FP16/BF16 storage: equal to the persistent
+    kernel within relative L2 1e-6 (same arithmetic; only the FP32 summation
+    order of views gathered from HBM may differ with the launch shape)."""
+    import torch
+    from mirt_amd import quantize_views
+    cases = _cases(data="smooth")
+    b = _batch(cases, fmt=fmt, math_mode="fast")
+    projs, scales = [], []
+    for _, p, _, _ in cases:
+        q, s = quantize_views(torch.from_numpy(p).to(device), fmt)
+        projs.append(q)
+        scales.append(s)
+    ref = [o.cpu().numpy() for o in b(projs, scales)]
+    b.set_per_scan()
+    outs = [o.cpu().numpy() for o in b(projs, scales)]
+    for i in range(len(cases)):
+        err = O.rel_l2(outs[i], ref[i])
+        assert err <= 1e-6, (i, err)
+
+
 def test_plan_autotune_keeps_results(device):
     import torch
     g, proj, beta, mask = make_case("arc_tall", seed=11)
@@ -201,11 +259,14 @@ def test_plan_autotune_keeps_results(device):
 
 
 def test_batch_autotune_keeps_results(device):
+    """Whichever the tuner keeps (a persistent shape or one launch per scan),
+    EXACT results stay bit for bit."""
     import torch
     cases = _cases()
     b = _batch(cases)
     projs = [torch.from_numpy(p).to(device) for _, p, _, _ in cases]
     assert b.autotune(projs, reps=1) > 0
+    print(f"batch autotune: per_scan={b.per_scan} config {b.config}", flush=True)
     outs = [o.cpu().numpy() for o in b(projs)]
     for i, (g, proj, beta, mask) in enumerate(cases):
         assert np.array_equal(outs[i], O.back_ts_f32(g, proj, beta, mask)), i
