@@ -276,13 +276,15 @@ __device__ __forceinline__ bool setup_walk(const GeomF &g, const ColSetup &c,
 //   u    = U[k] + j * t_inc                             (slice zg0 + k*ZL + j)
 // each step one fmaf (one rounding each; nearer the exact value than the
 // reference's running sum).  [zlo, zhi) is [iz_min - 1, iz_max + 1) within
-// the group: column_setup_fast's bounds are within one slice of where u
-// leaves (0, nt + 1), so the range holds every slice with 0 < u < nt + 1,
-// and u <= 0 below it and u >= nt + 1 above it; the few slices it holds
-// outside (0, nt + 1) read zero rows with weight 0 under the consumer's
-// clamp (ColRange).  first/last are u at zlo and zhi - 1.  Uk[0] receives U0.
+// the group and the view's window: column_setup_fast's bounds are within one
+// slice of where u leaves (0, nt + 1), so the range holds every slice with
+// 0 < u < nt + 1 in the window, and u <= 0 below it and u >= nt + 1 above
+// it unless the window ends it (the consumer skips the slices outside the
+// window); the few slices it holds outside (0, nt + 1) read zero rows with
+// weight 0 under the consumer's clamp (ColRange).  first/last are u at zlo
+// and zhi - 1.  Uk[0] receives U0.
 template <int NW, int ZL>
-__device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
+__device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c, const ViewGeo &vg,
 	int zg0, int zg1, float (&Uk)[NW], int &zlo, int &zhi, float &first, float &last)
 {
 	const float ti = c.t_inc;
@@ -292,8 +294,9 @@ __device__ __forceinline__ bool setup_direct(const GeomF &g, const ColSetup &c,
 		const unsigned k = (unsigned) (z - zg0), j0 = k & ~(unsigned) (ZL - 1);
 		return fmaf((float) (k - j0), ti, fmaf((float) j0, ti, U0));
 	};
-	const int lo = min(max(c.iz_min - 1, zg0), zg1);
-	const int hi = max(min(c.iz_max + 1, zg1), lo);
+	// the extra slice at each end stays inside the view's window (a helix)
+	const int lo = min(max(max(c.iz_min - 1, vg.izb0), zg0), zg1);
+	const int hi = max(min(min(c.iz_max + 1, vg.izb1), zg1), lo);
 	zlo = lo;
 	zhi = hi;
 	first = lo < hi ? uz(lo) : 0.f;
@@ -332,7 +335,7 @@ __device__ __forceinline__ bool col_range(const GeomF &g, float xx, float yy, co
 	if constexpr (DIRECT) {
 		if (column_setup_fast(g, xx, yy, vg, r.c) != SETUP_ACTIVE)
 			return false;
-		if (!setup_direct<NW, ZL>(g, r.c, zg0, zg1, Tk, r.zlo, r.zhi, r.first, r.last))
+		if (!setup_direct<NW, ZL>(g, r.c, vg, zg0, zg1, Tk, r.zlo, r.zhi, r.first, r.last))
 			return false;
 		// every slice the consumer computes reads rows floor(uc), floor(uc) + 1
 		// of uc = u clamped to [0, u_hi] (truncation is floor for uc >= 0)
@@ -871,10 +874,27 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 // changes the FP32 summation order -- any view whose footprint exceeded the
 // slot, gathered from HBM.  Keeping the two apart keeps one code path over
 // the accumulators in the main loop, so they stay in place in registers.
-// Every slice of a chunk a wave works on is computed with u clamped to
-// [0, u_hi] (ColRange): slices outside a lane's range read zero rows with
-// vertical weight 0, and lanes without work read the zero words.
-template <int NW, int ZL, int FMT>
+// Every slice of a chunk a wave works on inside the view's slice window is
+// computed with u clamped to [0, u_hi] (ColRange): slices outside a lane's
+// range read zero rows with vertical weight 0, and lanes without work read
+// the zero words.  WINDOW (a helical plan): the view's slice window is the
+// same for every lane; groups of 8 slices that it cuts are added in a
+// second pass over the stage's views, with a select on the window, so that
+// the main loop keeps one code path over the accumulators (and they stay in
+// place in registers).  The slices outside the window
+// are not on the clamp's zero rows when the window, not the cone, ends a
+// lane's range, so they must not be added.
+template <int ZL, bool WINDOW>
+__device__ __forceinline__ int2 view_window(const BackArgs &a, int v, int zc)
+{
+	if (!WINDOW)
+		return make_int2(0, ZL);
+	const int2 w = a.izb[v];
+	return make_int2(__builtin_amdgcn_readfirstlane(max(w.x - zc, 0)),
+		__builtin_amdgcn_readfirstlane(min(w.y - zc, ZL)));
+}
+
+template <int NW, int ZL, int FMT, bool WINDOW>
 __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const Lds<NW, false> &L,
 	int v0, int nv, int wave, int lane, int zc, float (&acc)[ZL])
 {
@@ -884,7 +904,7 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 	const float jc = (float) (wave * ZL);   // this wave's first slice in the z-group
 	float4 np0 = P4[0], np1 = P4[1];
 	int2 nvi = VI[0];
-	bool hbm_views = false;
+	bool hbm_views = false, cut_views = false;
 	for (int vv = 0; vv < nv; ++vv) {
 		const float4 p0 = np0, p1 = np1;
 		const int2 vi = nvi;
@@ -908,15 +928,56 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 		const float ti = p1.x;
 		const float t0 = fmaf(jc, ti, p0.w);   // u at this wave's first slice
 		const float uh = p1.z;
+		const int2 w = view_window<ZL, WINDOW>(a, v0 + vv, zc);
 #pragma unroll
 		for (int g0 = 0; g0 < ZL; g0 += 8) {
-			// groups of 8 slices that no lane of the wave adds are skipped
-			if (!__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
+			// groups of 8 slices that no lane of the wave adds are skipped;
+			// groups the view's window cuts (a helix) are left to the pass below
+			if (g0 >= w.y || g0 + 8 <= w.x
+				|| !__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
 				continue;
+			if (g0 < w.x || g0 + 8 > w.y) {
+				cut_views = true;
+				continue;
+			}
 #pragma unroll
 			for (int j = g0; j < g0 + 8; ++j) {
 				const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
 				acc[j] = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+			}
+		}
+	}
+	if (cut_views) {
+		// groups of 8 slices that a view's window cuts: the slices outside the
+		// window keep their sums (a select on a wave-uniform condition)
+		for (int vv = 0; vv < nv; ++vv) {
+			const int2 vi = VI[vv * 2];
+			if ((__builtin_amdgcn_readfirstlane(vi.y) >> 16) != MODE_LDS)
+				continue;
+			const int2 w = view_window<ZL, WINDOW>(a, v0 + vv, zc);
+			if (w.x <= 0 && w.y >= ZL)
+				continue;
+			const float4 p0 = P4[vv * 128], p1 = P4[vv * 128 + 1];
+			const int zlohi = __float_as_int(p1.y);
+			const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
+			const int jlo = max(zlo - zc, 0);
+			const int jhi = min(zhi - zc, ZL);
+			const int c1 = bo + __float_as_int(p0.x);
+			const float wl = p0.y, wr = p0.z;
+			const float ti = p1.x;
+			const float t0 = fmaf(jc, ti, p0.w);
+			const float uh = p1.z;
+#pragma unroll
+			for (int g0 = 0; g0 < ZL; g0 += 8) {
+				if (g0 >= w.y || g0 + 8 <= w.x || (w.x <= g0 && g0 + 8 <= w.y)
+					|| !__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
+					continue;
+#pragma unroll
+				for (int j = g0; j < g0 + 8; ++j) {
+					const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
+					const float up = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+					acc[j] = (j >= w.x && j < w.y) ? up : acc[j];
+				}
 			}
 		}
 	}
@@ -943,16 +1004,18 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 		const float uh = p1.z;
 		const uint16_t *pv = static_cast<const uint16_t *>(a.proj)
 			+ (size_t) (v0 + vv) * g.ns * g.nt;
+		const int2 w = view_window<ZL, WINDOW>(a, v0 + vv, zc);
 #pragma unroll
 		for (int j = 0; j < ZL; ++j) {
 			const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
 			const int iu = (int) u;
 			const float wu = __builtin_amdgcn_fractf(u);
-			acc[j] = bilerp_d(acc[j],
+			const float up = bilerp_d(acc[j],
 				sample16<FMT>(padded16(pv, g, is + 1, iu)),
 				sample16<FMT>(padded16(pv, g, is + 2, iu)),
 				sample16<FMT>(padded16(pv, g, is + 1, iu + 1)),
 				sample16<FMT>(padded16(pv, g, is + 2, iu + 1)), wl, wr, wu);
+			acc[j] = (j >= w.x && j < w.y) ? up : acc[j];
 			// one slice's four gathers at a time (register budget)
 			__builtin_amdgcn_sched_barrier(0);
 		}
@@ -963,9 +1026,10 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 // NP producer waves.  Views go in stages of `vstage` through two LDS
 // buffers: while the consumers add stage s from one buffer, the producers
 // set up and stage s+1 into the other; one barrier per stage.
-template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1>
+template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1, bool WIN = false>
 __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int group)
 {
+	static_assert(!WIN || FMT != FMT_F32, "the walk modes keep to the windows by their ranges");
 	static_assert(R == 1 || FMT == FMT_F32, "several chunks per wave: FP32 walk modes only");
 	constexpr int NC = NW * R;   // ZL-slice chunks of the z-group
 	const GeomF &g = a.g;
@@ -1028,7 +1092,7 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 				consume_walk<NW, R, ZL, FMT, FAST>(a, (s & 1) * L.total, L, v0,
 					min(a.vstage, vb - v0), wave, lane, zg0, accs);
 			else
-				consume_direct<NW, ZL, FMT>(a, (s & 1) * L.total, L, v0,
+				consume_direct<NW, ZL, FMT, WIN>(a, (s & 1) * L.total, L, v0,
 					min(a.vstage, vb - v0), wave, lane, zg0 + wave * ZL, accs[0]);
 			__syncthreads();
 		}
@@ -1052,14 +1116,15 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 // 4 consumer + 4 producer waves: two workgroups share a CU; 8 + 8: one.
 // Either way 4 waves per SIMD, so at most 128 VGPRs (the second launch
 // bound is the minimum waves per SIMD).
-template <int NW, int NP, int ZL, int FMT, bool FAST, int R>
+// WIN: FP16/BF16 views of a helical plan (slice windows per view)
+template <int NW, int NP, int ZL, int FMT, bool FAST, int R, bool WIN>
 __global__ void __launch_bounds__((NW + NP) * 64, 4)
 back_kernel(const BackArgs a)
 {
 	const int nblocks = a.ntiles * a.ngroups;
 	const int lb = xcd_block(blockIdx.x, nblocks);
 	const int tile = lb / a.ngroups;
-	tile_pipeline<NW, NP, ZL, FMT, FAST, R>(a, tile, lb - tile * a.ngroups);
+	tile_pipeline<NW, NP, ZL, FMT, FAST, R, WIN>(a, tile, lb - tile * a.ngroups);
 }
 
 // Persistent form for a batch of scans of uneven sizes: every workgroup takes
@@ -1411,7 +1476,17 @@ template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1>
 static hipError_t launch_back_t(const BackArgs &a, size_t lds_bytes, hipStream_t s)
 {
 	const int nblocks = a.ntiles * a.ngroups;
-	hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST, R>), dim3(nblocks),
+	// FP16/BF16 views of a helical plan: the instantiation whose consumers
+	// keep to the views' slice windows (a kernel of its own, so that the
+	// axial kernel's code stays as small as it is)
+	if constexpr (FMT != FMT_F32) {
+		if (a.izb) {
+			hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST, R, true>), dim3(nblocks),
+				dim3((NW + NP) * 64), lds_bytes, s, a);
+			return hipGetLastError();
+		}
+	}
+	hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST, R, false>), dim3(nblocks),
 		dim3((NW + NP) * 64), lds_bytes, s, a);
 	return hipGetLastError();
 }

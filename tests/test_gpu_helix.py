@@ -150,16 +150,26 @@ def _helix_plan(g, beta, zs, mask, ops, fmt="f32", math_mode="exact"):
                    helix=(zs, ops["na1_half"], ops["pitch"]))
 
 
-@pytest.mark.parametrize("dfs", [0.0, np.inf])
-def test_helix_plan_every_launch_shape(device, dfs):
+def _window_binds(g, zs, ops):
+    """True when a view's slice window (2 na1_half views of source travel) is
+    shorter than the detector's height at the isocentre, so that it, not the
+    cone, ends the slice range of the columns far from the source."""
+    window = 2 * ops["na1_half"] * abs(zs[1] - zs[0]) / g.dz
+    return window < g.nt * g.dt * g.dso / g.dsd / g.dz
+
+
+@pytest.mark.parametrize("dfs,pitch", [(0.0, 1.0), (np.inf, 1.0), (0.0, 0.3)])
+def test_helix_plan_every_launch_shape(device, dfs, pitch):
     """Native helical geometry on the axial pipeline (fdkhip_plan_create_helix):
     per-view source heights and slice windows, each z-group visiting only the
     views whose window meets it.  Every launch shape and stage depth is the
     reference bit for bit (restatement; reference C when built), and the
-    direct entry's cone_par = 0 route equals it too."""
+    direct entry's cone_par = 0 route equals it too.  Pitch 1 windows are
+    wider than the cone; pitch 0.3 windows end the slice ranges."""
     import torch
     from mirt_amd import _lib
-    g, proj, beta, zs, mask, ops = _seeded(dfs, pitch=1.0, cone_par=0, w3d=0, seed=41, nz=70)
+    g, proj, beta, zs, mask, ops = _seeded(dfs, pitch=pitch, cone_par=0, w3d=0, seed=41, nz=70)
+    assert _window_binds(g, zs, ops) == (pitch < 1)
     want = O.back_ts_helix_f32(g, proj, beta, zs, mask, **ops)
     if O.have_reference():
         assert np.array_equal(want, O.ref_back_ts_helix(g, proj, beta, zs, mask, nthread=4,
@@ -191,19 +201,23 @@ def test_helix_plan_every_launch_shape(device, dfs):
 
 
 @pytest.mark.parametrize("fmt", ["f16", "bf16"])
-def test_helix_plan_half_storage(device, fmt):
+@pytest.mark.parametrize("pitch", [1.0, 0.3])
+def test_helix_plan_half_storage(device, fmt, pitch):
     """FP16/BF16-stored views on a helical plan (FAST arithmetic, closed-form
     slice coordinate): within the storage bar of the exact FP32 result
     (relative L2 1e-3 for FP16, 4e-3 for BF16, the formats' unit roundoffs
-    2^-11 and 2^-8 with margin)."""
+    2^-11 and 2^-8 with margin), also where the views' slice windows, not
+    the cone, end the slice ranges (pitch 0.3): a slice outside a view's
+    window must add nothing even where its row is on the detector."""
     import torch
     from mirt_amd.cbct_back import quantize_views
-    g, proj, beta, zs, mask, ops = _seeded(0.0, pitch=1.0, cone_par=0, w3d=0, seed=43, nz=70)
+    g, proj, beta, zs, mask, ops = _seeded(0.0, pitch=pitch, cone_par=0, w3d=0, seed=43, nz=70)
+    assert _window_binds(g, zs, ops) == (pitch < 1)
     want = O.back_ts_helix_f32(g, proj, beta, zs, mask, **ops)
     plan = _helix_plan(g, beta, zs, mask, ops, fmt=fmt)
     pt = torch.from_numpy(np.ascontiguousarray(proj)).to(device)
     q, sc = quantize_views(pt, fmt)
     got = plan(q, sc).cpu().numpy()
     err = O.rel_l2(got, want)
-    print(f"helix plan {fmt}: relL2 {err:.2e}", flush=True)
+    print(f"helix plan {fmt} pitch {pitch}: relL2 {err:.2e}", flush=True)
     assert err <= {"f16": 1e-3, "bf16": 4e-3}[fmt], err
