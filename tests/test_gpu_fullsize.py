@@ -22,7 +22,10 @@ rest of the volume is held to size-independent properties:
   scan's sampled columns bit-for-bit equal to the oracle; with FP16 storage,
   autotuned, every scan within 1e-3 of the FP64 oracle on sampled columns;
 * whole volume at ``clinical_arc``: FP16 and BF16 storage of the filtered
-  phantom views within 1e-3 of the FP64 oracle over every voxel.
+  phantom views within 1e-3 of the FP64 oracle over every voxel;
+* ``clinical_helix`` on helical plans, on a column mask of seeded columns:
+  EXACT bit for bit equal to the helical oracle, FP16/BF16 storage within
+  their bars of it.
 """
 from __future__ import annotations
 
@@ -227,3 +230,49 @@ def test_fullsize_whole_volume_vs_fp64(device, fmt):
     err = O.rel_l2(img, r64)
     print(f"clinical_arc whole volume {fmt} vs FP64: {err:.3e}", flush=True)
     assert err <= 1e-3, err
+
+
+def test_fullsize_helix(device):
+    """clinical_helix (3 turns, 2952 views, 512 x 512 x 192) on helical plans,
+    on 1024 seeded columns (a column mask; every slice, every view): EXACT
+    bit for bit equal to the oracle's FP32 restatement of the fdk-helix
+    kernel (and to the reference C when oracle/_ref is built); FP16 and BF16
+    storage within 1e-3 and 4e-3 of it.  At this geometry a view's slice
+    window (one turn of source travel) is as tall as the detector at the
+    isocentre, so for the columns far from the source the window, not the
+    cone, ends the slice range."""
+    from mirt_amd import FdkPlan, geom_of, quantize_views
+    from mirt_amd.cbct_back import _views
+    from mirt_amd.cbct_back_helix import helix_operands
+    from mirt_amd.workloads import WORKLOADS, filtered_views
+    scan = WORKLOADS["clinical_helix"]()
+    cg, ig = scan.cg, scan.ig
+    na1_half, dpr = helix_operands(cg, 1)
+    zs = np.broadcast_to(np.asarray(cg.source_zs, np.float64), (cg.na,))
+    window = 2 * na1_half * abs(zs[1] - zs[0]) / ig.dz
+    assert window <= cg.nt * cg.dt * cg.dso / cg.dsd / ig.dz * 1.01
+    cols = _cols(ig, 1024, seed=61)
+    m = np.zeros(ig.nx * ig.ny, np.uint8)
+    m[cols] = np.asarray(ig.mask_or).T.reshape(-1)[cols]
+    mask_xy = m.reshape(ig.ny, ig.nx).T                   # [nx ny] for the plans
+    tsa = filtered_views(scan, device)
+    beta = _views(cg, cg.na)
+    ops = dict(na1_half=na1_half, source_dz_per_rad=dpr, pitch=float(cg.pitch))
+    g = _oracle_geom(cg, ig)
+    host = tsa.cpu().numpy()
+    want = O.back_ts_helix_f32(g, host, beta, zs, m, **ops).reshape(-1, ig.nz)[cols]
+    if O.have_reference():
+        ref = O.ref_back_ts_helix(g, host, beta, zs, m, nthread=8, **ops)
+        assert np.array_equal(ref.reshape(-1, ig.nz)[cols], want)
+    print(f"clinical_helix: {cols.size} columns, window {window:.1f} slices", flush=True)
+    gg = geom_of(cg, ig)
+    plan_m = FdkPlan(gg, beta, mask_xy, helix=(zs, na1_half, float(cg.pitch)))
+    got = plan_m(tsa).cpu().numpy().reshape(-1, ig.nz)[cols]
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+    for fmt, bar in (("f16", 1e-3), ("bf16", 4e-3)):
+        q, s = quantize_views(tsa, fmt)
+        p16 = FdkPlan(gg, beta, mask_xy, fmt=fmt, helix=(zs, na1_half, float(cg.pitch)))
+        got = p16(q, s).cpu().numpy().reshape(-1, ig.nz)[cols]
+        err = O.rel_l2(got, want)
+        print(f"clinical_helix {fmt} vs FP32 EXACT: relL2 {err:.2e}", flush=True)
+        assert err <= bar, (fmt, err)
