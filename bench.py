@@ -181,8 +181,10 @@ def cpu_reference_rate(g, tsa_host, beta, mask, nx_ny_nz, target_s=15.0, nthread
 
 def end_to_end_ms(g_mex, tsa_host, beta, reps=2):
     """Host buffers in, host image out through the drop-in C-ABI entry
-    (fdkhip_fdk_ts_back: H2D views, plan set-up, FP32 EXACT kernel, D2H
-    image): wall milliseconds, best of `reps`."""
+    (fdkhip_fdk_ts_back: views to the device through pinned chunks, FP32
+    EXACT kernel, image back the same way; the plan and device buffers of
+    the first call are reused by the next): wall milliseconds, best of
+    `reps`."""
     from mirt_amd import fdk_mex
     ptsa = np.asfortranarray(tsa_host.transpose(2, 1, 0))   # [nt ns na]
     best = None

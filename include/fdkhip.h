@@ -85,7 +85,12 @@ const char *fdkhip_version(void);
  * be NULL (all ones).  nthread is accepted for signature compatibility and
  * checked as the reference does (1..250); the GPU ignores it.
  * Copies proj to the device, backprojects on the current HIP device, copies
- * the image back.  Synchronous.
+ * the image back.  Synchronous.  Keeps, per device, the plan of its last call
+ * and its device buffers (views, image; grown, never shrunk) for the next
+ * call: a call with the same geometry, beta and mask skips the plan set-up,
+ * and calls allocate no device memory once the buffers are large enough.
+ * Calls on one device are serialised.  At clinical_arc size that holds
+ * about 0.73 GB of device memory until fdkhip_release_host_cache.
  */
 int fdkhip_fdk_ts_back(
 	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
@@ -95,6 +100,12 @@ int fdkhip_fdk_ts_back(
 	const float *proj, int32_t ns, int32_t nt, int32_t na,
 	const double *beta, int32_t nthread,
 	float *image);
+
+/* Frees the plans and device buffers that fdkhip_fdk_ts_back keeps between
+ * calls, on every device; the next call sets them up again.  No counterpart
+ * in the reference (which allocates per call); for callers that need the
+ * device memory back. */
+int fdkhip_release_host_cache(void);
 
 /*
  * The whole use_mex = 1 path of cbct_back_mex (fbp/cbct_back.m:74-137) on host

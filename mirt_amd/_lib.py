@@ -55,6 +55,7 @@ _SIGS = {
     "fdkhip_fdk_ts_back": (C.c_int, [_P, _P, _P, _P, C.c_double, C.c_double, C.c_double,
                                      _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, _P,
                                      C.c_int32, _P]),
+    "fdkhip_release_host_cache": (C.c_int, []),
     "fdkhip_fdk_st_back": (C.c_int, [_P, _P, _P, _P, C.c_double, C.c_double, C.c_double,
                                      _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, _P,
                                      C.c_int32, _P]),

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <functional>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -881,6 +882,194 @@ struct DevBuf {
 	~DevBuf() { if (p) (void) hipFree(p); }
 };
 
+// The host-array entry keeps, per device, the plan of its last call and its
+// device buffers (grown, never shrunk) until fdkhip_release_host_cache or
+// the end of the process: a call with the same geometry, view angles and
+// mask as the one before skips the plan set-up (tables, surveys, the padded
+// copy's allocation), and no call allocates device memory once the buffers
+// are large enough.  Calls on one device are serialised by its mutex.
+// Views and image move through two pinned host chunks: host threads copy one
+// chunk while the DMA engine moves the other, so that the caller's pageable
+// buffers are neither registered nor copied by the runtime's own staging.
+namespace {
+
+constexpr size_t kStageChunk = (size_t) 32 << 20;   // bytes per pinned chunk
+
+struct HostCache {
+	std::mutex mu;
+	fdkhip_plan *plan = nullptr;
+	std::vector<unsigned char> key;    // geometry, na, beta and mask of `plan`
+	void *dproj = nullptr, *dimg = nullptr;
+	size_t proj_cap = 0, img_cap = 0;  // bytes
+	char *pinned = nullptr;            // 2 x kStageChunk, hipHostMalloc
+	hipStream_t stream = nullptr;
+	hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+constexpr int kCacheDevices = 64;
+HostCache g_host_cache[kCacheDevices];
+
+void cache_key(std::vector<unsigned char> &k, const fdkhip_geom &g, int32_t na,
+	const double *beta, const uint8_t *mask)
+{
+	const size_t nxy = (size_t) g.nx * g.ny;
+	k.resize(sizeof g + sizeof na + sizeof(double) * (size_t) na + 1 + (mask ? nxy : 0));
+	unsigned char *q = k.data();
+	memcpy(q, &g, sizeof g);
+	q += sizeof g;
+	memcpy(q, &na, sizeof na);
+	q += sizeof na;
+	if (na > 0)
+		memcpy(q, beta, sizeof(double) * (size_t) na);
+	q += sizeof(double) * (size_t) na;
+	*q++ = mask ? 1 : 0;
+	if (mask)
+		memcpy(q, mask, nxy);
+}
+
+// grow a cached device buffer to at least `bytes`
+int cache_grow(void *&buf, size_t &cap, size_t bytes)
+{
+	if (bytes <= cap)
+		return FDKHIP_OK;
+	if (buf)
+		(void) hipFree(buf);
+	buf = nullptr;
+	cap = 0;
+	if (hipMalloc(&buf, bytes) != hipSuccess)
+		return fail(FDKHIP_ENOMEM, "device allocation of %zu bytes failed", bytes);
+	cap = bytes;
+	return FDKHIP_OK;
+}
+
+void cache_clear(HostCache &c)
+{
+	if (c.plan)
+		plan_free(c.plan);   // waits for the device
+	c.plan = nullptr;
+	c.key.clear();
+	if (c.stream)
+		(void) hipStreamSynchronize(c.stream);
+	if (c.dproj) (void) hipFree(c.dproj);
+	if (c.dimg) (void) hipFree(c.dimg);
+	c.dproj = c.dimg = nullptr;
+	c.proj_cap = c.img_cap = 0;
+	if (c.pinned) (void) hipHostFree(c.pinned);
+	c.pinned = nullptr;
+	for (auto &e : c.ev)
+		if (e) (void) hipEventDestroy(e);
+	c.ev[0] = c.ev[1] = nullptr;
+	if (c.stream) (void) hipStreamDestroy(c.stream);
+	c.stream = nullptr;
+}
+
+// the pinned chunks, the copy stream and its events, made once per device
+int cache_staging(HostCache &c)
+{
+	if (!c.pinned && hipHostMalloc((void **) &c.pinned, 2 * kStageChunk) != hipSuccess) {
+		c.pinned = nullptr;
+		return fail(FDKHIP_ENOMEM, "pinned host allocation of %zu bytes failed",
+			2 * kStageChunk);
+	}
+	if (!c.stream)
+		HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+	for (auto &e : c.ev)
+		if (!e)
+			HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+	return FDKHIP_OK;
+}
+
+// memcpy of n bytes on up to 8 host threads (pieces of at least 1 MiB)
+void par_memcpy(void *dst, const void *src, size_t n)
+{
+	const size_t piece = (size_t) 1 << 20;
+	const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+	const size_t nt = std::min<size_t>({(size_t) 8, (size_t) hw, (n + piece - 1) / piece});
+	if (nt <= 1) {
+		memcpy(dst, src, n);
+		return;
+	}
+	const size_t part = (n + nt - 1) / nt;
+	std::vector<std::thread> th;
+	for (size_t t = 1; t < nt; ++t) {
+		const size_t a = t * part, b = std::min(n, a + part);
+		if (a < b)
+			th.emplace_back([=] { memcpy((char *) dst + a, (const char *) src + a, b - a); });
+	}
+	memcpy(dst, src, std::min(n, part));
+	for (auto &x : th)
+		x.join();
+}
+
+// host -> device through the pinned chunks, on c.stream: chunk i is copied
+// into pinned half i % 2 once the DMA of chunk i - 2 has read it
+int staged_h2d(HostCache &c, void *dst, const void *src, size_t n)
+{
+	size_t i = 0;
+	for (size_t off = 0; off < n; off += kStageChunk, ++i) {
+		const size_t m = std::min(kStageChunk, n - off);
+		char *buf = c.pinned + (i & 1) * kStageChunk;
+		if (i >= 2)
+			HIP_TRY(hipEventSynchronize(c.ev[i & 1]));
+		par_memcpy(buf, (const char *) src + off, m);
+		HIP_TRY(hipMemcpyAsync((char *) dst + off, buf, m, hipMemcpyHostToDevice, c.stream));
+		HIP_TRY(hipEventRecord(c.ev[i & 1], c.stream));
+	}
+	return FDKHIP_OK;
+}
+
+// device -> host through the pinned chunks, after the work queued on
+// c.stream: the DMA of chunk i + 1 runs while host threads copy chunk i out
+int staged_d2h(HostCache &c, void *dst, const void *src, size_t n)
+{
+	const size_t nchunk = (n + kStageChunk - 1) / kStageChunk;
+	auto issue = [&](size_t i) -> int {
+		const size_t off = i * kStageChunk, m = std::min(kStageChunk, n - off);
+		HIP_TRY(hipMemcpyAsync(c.pinned + (i & 1) * kStageChunk, (const char *) src + off, m,
+			hipMemcpyDeviceToHost, c.stream));
+		HIP_TRY(hipEventRecord(c.ev[i & 1], c.stream));
+		return FDKHIP_OK;
+	};
+	for (size_t i = 0; i < std::min<size_t>(2, nchunk); ++i) {
+		const int rc = issue(i);
+		if (rc)
+			return rc;
+	}
+	for (size_t i = 0; i < nchunk; ++i) {
+		const size_t off = i * kStageChunk, m = std::min(kStageChunk, n - off);
+		HIP_TRY(hipEventSynchronize(c.ev[i & 1]));
+		par_memcpy((char *) dst + off, c.pinned + (i & 1) * kStageChunk, m);
+		if (i + 2 < nchunk) {
+			const int rc = issue(i + 2);
+			if (rc)
+				return rc;
+		}
+	}
+	return FDKHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int fdkhip_release_host_cache(void)
+{
+	int cur = 0;
+	if (hipGetDevice(&cur) != hipSuccess)
+		return FDKHIP_OK;   // no device: nothing was cached
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess)
+		ndev = 0;
+	for (int d = 0; d < std::min(ndev, kCacheDevices); ++d) {
+		HostCache &c = g_host_cache[d];
+		std::lock_guard<std::mutex> lk(c.mu);
+		if (!c.plan && !c.dproj && !c.dimg)
+			continue;
+		(void) hipSetDevice(d);
+		cache_clear(c);
+	}
+	(void) hipSetDevice(cur);
+	return FDKHIP_OK;
+}
+
 extern "C" int fdkhip_fdk_ts_back(
 	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],
 	const uint8_t *mask, double dso, double dsd, double dfs,
@@ -898,35 +1087,44 @@ extern "C" int fdkhip_fdk_ts_back(
 	const size_t nvox = (size_t) g.nx * g.ny * g.nz;
 	if (nvox == 0)
 		return FDKHIP_OK;
-	fdkhip_plan *p = nullptr;
-	rc = fdkhip_plan_create(&p, &g, na, beta, mask, FDKHIP_F32, FDKHIP_MATH_EXACT);
-	if (rc)
-		return rc;
-	DevBuf dproj, dimg;
+	int dev = 0;
+	HIP_TRY(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kCacheDevices)
+		return fail(FDKHIP_EUNSUPPORTED, "device %d: at most %d devices", dev, kCacheDevices);
+	HostCache &c = g_host_cache[dev];
+	std::lock_guard<std::mutex> lk(c.mu);
+	std::vector<unsigned char> key;
+	cache_key(key, g, na, beta, mask);
+	if (!c.plan || key != c.key) {
+		if (c.plan)
+			plan_free(c.plan);
+		c.plan = nullptr;
+		c.key.clear();
+		rc = fdkhip_plan_create(&c.plan, &g, na, beta, mask, FDKHIP_F32, FDKHIP_MATH_EXACT);
+		if (rc) {
+			c.plan = nullptr;
+			return rc;
+		}
+		c.key.swap(key);
+	}
 	const size_t nproj = (size_t) ns * nt * na;
-	hipError_t e = hipSuccess;
-	if (nproj > 0)
-		e = hipMalloc(&dproj.p, nproj * sizeof(float));
-	if (e == hipSuccess)
-		e = hipMalloc(&dimg.p, nvox * sizeof(float));
-	if (e != hipSuccess) {
-		plan_free(p);
-		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+	rc = cache_grow(c.dproj, c.proj_cap, std::max(nproj, (size_t) 1) * sizeof(float));
+	if (!rc)
+		rc = cache_grow(c.dimg, c.img_cap, nvox * sizeof(float));
+	if (!rc)
+		rc = cache_staging(c);
+	if (!rc)
+		rc = staged_h2d(c, c.dproj, proj, nproj * sizeof(float));
+	if (!rc)
+		rc = fdkhip_plan_execute_direct(c.plan, c.dproj, nullptr, (float *) c.dimg, c.stream);
+	if (!rc)
+		rc = staged_d2h(c, image, c.dimg, nvox * sizeof(float));
+	if (rc) {
+		(void) hipStreamSynchronize(c.stream);   // no copy may still read a pinned chunk
+		return rc;
 	}
-	if (nproj > 0)
-		e = hipMemcpy(dproj.p, proj, nproj * sizeof(float), hipMemcpyHostToDevice);
-	if (e != hipSuccess) {
-		plan_free(p);
-		return fail(FDKHIP_EHIP, "hipMemcpy proj: %s", hipGetErrorString(e));
-	}
-	rc = fdkhip_plan_execute_direct(p, dproj.p, nullptr, (float *) dimg.p, nullptr);
-	if (rc == FDKHIP_OK) {
-		e = hipMemcpy(image, dimg.p, nvox * sizeof(float), hipMemcpyDeviceToHost);
-		if (e != hipSuccess)
-			rc = fail(FDKHIP_EHIP, "hipMemcpy image: %s", hipGetErrorString(e));
-	}
-	plan_free(p);
-	return rc;
+	HIP_TRY(hipStreamSynchronize(c.stream));
+	return FDKHIP_OK;
 }
 
 // cbct_back_mex's whole use_mex = 1 path (fbp/cbct_back.m:74-137) on host

@@ -64,6 +64,35 @@ def test_exact_mode_bit_for_bit(device, name, data):
         f"relL2={O.rel_l2(got, want):.2e}"
 
 
+def test_host_entry_cache(device):
+    """fdkhip_fdk_ts_back keeps its last plan and device buffers between
+    calls: repeated calls, new views in the same geometry, new angles, a new
+    mask, a new geometry (larger and smaller buffers) and a call after
+    fdkhip_release_host_cache each stay bit for bit equal to the oracle's
+    FP32 restatement."""
+    from mirt_amd import _lib
+    lib = _lib.load()
+    g, proj, beta, mask = make_case("arc", seed=21, data="noise")
+    g2, proj2, beta2, mask2 = make_case("flat_full", seed=22, data="noise")
+    rng = np.random.default_rng(23)
+    proj_b = rng.standard_normal(proj.shape).astype(np.float32)
+    beta_b = beta + 0.01
+    mask_b = mask.copy()
+    mask_b.reshape(-1)[::3] = 0
+    calls = [("first", g, proj, beta, mask), ("again", g, proj, beta, mask),
+             ("new views", g, proj_b, beta, mask), ("new angles", g, proj_b, beta_b, mask),
+             ("new mask", g, proj_b, beta_b, mask_b), ("new geometry", g2, proj2, beta2, mask2),
+             ("back", g, proj, beta, mask)]
+    for k, (what, gg, pp, bb, mm) in enumerate(calls):
+        if k == len(calls) - 1:
+            _lib.check(lib.fdkhip_release_host_cache())
+        print(f"host entry cache: {what}", flush=True)
+        got = gpu_ts_back(gg, pp, bb, mm)
+        assert np.array_equal(got, O.back_ts_f32(gg, pp, bb, mm)), what
+    _lib.check(lib.fdkhip_release_host_cache())
+    _lib.check(lib.fdkhip_release_host_cache())   # idempotent
+
+
 def _plan(g, beta, mask, fmt="f32", math_mode="exact"):
     from mirt_amd import FdkPlan
     from mirt_amd._lib import Geom
