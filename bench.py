@@ -369,8 +369,9 @@ def run_single(args, dev):
         e2e = end_to_end_ms(g_mex, tsa_host, np.ascontiguousarray(beta, np.float64))
         result["end_to_end"] = {
             "ms": e2e, "value": U / (e2e * 1e-3), "unit": "voxel-view updates/s",
-            "path": "host buffers through fdkhip_fdk_ts_back (H2D views, plan set-up, "
-                    "FP32 EXACT kernel, D2H image); not the headline value"}
+            "path": "host buffers through fdkhip_fdk_ts_back (views in and image out "
+                    "through pinned chunks, FP32 EXACT kernel; the first call's plan and "
+                    "device buffers reused, best of 2 calls); not the headline value"}
         log(f"end to end (host buffers, FP32 EXACT): {e2e:.1f} ms")
         imgs = {k: v.cpu().numpy() for k, v in images.items()}
         result["errors"] = column_errors(g, tsa_host, beta, m, imgs)
