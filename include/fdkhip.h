@@ -349,9 +349,12 @@ int fdkhip_batch_execute_direct(fdkhip_batch *batch, const void *const *proj,
  * cfg[5] = the workgroups of all the scans' launches. */
 int fdkhip_batch_config(const fdkhip_batch *batch, int32_t cfg[6]);
 
-/* One launch per scan in turn instead of the persistent kernel, each scan in
- * its plan's own launch shape (the default of fdkhip_plan_create, or what
- * fdkhip_plan_autotune chose earlier in this process for that geometry).
+/* One launch per scan instead of the persistent kernel, each scan in its
+ * plan's own launch shape (the default of fdkhip_plan_create, or what
+ * fdkhip_plan_autotune chose earlier in this process for that geometry), the
+ * launches split over the caller's stream and a second stream of the batch
+ * (forked from and joined back into the caller's), so that one scan's
+ * workgroups fill what another launch's last round leaves idle.
  * Every workgroup of a launch starts together and walks the views in step
  * with its neighbouring tiles, which then share their detector footprints
  * in L2; the persistent kernel's workgroups drift apart from item to item.

@@ -1988,9 +1988,12 @@ static int plan_default_config(fdkhip_plan *p)
 struct fdkhip_batch {
 	int nscan = 0, fmt = FMT_F32, math = FDKHIP_MATH_FAST;
 	std::vector<fdkhip_plan *> plans;
-	// one back_kernel launch per scan in turn, each plan with its own launch
-	// shape, instead of the persistent kernel (fdkhip_batch_set_per_scan)
+	// one back_kernel launch per scan, each plan with its own launch shape,
+	// instead of the persistent kernel (fdkhip_batch_set_per_scan); the
+	// launches go to two streams (the caller's and `side`), joined again
 	bool per_scan = false;
+	hipStream_t side = nullptr;
+	hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 	int nw = 8, zl = 16, vstage = 8, slot = 64, nwg = 256;
 	int vstage_req = 0;   // requested views per stage (0: largest that fits)
 	// per-scan slices per consumer wave: chosen by batch_configure
@@ -2039,6 +2042,9 @@ static int batch_free(fdkhip_batch *b)
 	quiesce(batch_device(b));
 	batch_drop_graph(b);
 	if (b->cap_stream) (void) hipStreamDestroy(b->cap_stream);
+	if (b->side) (void) hipStreamDestroy(b->side);
+	if (b->fork_ev) (void) hipEventDestroy(b->fork_ev);
+	if (b->join_ev) (void) hipEventDestroy(b->join_ev);
 	for (auto *p : b->plans)
 		plan_free(p);
 	(void) hipFree(b->d_scans);
@@ -2270,17 +2276,37 @@ extern "C" int fdkhip_batch_count_updates(fdkhip_batch *b, int64_t *n)
 }
 
 // the backprojection alone on the uploaded arguments: the persistent kernel
-// (with its queue reset), or each scan's own launch in turn
+// (with its queue reset), or each scan's own launch.  The per-scan launches
+// go to two streams, `s` and the batch's side stream (forked from `s` and
+// joined back into it), the scans in descending order of update count each
+// to the stream with fewer updates queued: the next scan's workgroups then
+// fill the CUs that a launch's last round leaves idle.
 static int batch_kernel_launch(fdkhip_batch *b, hipStream_t s)
 {
 	if (b->per_scan) {
-		for (int i = 0; i < b->nscan; ++i) {
-			const fdkhip_plan *p = b->plans[i];
-			if (p->ngroups == 0 || p->na == 0)
-				continue;
-			HIP_TRY(launch_back(b->host_args[i], p->nw, p->zl, p->zr, p->fmt,
-				p->math == FDKHIP_MATH_FAST, s));
+		if (!b->side) {
+			HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+			HIP_TRY(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+			HIP_TRY(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
 		}
+		std::vector<int> order;
+		for (int i = 0; i < b->nscan; ++i)
+			if (b->plans[i]->ngroups > 0 && b->plans[i]->na > 0)
+				order.push_back(i);
+		std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+			return b->plans[x]->n_updates > b->plans[y]->n_updates; });
+		HIP_TRY(hipEventRecord(b->fork_ev, s));
+		HIP_TRY(hipStreamWaitEvent(b->side, b->fork_ev, 0));
+		long long queued[2] = {0, 0};
+		for (int i : order) {
+			const fdkhip_plan *p = b->plans[i];
+			const int k = queued[1] < queued[0] ? 1 : 0;
+			queued[k] += p->n_updates;
+			HIP_TRY(launch_back(b->host_args[i], p->nw, p->zl, p->zr, p->fmt,
+				p->math == FDKHIP_MATH_FAST, k ? b->side : s));
+		}
+		HIP_TRY(hipEventRecord(b->join_ev, b->side));
+		HIP_TRY(hipStreamWaitEvent(s, b->join_ev, 0));
 		return FDKHIP_OK;
 	}
 	if (b->nitems == 0)
