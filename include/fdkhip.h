@@ -101,8 +101,9 @@ int fdkhip_fdk_ts_back(
 	const double *beta, int32_t nthread,
 	float *image);
 
-/* Frees the plans and device buffers that fdkhip_fdk_ts_back keeps between
- * calls, on every device; the next call sets them up again.  No counterpart
+/* Frees the plans, device buffers and pinned host chunks that
+ * fdkhip_fdk_ts_back and fdkhip_cbct_back keep between calls, on every
+ * device; the next call sets them up again.  No counterpart
  * in the reference (which allocates per call); for callers that need the
  * device memory back. */
 int fdkhip_release_host_cache(void);
@@ -118,6 +119,9 @@ int fdkhip_release_host_cache(void);
  * is 0.5 * deg2rad(|orbit|) / (na / ia_skip), or 1 without scale_dang).  The
  * permutes and the row replication run on the device; the backprojection is
  * the EXACT kernel of fdkhip_fdk_ts_back.  mask may be NULL.  Synchronous.
+ * Keeps its last plan between calls and shares the device buffers and pinned
+ * transfer chunks of fdkhip_fdk_ts_back (released by
+ * fdkhip_release_host_cache); calls on one device are serialised.
  */
 int fdkhip_cbct_back(const fdkhip_geom *g, const float *proj_sta, int32_t na,
 	const double *beta, int32_t ia_skip, const uint8_t *mask, int32_t extrapolate_t,

@@ -882,25 +882,31 @@ struct DevBuf {
 	~DevBuf() { if (p) (void) hipFree(p); }
 };
 
-// The host-array entry keeps, per device, the plan of its last call and its
-// device buffers (grown, never shrunk) until fdkhip_release_host_cache or
-// the end of the process: a call with the same geometry, view angles and
-// mask as the one before skips the plan set-up (tables, surveys, the padded
-// copy's allocation), and no call allocates device memory once the buffers
-// are large enough.  Calls on one device are serialised by its mutex.
-// Views and image move through two pinned host chunks: host threads copy one
-// chunk while the DMA engine moves the other, so that the caller's pageable
+// The host-array entries (fdkhip_fdk_ts_back, fdkhip_cbct_back) keep, per
+// device, the plan of each one's last call and their device buffers (grown,
+// never shrunk) until fdkhip_release_host_cache or the end of the process:
+// a call with the same geometry, view angles and mask as that entry's call
+// before skips the plan set-up (tables, surveys, the padded copy's
+// allocation), and no call allocates device memory once the buffers are
+// large enough.  Calls on one device are serialised by its mutex.  Views and
+// image move through two pinned host chunks: host threads copy one chunk
+// while the DMA engine moves the other, so that the caller's pageable
 // buffers are neither registered nor copied by the runtime's own staging.
 namespace {
 
 constexpr size_t kStageChunk = (size_t) 32 << 20;   // bytes per pinned chunk
+enum { kEntryTsBack = 0, kEntryCbctBack = 1, kEntries = 2, kCacheBufs = 4 };
+
+struct CachedPlan {
+	fdkhip_plan *plan = nullptr;
+	std::vector<unsigned char> key;    // geometry, na, beta and mask of `plan`
+};
 
 struct HostCache {
 	std::mutex mu;
-	fdkhip_plan *plan = nullptr;
-	std::vector<unsigned char> key;    // geometry, na, beta and mask of `plan`
-	void *dproj = nullptr, *dimg = nullptr;
-	size_t proj_cap = 0, img_cap = 0;  // bytes
+	CachedPlan plans[kEntries];
+	void *buf[kCacheBufs] = {};        // device buffers
+	size_t cap[kCacheBufs] = {};       // their sizes in bytes
 	char *pinned = nullptr;            // 2 x kStageChunk, hipHostMalloc
 	hipStream_t stream = nullptr;
 	hipEvent_t ev[2] = {nullptr, nullptr};
@@ -927,6 +933,30 @@ void cache_key(std::vector<unsigned char> &k, const fdkhip_geom &g, int32_t na,
 		memcpy(q, mask, nxy);
 }
 
+// the cached plan of an entry for this geometry, view angles and mask
+// (made anew when any of them differs from the entry's last call)
+int cached_plan(CachedPlan &cp, const fdkhip_geom &g, int32_t na, const double *beta,
+	const uint8_t *mask, fdkhip_plan *&out)
+{
+	std::vector<unsigned char> key;
+	cache_key(key, g, na, beta, mask);
+	if (!cp.plan || key != cp.key) {
+		if (cp.plan)
+			plan_free(cp.plan);
+		cp.plan = nullptr;
+		cp.key.clear();
+		const int rc = fdkhip_plan_create(&cp.plan, &g, na, beta, mask, FDKHIP_F32,
+			FDKHIP_MATH_EXACT);
+		if (rc) {
+			cp.plan = nullptr;
+			return rc;
+		}
+		cp.key.swap(key);
+	}
+	out = cp.plan;
+	return FDKHIP_OK;
+}
+
 // grow a cached device buffer to at least `bytes`
 int cache_grow(void *&buf, size_t &cap, size_t bytes)
 {
@@ -944,16 +974,19 @@ int cache_grow(void *&buf, size_t &cap, size_t bytes)
 
 void cache_clear(HostCache &c)
 {
-	if (c.plan)
-		plan_free(c.plan);   // waits for the device
-	c.plan = nullptr;
-	c.key.clear();
+	for (auto &cp : c.plans) {
+		if (cp.plan)
+			plan_free(cp.plan);   // waits for the device
+		cp.plan = nullptr;
+		cp.key.clear();
+	}
 	if (c.stream)
 		(void) hipStreamSynchronize(c.stream);
-	if (c.dproj) (void) hipFree(c.dproj);
-	if (c.dimg) (void) hipFree(c.dimg);
-	c.dproj = c.dimg = nullptr;
-	c.proj_cap = c.img_cap = 0;
+	for (int k = 0; k < kCacheBufs; ++k) {
+		if (c.buf[k]) (void) hipFree(c.buf[k]);
+		c.buf[k] = nullptr;
+		c.cap[k] = 0;
+	}
 	if (c.pinned) (void) hipHostFree(c.pinned);
 	c.pinned = nullptr;
 	for (auto &e : c.ev)
@@ -979,14 +1012,37 @@ int cache_staging(HostCache &c)
 	return FDKHIP_OK;
 }
 
-// memcpy of n bytes on up to 8 host threads (pieces of at least 1 MiB)
-void par_memcpy(void *dst, const void *src, size_t n)
+// A host byte stream: logical byte `off` is at src + (off / seg) * stride +
+// off % seg (seg == stride: contiguous).
+struct HostStream {
+	const char *src;
+	size_t seg, stride;
+	const char *at(size_t off, size_t &avail) const
+	{
+		const size_t k = off / seg, r = off - k * seg;
+		avail = seg - r;
+		return src + k * stride + r;
+	}
+};
+
+// bytes [off, off + n) of `in` into dst, on up to 8 host threads (pieces of
+// at least 1 MiB)
+void par_copy_in(char *dst, const HostStream &in, size_t off, size_t n)
 {
+	auto run = [&in, dst, off](size_t a, size_t b) {
+		while (a < b) {
+			size_t avail = 0;
+			const char *p = in.at(off + a, avail);
+			const size_t k = std::min(avail, b - a);
+			memcpy(dst + a, p, k);
+			a += k;
+		}
+	};
 	const size_t piece = (size_t) 1 << 20;
 	const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
 	const size_t nt = std::min<size_t>({(size_t) 8, (size_t) hw, (n + piece - 1) / piece});
 	if (nt <= 1) {
-		memcpy(dst, src, n);
+		run(0, n);
 		return;
 	}
 	const size_t part = (n + nt - 1) / nt;
@@ -994,16 +1050,21 @@ void par_memcpy(void *dst, const void *src, size_t n)
 	for (size_t t = 1; t < nt; ++t) {
 		const size_t a = t * part, b = std::min(n, a + part);
 		if (a < b)
-			th.emplace_back([=] { memcpy((char *) dst + a, (const char *) src + a, b - a); });
+			th.emplace_back(run, a, b);
 	}
-	memcpy(dst, src, std::min(n, part));
+	run(0, std::min(n, part));
 	for (auto &x : th)
 		x.join();
 }
 
+void par_memcpy(void *dst, const void *src, size_t n)
+{
+	par_copy_in((char *) dst, HostStream{(const char *) src, n ? n : 1, n ? n : 1}, 0, n);
+}
+
 // host -> device through the pinned chunks, on c.stream: chunk i is copied
 // into pinned half i % 2 once the DMA of chunk i - 2 has read it
-int staged_h2d(HostCache &c, void *dst, const void *src, size_t n)
+int staged_h2d(HostCache &c, void *dst, const HostStream &in, size_t n)
 {
 	size_t i = 0;
 	for (size_t off = 0; off < n; off += kStageChunk, ++i) {
@@ -1011,7 +1072,7 @@ int staged_h2d(HostCache &c, void *dst, const void *src, size_t n)
 		char *buf = c.pinned + (i & 1) * kStageChunk;
 		if (i >= 2)
 			HIP_TRY(hipEventSynchronize(c.ev[i & 1]));
-		par_memcpy(buf, (const char *) src + off, m);
+		par_copy_in(buf, in, off, m);
 		HIP_TRY(hipMemcpyAsync((char *) dst + off, buf, m, hipMemcpyHostToDevice, c.stream));
 		HIP_TRY(hipEventRecord(c.ev[i & 1], c.stream));
 	}
@@ -1061,7 +1122,12 @@ extern "C" int fdkhip_release_host_cache(void)
 	for (int d = 0; d < std::min(ndev, kCacheDevices); ++d) {
 		HostCache &c = g_host_cache[d];
 		std::lock_guard<std::mutex> lk(c.mu);
-		if (!c.plan && !c.dproj && !c.dimg)
+		bool any = c.pinned != nullptr;
+		for (const auto &cp : c.plans)
+			any = any || cp.plan != nullptr;
+		for (const void *b : c.buf)
+			any = any || b != nullptr;
+		if (!any)
 			continue;
 		(void) hipSetDevice(d);
 		cache_clear(c);
@@ -1093,32 +1159,24 @@ extern "C" int fdkhip_fdk_ts_back(
 		return fail(FDKHIP_EUNSUPPORTED, "device %d: at most %d devices", dev, kCacheDevices);
 	HostCache &c = g_host_cache[dev];
 	std::lock_guard<std::mutex> lk(c.mu);
-	std::vector<unsigned char> key;
-	cache_key(key, g, na, beta, mask);
-	if (!c.plan || key != c.key) {
-		if (c.plan)
-			plan_free(c.plan);
-		c.plan = nullptr;
-		c.key.clear();
-		rc = fdkhip_plan_create(&c.plan, &g, na, beta, mask, FDKHIP_F32, FDKHIP_MATH_EXACT);
-		if (rc) {
-			c.plan = nullptr;
-			return rc;
-		}
-		c.key.swap(key);
-	}
+	fdkhip_plan *p = nullptr;
+	rc = cached_plan(c.plans[kEntryTsBack], g, na, beta, mask, p);
+	if (rc)
+		return rc;
 	const size_t nproj = (size_t) ns * nt * na;
-	rc = cache_grow(c.dproj, c.proj_cap, std::max(nproj, (size_t) 1) * sizeof(float));
+	rc = cache_grow(c.buf[0], c.cap[0], std::max(nproj, (size_t) 1) * sizeof(float));
 	if (!rc)
-		rc = cache_grow(c.dimg, c.img_cap, nvox * sizeof(float));
+		rc = cache_grow(c.buf[1], c.cap[1], nvox * sizeof(float));
 	if (!rc)
 		rc = cache_staging(c);
+	const size_t pb = nproj * sizeof(float);
 	if (!rc)
-		rc = staged_h2d(c, c.dproj, proj, nproj * sizeof(float));
+		rc = staged_h2d(c, c.buf[0], HostStream{(const char *) proj, pb ? pb : 1, pb ? pb : 1},
+			pb);
 	if (!rc)
-		rc = fdkhip_plan_execute_direct(c.plan, c.dproj, nullptr, (float *) c.dimg, c.stream);
+		rc = fdkhip_plan_execute_direct(p, c.buf[0], nullptr, (float *) c.buf[1], c.stream);
 	if (!rc)
-		rc = staged_d2h(c, image, c.dimg, nvox * sizeof(float));
+		rc = staged_d2h(c, image, c.buf[1], nvox * sizeof(float));
 	if (rc) {
 		(void) hipStreamSynchronize(c.stream);   // no copy may still read a pinned chunk
 		return rc;
@@ -1164,56 +1222,53 @@ extern "C" int fdkhip_cbct_back(const fdkhip_geom *g, const float *proj_sta, int
 	std::vector<double> bsub((size_t) nsub);
 	for (int i = 0; i < nsub; ++i)
 		bsub[i] = beta[(size_t) i * ia_skip];
+	int dev = 0;
+	HIP_TRY(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kCacheDevices)
+		return fail(FDKHIP_EUNSUPPORTED, "device %d: at most %d devices", dev, kCacheDevices);
+	HostCache &c = g_host_cache[dev];
+	std::lock_guard<std::mutex> lk(c.mu);
 	fdkhip_plan *p = nullptr;
-	rc = fdkhip_plan_create(&p, &gk, nsub, bsub.data(), mask, FDKHIP_F32, FDKHIP_MATH_EXACT);
+	rc = cached_plan(c.plans[kEntryCbctBack], gk, nsub, bsub.data(), mask, p);
 	if (rc)
 		return rc;
 	const size_t view = (size_t) ns * nt, viewe = (size_t) ns * nte;
-	DevBuf dsta, dtsa, dzxy, dxyz;
-	hipError_t err = hipSuccess;
-	if (nsub > 0 && view > 0) {
-		err = hipMalloc(&dsta.p, view * nsub * sizeof(float));
-		if (err == hipSuccess)
-			err = hipMalloc(&dtsa.p, viewe * nsub * sizeof(float));
-	}
-	if (err == hipSuccess)
-		err = hipMalloc(&dzxy.p, nvox * sizeof(float));
-	if (err == hipSuccess)
-		err = hipMalloc(&dxyz.p, nvox * sizeof(float));
-	if (err != hipSuccess) {
-		plan_free(p);
-		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(err));
-	}
-#define CB_TRY(expr)                                                             \
-	do {                                                                         \
-		hipError_t e_ = (expr);                                                  \
-		if (e_ != hipSuccess) {                                                  \
-			plan_free(p);                                                        \
-			return fail(FDKHIP_EHIP, "%s: %s", #expr, hipGetErrorString(e_));    \
-		}                                                                        \
-	} while (0)
-	if (nsub > 0 && view > 0) {
-		// views 0, ia_skip, 2 ia_skip, ... of [ns nt na] in one strided copy
-		CB_TRY(hipMemcpy2D(dsta.p, view * sizeof(float), proj_sta,
-			view * ia_skip * sizeof(float), view * sizeof(float), nsub,
-			hipMemcpyHostToDevice));
+	const size_t nsta = view * nsub, ntsa = viewe * nsub;
+	// buffers: [ns nt nsub] views as given, [nt+2e ns nsub] views, the
+	// [nz nx ny] image, the [nx ny nz] image
+	rc = cache_grow(c.buf[0], c.cap[0], std::max(nsta, (size_t) 1) * sizeof(float));
+	if (!rc)
+		rc = cache_grow(c.buf[2], c.cap[2], std::max(ntsa, (size_t) 1) * sizeof(float));
+	if (!rc)
+		rc = cache_grow(c.buf[1], c.cap[1], nvox * sizeof(float));
+	if (!rc)
+		rc = cache_grow(c.buf[3], c.cap[3], nvox * sizeof(float));
+	if (!rc)
+		rc = cache_staging(c);
+	if (!rc && nsta > 0) {
+		// views 0, ia_skip, 2 ia_skip, ... of [ns nt na]
+		rc = staged_h2d(c, c.buf[0], HostStream{(const char *) proj_sta, view * sizeof(float),
+			view * ia_skip * sizeof(float)}, nsta * sizeof(float));
 		// [ns nt] -> [nt+2e ns] per view, edge rows replicated (cbct_back.m:86-98)
-		CB_TRY(launch_transpose((const float *) dsta.p, (float *) dtsa.p, nsub, nt, ns,
-			(long) view, ns, (long) viewe, nte, nte, e, 1.f, nullptr));
+		if (!rc && launch_transpose((const float *) c.buf[0], (float *) c.buf[2], nsub, nt, ns,
+				(long) view, ns, (long) viewe, nte, nte, e, 1.f, c.stream) != hipSuccess)
+			rc = fail(FDKHIP_EHIP, "launch_transpose (views)");
 	}
-	rc = plan_launch(p, dtsa.p, nullptr, (float *) dzxy.p, nullptr);
-	if (rc) {
-		plan_free(p);
-		return rc;
-	}
+	if (!rc)
+		rc = plan_launch(p, c.buf[2], nullptr, (float *) c.buf[1], c.stream);
 	// [nz nx ny] -> [nx ny nz] with the angle scale (cbct_back.m:104,133-135);
 	// the double scale multiplies a single image in single, as MATLAB does
-	CB_TRY(launch_transpose((const float *) dzxy.p, (float *) dxyz.p, g->ny, g->nx, g->nz,
-		(long) g->nx * g->nz, g->nz, g->nx, (long) g->nx * g->ny, g->nx, 0, (float) scale,
-		nullptr));
-	CB_TRY(hipMemcpy(image_xyz, dxyz.p, nvox * sizeof(float), hipMemcpyDeviceToHost));
-#undef CB_TRY
-	plan_free(p);
+	if (!rc && launch_transpose((const float *) c.buf[1], (float *) c.buf[3], g->ny, g->nx,
+			g->nz, (long) g->nx * g->nz, g->nz, g->nx, (long) g->nx * g->ny, g->nx, 0,
+			(float) scale, c.stream) != hipSuccess)
+		rc = fail(FDKHIP_EHIP, "launch_transpose (image)");
+	if (!rc)
+		rc = staged_d2h(c, image_xyz, c.buf[3], nvox * sizeof(float));
+	if (rc) {
+		(void) hipStreamSynchronize(c.stream);   // no copy may still read a pinned chunk
+		return rc;
+	}
+	HIP_TRY(hipStreamSynchronize(c.stream));
 	return FDKHIP_OK;
 }
 

@@ -46,3 +46,24 @@ def test_device_route_refuses_offset_t_with_extrapolation(device):
     cg, ig, proj = _scan("flat", seed=1, offset_t=0.5)
     with pytest.raises(ValueError, match="offset_t not yet done"):
         cbct_back(proj, cg, ig, extrapolate_t=2)
+
+
+def test_device_route_cache(device):
+    """fdkhip_cbct_back keeps its plan and buffers between calls (shared with
+    fdkhip_fdk_ts_back's): the same call twice, new views in the same
+    geometry, a new geometry, and the first again after
+    fdkhip_release_host_cache, each bit for bit equal to the host route."""
+    from mirt_amd import _lib, cbct_back, fdk_mex
+    cg, ig, proj = _scan("arc", seed=31)
+    cg2, ig2, proj2 = _scan("flat", seed=32)
+    rng = np.random.default_rng(33)
+    proj_b = np.asfortranarray(rng.standard_normal(proj.shape).astype(np.float32))
+    calls = [("first", cg, ig, proj), ("again", cg, ig, proj), ("new views", cg, ig, proj_b),
+             ("new geometry", cg2, ig2, proj2), ("after release", cg, ig, proj)]
+    for what, c, i, p in calls:
+        if what == "after release":
+            _lib.check(_lib.load().fdkhip_release_host_cache())
+        print(f"cbct_back cache: {what}", flush=True)
+        dev = cbct_back(p, c, i, ia_skip=2, extrapolate_t=1)
+        host = cbct_back(p, c, i, ia_skip=2, extrapolate_t=1, back_call=fdk_mex)
+        assert np.array_equal(dev, host), what
