@@ -102,8 +102,8 @@ int fdkhip_fdk_ts_back(
 	float *image);
 
 /* Frees the plans, device buffers and pinned host chunks that
- * fdkhip_fdk_ts_back and fdkhip_cbct_back keep between calls, on every
- * device; the next call sets them up again.  No counterpart
+ * fdkhip_fdk_ts_back, fdkhip_cbct_back and fdkhip_fdk_ts_back_helix keep
+ * between calls, on every device; the next call sets them up again.  No counterpart
  * in the reference (which allocates per call); for callers that need the
  * device memory back. */
 int fdkhip_release_host_cache(void);
@@ -154,7 +154,9 @@ int fdkhip_fdk_st_back(
  * mex/src/fdk-helix/mex/fdk,ts,t.c:107-167 with every branch of fdk_ts_back1
  * (fdk,ts.c:55-251), bit for bit.  w3d table entries past the reference's
  * allocation read 0.  Synchronous.  cone_par = 0 runs on a helical plan
- * (fdkhip_plan_create_helix); cone_par = 1 on the dedicated helical kernel.
+ * (fdkhip_plan_create_helix), kept between calls with the same operands;
+ * cone_par = 1 on the dedicated helical kernel.  Shares the cached device
+ * buffers and pinned transfer chunks of fdkhip_fdk_ts_back.
  */
 int fdkhip_fdk_ts_back_helix(
 	const int32_t nxyz[3], const double dxyz[3], const double cxyz[3],

@@ -882,7 +882,8 @@ struct DevBuf {
 	~DevBuf() { if (p) (void) hipFree(p); }
 };
 
-// The host-array entries (fdkhip_fdk_ts_back, fdkhip_cbct_back) keep, per
+// The host-array entries (fdkhip_fdk_ts_back, fdkhip_cbct_back,
+// fdkhip_fdk_ts_back_helix) keep, per
 // device, the plan of each one's last call and their device buffers (grown,
 // never shrunk) until fdkhip_release_host_cache or the end of the process:
 // a call with the same geometry, view angles and mask as that entry's call
@@ -895,7 +896,7 @@ struct DevBuf {
 namespace {
 
 constexpr size_t kStageChunk = (size_t) 32 << 20;   // bytes per pinned chunk
-enum { kEntryTsBack = 0, kEntryCbctBack = 1, kEntries = 2, kCacheBufs = 4 };
+enum { kEntryTsBack = 0, kEntryCbctBack = 1, kEntryHelix = 2, kEntries = 3, kCacheBufs = 4 };
 
 struct CachedPlan {
 	fdkhip_plan *plan = nullptr;
@@ -916,10 +917,11 @@ constexpr int kCacheDevices = 64;
 HostCache g_host_cache[kCacheDevices];
 
 void cache_key(std::vector<unsigned char> &k, const fdkhip_geom &g, int32_t na,
-	const double *beta, const uint8_t *mask)
+	const double *beta, const uint8_t *mask, const std::vector<unsigned char> &extra = {})
 {
 	const size_t nxy = (size_t) g.nx * g.ny;
-	k.resize(sizeof g + sizeof na + sizeof(double) * (size_t) na + 1 + (mask ? nxy : 0));
+	k.resize(sizeof g + sizeof na + sizeof(double) * (size_t) na + 1 + (mask ? nxy : 0)
+		+ extra.size());
 	unsigned char *q = k.data();
 	memcpy(q, &g, sizeof g);
 	q += sizeof g;
@@ -931,22 +933,37 @@ void cache_key(std::vector<unsigned char> &k, const fdkhip_geom &g, int32_t na,
 	*q++ = mask ? 1 : 0;
 	if (mask)
 		memcpy(q, mask, nxy);
+	q += mask ? nxy : 0;
+	if (!extra.empty())
+		memcpy(q, extra.data(), extra.size());
 }
 
 // the cached plan of an entry for this geometry, view angles and mask
 // (made anew when any of them differs from the entry's last call)
+// (source_zs: a helical plan with na1_half and pitch, which join the key)
 int cached_plan(CachedPlan &cp, const fdkhip_geom &g, int32_t na, const double *beta,
-	const uint8_t *mask, fdkhip_plan *&out)
+	const uint8_t *mask, fdkhip_plan *&out, const double *source_zs = nullptr,
+	int32_t na1_half = 0, float pitch = 0.f)
 {
-	std::vector<unsigned char> key;
-	cache_key(key, g, na, beta, mask);
+	std::vector<unsigned char> key, extra;
+	if (source_zs) {
+		extra.resize(sizeof(double) * (size_t) na + sizeof na1_half + sizeof pitch);
+		if (na > 0)
+			memcpy(extra.data(), source_zs, sizeof(double) * (size_t) na);
+		memcpy(extra.data() + sizeof(double) * (size_t) na, &na1_half, sizeof na1_half);
+		memcpy(extra.data() + sizeof(double) * (size_t) na + sizeof na1_half, &pitch,
+			sizeof pitch);
+	}
+	cache_key(key, g, na, beta, mask, extra);
 	if (!cp.plan || key != cp.key) {
 		if (cp.plan)
 			plan_free(cp.plan);
 		cp.plan = nullptr;
 		cp.key.clear();
-		const int rc = fdkhip_plan_create(&cp.plan, &g, na, beta, mask, FDKHIP_F32,
-			FDKHIP_MATH_EXACT);
+		const int rc = source_zs
+			? fdkhip_plan_create_helix(&cp.plan, &g, na, beta, source_zs, na1_half, pitch,
+				mask, FDKHIP_F32, FDKHIP_MATH_EXACT)
+			: fdkhip_plan_create(&cp.plan, &g, na, beta, mask, FDKHIP_F32, FDKHIP_MATH_EXACT);
 		if (rc) {
 			cp.plan = nullptr;
 			return rc;
@@ -1535,27 +1552,46 @@ extern "C" int fdkhip_fdk_ts_back_helix(
 	if (nvox == 0)
 		return FDKHIP_OK;
 	const size_t nproj = (size_t) ns * nt * na;
-	DevBuf dproj, dimg;
-	hipError_t e = hipMalloc(&dimg.p, nvox * sizeof(float));
-	if (e == hipSuccess && nproj > 0)
-		e = hipMalloc(&dproj.p, nproj * sizeof(float));
-	if (e != hipSuccess)
-		return fail(FDKHIP_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
-	if (nproj > 0)
-		HIP_TRY(hipMemcpy(dproj.p, proj, nproj * sizeof(float), hipMemcpyHostToDevice));
-	a.proj = (const float *) dproj.p;
-	a.image = (float *) dimg.p;
+	// the cached buffers, pinned transfers and (native helical geometry) plan
+	// of the host-array entries
+	int dev = 0;
+	HIP_TRY(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kCacheDevices)
+		return fail(FDKHIP_EUNSUPPORTED, "device %d: at most %d devices", dev, kCacheDevices);
+	HostCache &c = g_host_cache[dev];
+	std::lock_guard<std::mutex> lk(c.mu);
+	rc = cache_grow(c.buf[0], c.cap[0], std::max(nproj, (size_t) 1) * sizeof(float));
+	if (!rc)
+		rc = cache_grow(c.buf[1], c.cap[1], nvox * sizeof(float));
+	if (!rc)
+		rc = cache_staging(c);
+	if (rc)
+		return rc;
+	a.proj = (const float *) c.buf[0];
+	a.image = (float *) c.buf[1];
+	const size_t pb = nproj * sizeof(float);
 	if (nproj == 0) {
-		HIP_TRY(launch_fill(a.image, (long) g.nx * g.ny, g.nz, 0, 0, nullptr));
-		HIP_TRY(hipDeviceSynchronize());
-	} else if (!cone_par) {
-		if ((rc = helix_plan_run(&g, a.proj, na, beta, source_zs, na1_half, pitch, mask,
-				a.image, nullptr, nullptr)))
-			return rc;
-	} else if ((rc = helix_run(a, h, mask, nullptr, nullptr))) {
+		if (launch_fill(a.image, (long) g.nx * g.ny, g.nz, 0, 0, c.stream) != hipSuccess)
+			rc = fail(FDKHIP_EHIP, "launch_fill");
+	} else {
+		rc = staged_h2d(c, c.buf[0], HostStream{(const char *) proj, pb, pb}, pb);
+		if (!rc && !cone_par) {
+			fdkhip_plan *p = nullptr;
+			rc = cached_plan(c.plans[kEntryHelix], g, na, beta, mask, p, source_zs, na1_half,
+				pitch);
+			if (!rc)
+				rc = plan_launch(p, a.proj, nullptr, a.image, c.stream);
+		} else if (!rc) {
+			rc = helix_run(a, h, mask, c.stream, nullptr);
+		}
+	}
+	if (!rc)
+		rc = staged_d2h(c, image, a.image, nvox * sizeof(float));
+	if (rc) {
+		(void) hipStreamSynchronize(c.stream);   // no copy may still read a pinned chunk
 		return rc;
 	}
-	HIP_TRY(hipMemcpy(image, dimg.p, nvox * sizeof(float), hipMemcpyDeviceToHost));
+	HIP_TRY(hipStreamSynchronize(c.stream));
 	return FDKHIP_OK;
 }
 
