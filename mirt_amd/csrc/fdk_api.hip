@@ -1064,10 +1064,16 @@ void par_copy_in(char *dst, const HostStream &in, size_t off, size_t n)
 	}
 	const size_t part = (n + nt - 1) / nt;
 	std::vector<std::thread> th;
+	th.reserve(nt);
 	for (size_t t = 1; t < nt; ++t) {
 		const size_t a = t * part, b = std::min(n, a + part);
-		if (a < b)
+		if (a >= b)
+			continue;
+		try {
 			th.emplace_back(run, a, b);
+		} catch (...) {
+			run(a, b);   // no thread to be had: this piece on the calling thread
+		}
 	}
 	run(0, std::min(n, part));
 	for (auto &x : th)
