@@ -636,6 +636,31 @@ def run_helix(args, dev):
                              f"{kernel_ms:.3f} ms back_kernel (HIP events on its stream)"},
         "cpu_baseline": None,
     }
+    # the other arithmetic on the same helical plans: FAST FP32 and FP16/BF16
+    # storage, each against the EXACT image over the whole volume
+    from mirt_amd import quantize_views
+    ref = img.clone()
+    zs_all = np.broadcast_to(np.asarray(cg.source_zs, np.float64), (cg.na,))
+    others = {}
+    for key, fmt, mm in (("f32_fast", "f32", "fast"), ("f16", "f16", "fast"),
+                         ("bf16", "bf16", "fast")):
+        inp = (tsa,) if fmt == "f32" else quantize_views(tsa, fmt)
+        p2 = FdkPlan(plan.geom, beta, np.asarray(ig.mask_or), fmt=fmt, math_mode=mm,
+                     helix=(zs_all, na1_half, float(cg.pitch)))
+        if not args.no_autotune:
+            p2.autotune(*inp, out=img, reps=2)
+        ms2 = time_steps(lambda: p2(*inp, out=img), args.steps, args.warmup)
+        k2 = p2.profile(*inp, out=img, reps=max(2, args.steps))["back"]
+        p2(*inp, out=img)
+        torch.cuda.synchronize()
+        err = float(torch.linalg.vector_norm(img - ref) / torch.linalg.vector_norm(ref))
+        c2 = p2.config
+        others[key] = {"ms_per_step": ms2, "kernel_ms": k2,
+                       "value": U / (ms2 * 1e-3), "relL2_vs_exact": err,
+                       "launch": f"waves={c2['waves']} zper={c2['zper']} vstage={c2['vstage']}"}
+        log(f"helix {key}: {ms2:.3f} ms/step, back_kernel {k2:.3f} ms, relL2 vs EXACT {err:.2e}")
+        del p2
+    result["other_paths"] = others
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         if O.have_reference("fixed"):
