@@ -316,7 +316,7 @@ static bool choose_layout(int nw, int fmt, int maxarea, int vstage_req, int slot
 }
 
 // zper slices per consumer wave: one chunk of zper slices where that shape is
-// built, else chunks of 16 interleaved over the waves (FP32 walk modes)
+// built, else chunks of 16 interleaved over the waves
 static bool split_zper(int nw, int zper, int fmt, int &zl, int &zr)
 {
 	if (back_config_supported(nw, zper, 1, fmt)) {
@@ -1890,8 +1890,8 @@ extern "C" int fdkhip_cbct_back_helix_mat(const fdkhip_geom *g, const void *proj
 namespace {
 
 // launch shapes (consumer waves, slices per lane) the library is built with;
-// a plan's FP32 walk modes also have shapes of several interleaved 16-slice
-// chunks per wave (4 x 48, 4 x 64, 8 x 32), the batch kernel does not
+// plans also have shapes of several interleaved 16-slice chunks per wave
+// (4 x 48, 4 x 64, 8 x 32), the batch kernel does not
 const int kShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}};
 const int kPlanShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}, {4, 48}, {4, 64},
 	{8, 32}};

@@ -108,8 +108,8 @@ struct MatArgs {
 };
 
 size_t back_lds_bytes(int nw, int vstage, int slot, int fmt);
-// nw consumer waves, zl slices per chunk, zr chunks per wave (interleaved;
-// zr > 1: FP32 walk modes only); the survey takes the z-group's chunk count
+// nw consumer waves, zl slices per chunk, zr chunks per wave (interleaved);
+// the survey takes the z-group's chunk count
 bool back_config_supported(int nw, int zl, int zr, int fmt);
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int zr, int fmt, bool fast,
 	hipStream_t s);

@@ -1030,7 +1030,6 @@ template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1, bool WIN = fals
 __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int group)
 {
 	static_assert(!WIN || FMT != FMT_F32, "the walk modes keep to the windows by their ranges");
-	static_assert(R == 1 || FMT == FMT_F32, "several chunks per wave: FP32 walk modes only");
 	constexpr int NC = NW * R;   // ZL-slice chunks of the z-group
 	const GeomF &g = a.g;
 	const int tid = threadIdx.x;
@@ -1091,9 +1090,15 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 			if constexpr (FMT == FMT_F32)
 				consume_walk<NW, R, ZL, FMT, FAST>(a, (s & 1) * L.total, L, v0,
 					min(a.vstage, vb - v0), wave, lane, zg0, accs);
-			else
-				consume_direct<NW, ZL, FMT, WIN>(a, (s & 1) * L.total, L, v0,
-					min(a.vstage, vb - v0), wave, lane, zg0 + wave * ZL, accs[0]);
+			else {
+				// chunk wave + r NW of the z-group, one pass over the stage's
+				// views each (u is in closed form from the group's U0)
+#pragma unroll
+				for (int r = 0; r < R; ++r)
+					consume_direct<NC, ZL, FMT, WIN>(a, (s & 1) * L.total, L, v0,
+						min(a.vstage, vb - v0), wave + r * NW, lane,
+						zg0 + (wave + r * NW) * ZL, accs[r]);
+			}
 			__syncthreads();
 		}
 		// one write per voxel, [nz nx ny]
@@ -1514,12 +1519,12 @@ hipError_t launch_back(const BackArgs &a, int nw, int zl, int zr, int fmt, bool 
 {
 	const size_t lds = back_lds_bytes(nw * zr, a.vstage, a.slot, fmt);
 	if (zr > 1) {
-		// FP32 walk modes, chunks interleaved over the waves
-		if (fmt != FMT_F32)
-			return hipErrorInvalidValue;
+		// chunks interleaved over the waves
 #define FDK_DISPATCH_R(NW, ZL, R)                                                            \
 		if (nw == NW && zl == ZL && zr == R) {                                               \
 			constexpr int NP_ = (NW) >= 8 ? 8 : 4;                                           \
+			if (fmt == FMT_F16) return launch_back_t<NW, NP_, ZL, FMT_F16, true, R>(a, lds, s); \
+			if (fmt == FMT_BF16) return launch_back_t<NW, NP_, ZL, FMT_BF16, true, R>(a, lds, s); \
 			return fast ? launch_back_t<NW, NP_, ZL, FMT_F32, true, R>(a, lds, s)            \
 				: launch_back_t<NW, NP_, ZL, FMT_F32, false, R>(a, lds, s);                  \
 		}
@@ -1611,9 +1616,9 @@ int batch_blocks_per_cu(int nw, int zl, int vstage, int slot, int fmt, bool fast
 
 bool back_config_supported(int nw, int zl, int zr, int fmt)
 {
+	(void) fmt;
 	if (zr > 1)
-		return fmt == FMT_F32 && zl == 16 && ((nw == 4 && (zr == 3 || zr == 4))
-			|| (nw == 8 && zr == 2));
+		return zl == 16 && ((nw == 4 && (zr == 3 || zr == 4)) || (nw == 8 && zr == 2));
 	return zr == 1 && ((zl == 16 && (nw == 4 || nw == 8)) || (zl == 8 && (nw == 4 || nw == 8))
 		|| (zl == 32 && nw == 4));
 }

@@ -208,7 +208,9 @@ def test_helix_plan_half_storage(device, fmt, pitch):
     (relative L2 1e-3 for FP16, 4e-3 for BF16, the formats' unit roundoffs
     2^-11 and 2^-8 with margin), also where the views' slice windows, not
     the cone, end the slice ranges (pitch 0.3): a slice outside a view's
-    window must add nothing even where its row is on the detector."""
+    window must add nothing even where its row is on the detector.  The
+    default launch shape and every shape with one or several chunks per
+    consumer wave."""
     import torch
     from mirt_amd.cbct_back import quantize_views
     g, proj, beta, zs, mask, ops = _seeded(0.0, pitch=pitch, cone_par=0, w3d=0, seed=43, nz=70)
@@ -217,7 +219,10 @@ def test_helix_plan_half_storage(device, fmt, pitch):
     plan = _helix_plan(g, beta, zs, mask, ops, fmt=fmt)
     pt = torch.from_numpy(np.ascontiguousarray(proj)).to(device)
     q, sc = quantize_views(pt, fmt)
-    got = plan(q, sc).cpu().numpy()
-    err = O.rel_l2(got, want)
-    print(f"helix plan {fmt} pitch {pitch}: relL2 {err:.2e}", flush=True)
-    assert err <= {"f16": 1e-3, "bf16": 4e-3}[fmt], err
+    for waves, zper in [(0, 0), (4, 16), (8, 8), (4, 48), (4, 64), (8, 32)]:
+        if waves:
+            plan.set_config(waves, zper, 0)
+        got = plan(q, sc).cpu().numpy()
+        err = O.rel_l2(got, want)
+        print(f"helix plan {fmt} pitch {pitch} {plan.config}: relL2 {err:.2e}", flush=True)
+        assert err <= {"f16": 1e-3, "bf16": 4e-3}[fmt], (waves, zper, err)

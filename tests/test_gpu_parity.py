@@ -229,12 +229,12 @@ def test_all_configs(device, waves, zper):
     assert O.rel_l2(img, want) <= 1e-5
 
 
-@pytest.mark.parametrize("waves,zper", [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32)])
+@pytest.mark.parametrize("waves,zper", [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32), (4, 48),
+                                        (4, 64), (8, 32)])
 def test_all_configs_f16(device, waves, zper):
-    """FP16 storage at every built launch shape (consumers of 16 or fewer
-    slices take the independent-dot slice update, 32-slice ones the chained
-    one) within 1e-3 of the FP64 restatement, with the reference's own error
-    on the same views as the scale of comparison."""
+    """FP16 storage at every built launch shape (including several interleaved
+    16-slice chunks per consumer wave: 4 x 48, 4 x 64, 8 x 32) within 1e-3 of
+    the FP64 restatement."""
     import torch
     from mirt_amd import quantize_views
     g, proj, beta, mask = make_case("arc_tall", seed=9, data="smooth")
@@ -252,7 +252,7 @@ def _stored(q, sc):
     return (q.float() * sc[:, None, None]).cpu().numpy()
 
 
-SHAPES = [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32)]
+SHAPES = [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32), (4, 48), (4, 64), (8, 32)]
 
 
 @pytest.mark.parametrize("fmt", ["f16", "bf16"])
@@ -299,12 +299,13 @@ def test_direct_hbm_gather_path(device, fmt):
     import torch
     from mirt_amd import quantize_views
     for name in ("flat_full", "wide_cone"):
-        g, proj, beta, mask = make_case(name, seed=13, data="noise")
-        print(f"case={name} fmt={fmt} slot cap 128", flush=True)
-        q, sc = quantize_views(torch.from_numpy(proj).to(device), fmt)
-        plan = _plan(g, beta, mask, fmt)
-        plan.set_config(8, 16, 0, 128)
-        img = plan(q, sc).cpu().numpy()
-        err = O.rel_l2(img, O.back_ts_f64(g, _stored(q, sc), beta, mask))
-        print(f"  relL2 = {err:.3e}", flush=True)
-        assert err <= 1e-5, err
+        for waves, zper in ((8, 16), (4, 48)):
+            g, proj, beta, mask = make_case(name, seed=13, data="noise")
+            print(f"case={name} fmt={fmt} waves={waves} zper={zper} slot cap 128", flush=True)
+            q, sc = quantize_views(torch.from_numpy(proj).to(device), fmt)
+            plan = _plan(g, beta, mask, fmt)
+            plan.set_config(waves, zper, 0, 128)
+            img = plan(q, sc).cpu().numpy()
+            err = O.rel_l2(img, O.back_ts_f64(g, _stored(q, sc), beta, mask))
+            print(f"  relL2 = {err:.3e}", flush=True)
+            assert err <= 1e-5, err
