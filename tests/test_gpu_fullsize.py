@@ -18,9 +18,10 @@ rest of the volume is held to size-independent properties:
   (FAST FP32) over the whole volume -- FP32 rounding of ~1000-view sums of
   white noise is ~sqrt(na) * 2^-24 ~ 2e-6, a dropped or doubled view ~3e-2;
 * batches of uneven scans (all 32 seeded scans of the batched configuration,
-  one persistent launch replayed from a HIP graph): in EXACT mode every
-  scan's sampled columns bit-for-bit equal to the oracle; with FP16 storage,
-  autotuned, every scan within 1e-3 of the FP64 oracle on sampled columns;
+  replayed from a HIP graph, as the persistent kernel and as one launch per
+  scan): in EXACT mode every scan's sampled columns bit-for-bit equal to the
+  oracle; with FP16 storage, autotuned, every scan within 1e-3 of the FP64
+  oracle on sampled columns;
 * whole volume at ``clinical_arc``: FP16 and BF16 storage of the filtered
   phantom views within 1e-3 of the FP64 oracle over every voxel;
 * ``clinical_helix`` on helical plans, on a column mask of seeded columns:
@@ -147,9 +148,11 @@ def test_fullsize_additivity(device, name):
     assert err <= 1e-5, err
 
 
-def test_fullsize_uneven_batch(device):
-    """All 32 uneven scans in one EXACT batch (graph replay, twice):
-    sampled columns of every scan bit for bit equal to the oracle."""
+@pytest.mark.parametrize("per_scan", [False, True])
+def test_fullsize_uneven_batch(device, per_scan):
+    """All 32 uneven scans in one EXACT batch (graph replay, twice), as the
+    persistent kernel and as one launch per scan on two streams: sampled
+    columns of every scan bit for bit equal to the oracle."""
     import torch
     from mirt_amd import FdkBatch, geom_of
     from mirt_amd.cbct_back import _views
@@ -159,6 +162,9 @@ def test_fullsize_uneven_batch(device):
         f"{s.ig.nx}^2x{s.ig.nz}/{s.cg.na}v/{s.cg.ns}x{s.cg.nt}" for s in scans), flush=True)
     betas = [_views(s.cg, s.cg.na) for s in scans]
     batch = FdkBatch([(geom_of(s.cg, s.ig), b, None) for s, b in zip(scans, betas)])
+    if per_scan:
+        batch.set_per_scan()
+    print(f"  per_scan={batch.per_scan}", flush=True)
     projs = [_noise_views(s.cg, device, seed=100 + i) for i, s in enumerate(scans)]
     outs = batch(projs)
     first = [o.clone() for o in outs]
@@ -177,7 +183,8 @@ def test_fullsize_uneven_batch(device):
 def test_fullsize_uneven_batch_f16(device):
     """The batched configuration at its specification: 32 seeded uneven scans
     of FDK-filtered phantom views, FP16 storage with per-view scale, one
-    autotuned persistent launch replayed from a HIP graph; every scan within
+    autotuned batch execute replayed from a HIP graph (the persistent kernel
+    or one launch per scan, whichever the tuner keeps); every scan within
     1e-3 of the FP64 oracle on 64 sampled columns (all slices, all views)."""
     import torch
     from mirt_amd import FdkBatch, geom_of, quantize_views
