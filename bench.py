@@ -271,18 +271,20 @@ def run_single(args, dev):
         a = prepare(tsa, fmt)
         out = torch.empty((ig.ny, ig.nx, ig.nz), dtype=torch.float32, device=dev)
         if args.launch:
-            plan.set_config(*args.launch)
+            plan.set_config(*args.launch[:3])
+            if len(args.launch) > 3:
+                plan.set_occupancy(args.launch[3])
         elif not args.no_autotune:
             t0 = time.perf_counter()
             best = plan.autotune(*a, out=out, reps=2)
-            log(f"{name}: autotune {time.perf_counter() - t0:.1f} s -> {plan.config} "
-                f"({best:.3f} ms)")
+            log(f"{name}: autotune {time.perf_counter() - t0:.1f} s -> {plan.config}, "
+                f"{plan.occupancy} waves/SIMD ({best:.3f} ms)")
         paths[name] = time_steps(lambda: plan(*a, out=out), args.steps, args.warmup)
         kernel[name] = plan.profile(*a, out=out, reps=max(2, args.steps))
         images[name] = out
         if name == head:
             U_eff = plan.count_updates()
-            cfg = plan.config
+            cfg = dict(plan.config, occupancy=plan.occupancy)
         log(f"{name}: {paths[name]:.3f} ms/step, kernel {kernel[name]['back']:.3f} ms, "
             f"{U / paths[name] * 1e3:.3e} updates/s, config {plan.config}")
         del plan, a
@@ -588,7 +590,9 @@ def run_helix(args, dev):
     img = torch.empty((ig.ny, ig.nx, ig.nz), dtype=torch.float32, device=dev)
     plan = FdkPlan.helix_from_geoms(cg, ig)
     if args.launch:
-        plan.set_config(*args.launch)
+        plan.set_config(*args.launch[:3])
+        if len(args.launch) > 3:
+            plan.set_occupancy(args.launch[3])
     elif not args.no_autotune:
         t0 = time.perf_counter()
         plan.autotune(tsa, out=img, reps=2)
@@ -725,6 +729,8 @@ def slab_times(args, geom, beta, mask, ig, tsa, head, cfg, axis):
             msk = np.asarray(mask)[:, gi * ns_:(gi + 1) * ns_]
         plan = FdkPlan(gs, beta, msk, fmt=fmt, math_mode=mm)
         plan.set_config(cfg["waves"], cfg["zper"], cfg["vstage"])
+        if "occupancy" in cfg:
+            plan.set_occupancy(cfg["occupancy"])
         out = torch.empty((gs.ny, gs.nx, gs.nz), dtype=torch.float32, device=tsa.device)
         ms = plan.profile(*a, out=out, reps=max(2, args.steps))["back"]
         rows.append({"part": gi, axis: [gi * ns_, (gi + 1) * ns_], "kernel_ms": ms,
@@ -794,7 +800,7 @@ def run_batch(args, dev, errors_without_baseline=False):
     outs = [torch.empty((sc.ig.ny, sc.ig.nx, sc.ig.nz), dtype=torch.float32, device=dev)
             for sc in scans]
     if args.launch:
-        batch.set_config(*args.launch)
+        batch.set_config(*args.launch[:3])
     elif not args.no_autotune:
         t0 = time.perf_counter()
         tuned = batch.autotune(projs, scales, outs, reps=2)
@@ -908,8 +914,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the plans' default launch shape instead of the tuned one")
     ap.add_argument("--launch", type=lambda v: tuple(int(x) for x in v.split(",")),
-                    default=None, metavar="WAVES,ZPER,VSTAGE",
-                    help="fixed launch shape (no autotune), e.g. 4,32,4 (profiling runs)")
+                    default=None, metavar="WAVES,ZPER,VSTAGE[,WAVES_PER_SIMD]",
+                    help="fixed launch shape (no autotune), e.g. 4,32,2,8 (profiling runs); "
+                         "the fourth value picks the 4- or 8-waves-per-SIMD kernel build")
     ap.add_argument("--fmt-only", action="store_true",
                     help="time only the headline path (profiling runs)")
     ap.add_argument("--filter-only", action="store_true",

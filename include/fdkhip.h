@@ -291,6 +291,18 @@ int fdkhip_plan_config(const fdkhip_plan *plan, int32_t cfg[6]);
 int fdkhip_plan_set_config(fdkhip_plan *plan, int32_t waves, int32_t zper,
 	int32_t vstage, int32_t slot_cap);
 
+/* Minimum waves per SIMD of the kernel build the plan launches: 4 (the
+ * default; up to 128 VGPRs a lane) or 8 (up to 64), built for one-chunk
+ * FP16/BF16 launch shapes only, where small LDS stages then let four 8-wave
+ * workgroups share a CU.  Shapes without an 8-wave build launch the 4-wave
+ * one.  fdkhip_plan_autotune chooses it with the shape; results are the
+ * same bits either way.  No reference counterpart (a launch parameter). */
+int fdkhip_plan_set_occupancy(fdkhip_plan *plan, int32_t waves_per_simd);
+
+/* *waves_per_simd = the minimum waves per SIMD of the build the plan's
+ * current shape launches. */
+int fdkhip_plan_occupancy(const fdkhip_plan *plan, int32_t *waves_per_simd);
+
 /* Waits for the plan's device to finish its queued work (a launch of the
  * plan may still be running on any stream), then frees the plan. */
 int fdkhip_plan_destroy(fdkhip_plan *plan);

@@ -86,6 +86,8 @@ _SIGS = {
     "fdkhip_plan_execute_direct": (C.c_int, [_P, _P, _P, _P, _P]),
     "fdkhip_plan_config": (C.c_int, [_P, _P]),
     "fdkhip_plan_set_config": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "fdkhip_plan_set_occupancy": (C.c_int, [_P, C.c_int32]),
+    "fdkhip_plan_occupancy": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "fdkhip_plan_destroy": (C.c_int, [_P]),
     "fdkhip_plan_count_updates": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "fdkhip_quantize_views": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,

@@ -460,6 +460,19 @@ class FdkPlan:
         _lib.check(self.lib.fdkhip_plan_set_config(self.handle, waves, zper, vstage,
                                                    slot_cap))
 
+    @property
+    def occupancy(self) -> int:
+        """Minimum waves per SIMD of the kernel build the plan launches (4, or
+        8 for one-chunk FP16/BF16 shapes when chosen)."""
+        w = C.c_int32()
+        _lib.check(self.lib.fdkhip_plan_occupancy(self.handle, C.byref(w)))
+        return int(w.value)
+
+    def set_occupancy(self, waves_per_simd):
+        """Choose the 4- or 8-waves-per-SIMD kernel build
+        (fdkhip_plan_set_occupancy)."""
+        _lib.check(self.lib.fdkhip_plan_set_occupancy(self.handle, int(waves_per_simd)))
+
     def count_updates(self) -> int:
         n = C.c_int64()
         _lib.check(self.lib.fdkhip_plan_count_updates(self.handle, C.byref(n)))

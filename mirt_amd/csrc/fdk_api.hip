@@ -203,6 +203,7 @@ struct fdkhip_plan {
 	// launch configuration
 	int nw = 8, zl = 16, vstage = 8, slot = 0;
 	int zr = 1;                 // chunks of zl slices per consumer wave
+	int minw = 4;               // kernel build: minimum waves per SIMD (4, or 8)
 	int z0 = 0, ngroups = 0, tiles_x = 0, tiles_y = 0;
 	int zs_lo = 0, zs_hi = 0;
 	long long n_updates = -1;
@@ -330,6 +331,13 @@ static bool split_zper(int nw, int zper, int fmt, int &zl, int &zr)
 		return true;
 	}
 	return false;
+}
+
+// the kernel build a plan launches: its chosen minimum waves per SIMD where
+// the current shape has that build, else 4
+static int plan_minw(const fdkhip_plan *p)
+{
+	return back_minw_supported(p->minw, p->zr, p->fmt) ? p->minw : 4;
 }
 
 static int configure(fdkhip_plan *p, int nw, int zper, int vstage_req, int slot_cap)
@@ -612,6 +620,26 @@ extern "C" int fdkhip_plan_set_config(fdkhip_plan *p, int32_t waves, int32_t zpe
 	return configure(p, waves, zper, vstage, slot_cap);
 }
 
+extern "C" int fdkhip_plan_set_occupancy(fdkhip_plan *p, int32_t waves_per_simd)
+{
+	if (!p)
+		return fail(FDKHIP_EARG, "plan is NULL");
+	if (waves_per_simd != 4 && waves_per_simd != 8)
+		return fail(FDKHIP_EARG, "waves per SIMD must be 4 or 8 (%d)", waves_per_simd);
+	if (p->minw != waves_per_simd)
+		drop_graph(p);
+	p->minw = waves_per_simd;
+	return FDKHIP_OK;
+}
+
+extern "C" int fdkhip_plan_occupancy(const fdkhip_plan *p, int32_t *waves_per_simd)
+{
+	if (!p || !waves_per_simd)
+		return fail(FDKHIP_EARG, "NULL argument");
+	*waves_per_simd = plan_minw(p);
+	return FDKHIP_OK;
+}
+
 extern "C" int fdkhip_plan_count_updates(fdkhip_plan *p, int64_t *n)
 {
 	if (!p || !n)
@@ -643,7 +671,8 @@ static int plan_launch(fdkhip_plan *p, const void *proj, const float *scale, flo
 		HIP_TRY(hipEventRecord(ev[2], s));
 	if (work) {
 		BackArgs a = make_args(p, proj, scale, image);
-		HIP_TRY(launch_back(a, p->nw, p->zl, p->zr, p->fmt, p->math == FDKHIP_MATH_FAST, s));
+		HIP_TRY(launch_back(a, p->nw, p->zl, p->zr, p->fmt, p->math == FDKHIP_MATH_FAST,
+			plan_minw(p), s));
 	}
 	if (ev)
 		HIP_TRY(hipEventRecord(ev[3], s));
@@ -1895,11 +1924,12 @@ namespace {
 const int kShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}};
 const int kPlanShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}, {4, 48}, {4, 64},
 	{8, 32}};
-const int kStages[] = {0, 3, 4, 6, 8, 12, 16};
+const int kStages[] = {0, 2, 3, 4, 6, 8, 12, 16};
 const int kTuneViews = 128;   // views per timed launch while tuning
 
 struct Tuned {
 	int nw, zl, vstage;   // zl: slices per consumer wave (the configuration's zper)
+	int minw = 4;         // kernel build: minimum waves per SIMD
 };
 
 std::mutex g_tune_mu;
@@ -1980,7 +2010,7 @@ extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const floa
 	if (reps < 1)
 		return fail(FDKHIP_EARG, "reps must be >= 1");
 	hipStream_t s = (hipStream_t) stream;
-	Tuned best{p->nw, p->zl * p->zr, p->vstage};
+	Tuned best{p->nw, p->zl * p->zr, p->vstage, plan_minw(p)};
 	float best_t = 3.4e38f;
 	// the padded views once; then the backprojection kernel alone over the
 	// first kTuneViews views (its time is proportional to the view count)
@@ -1997,25 +2027,33 @@ extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const floa
 			last_vstage = p->vstage;
 			if (p->ngroups == 0 || p->na == 0)
 				continue;
-			float t = 0.f;
-			rc = time_launches([&](hipStream_t st) {
-				BackArgs a = make_args(p, proj, scale, image);
-				a.g.na = std::min(p->na, kTuneViews);
-				HIP_TRY(launch_back(a, p->nw, p->zl, p->zr, p->fmt, p->math == FDKHIP_MATH_FAST,
-					st));
-				return FDKHIP_OK;
-			}, s, reps, &t);
-			if (rc)
-				return rc;
-			if (t < best_t) {
-				best_t = t;
-				best = Tuned{sh[0], sh[1], p->vstage};
+			// each kernel build this shape has (4 waves per SIMD; 8 for one-chunk
+			// FP16/BF16 shapes)
+			for (int mw : {4, 8}) {
+				if (!back_minw_supported(mw, p->zr, p->fmt))
+					continue;
+				p->minw = mw;
+				float t = 0.f;
+				rc = time_launches([&](hipStream_t st) {
+					BackArgs a = make_args(p, proj, scale, image);
+					a.g.na = std::min(p->na, kTuneViews);
+					HIP_TRY(launch_back(a, p->nw, p->zl, p->zr, p->fmt,
+						p->math == FDKHIP_MATH_FAST, plan_minw(p), st));
+					return FDKHIP_OK;
+				}, s, reps, &t);
+				if (rc)
+					return rc;
+				if (t < best_t) {
+					best_t = t;
+					best = Tuned{sh[0], sh[1], p->vstage, mw};
+				}
 			}
 		}
 	}
 	rc = configure(p, best.nw, best.zl, best.vstage, 0);
 	if (rc)
 		return rc;
+	p->minw = best.minw;
 	tuned_store(tune_key(p), best);
 	if (best_ms)
 		*best_ms = best_t;
@@ -2065,6 +2103,7 @@ static int apply_tuned(fdkhip_plan *p)
 	Tuned t;
 	if (!tuned_lookup(tune_key(p), t))
 		return FDKHIP_OK;
+	p->minw = t.minw;
 	return configure(p, t.nw, t.zl, t.vstage, 0);
 }
 
@@ -2400,7 +2439,7 @@ static int batch_kernel_launch(fdkhip_batch *b, hipStream_t s)
 			const int k = queued[1] < queued[0] ? 1 : 0;
 			queued[k] += p->n_updates;
 			HIP_TRY(launch_back(b->host_args[i], p->nw, p->zl, p->zr, p->fmt,
-				p->math == FDKHIP_MATH_FAST, k ? b->side : s));
+				p->math == FDKHIP_MATH_FAST, plan_minw(p), k ? b->side : s));
 		}
 		HIP_TRY(hipEventRecord(b->join_ev, b->side));
 		HIP_TRY(hipStreamWaitEvent(s, b->join_ev, 0));

@@ -111,8 +111,11 @@ size_t back_lds_bytes(int nw, int vstage, int slot, int fmt);
 // nw consumer waves, zl slices per chunk, zr chunks per wave (interleaved);
 // the survey takes the z-group's chunk count
 bool back_config_supported(int nw, int zl, int zr, int fmt);
+// minw: the kernel build's minimum waves per SIMD, 4, or 8 for one-chunk
+// (zr = 1) FP16/BF16 kernels (at most 64 VGPRs)
+bool back_minw_supported(int minw, int zr, int fmt);
 hipError_t launch_back(const BackArgs &a, int nw, int zl, int zr, int fmt, bool fast,
-	hipStream_t s);
+	int minw, hipStream_t s);
 hipError_t launch_survey(const BackArgs &a, int nchunk, int zl, bool direct, hipStream_t s);
 hipError_t launch_batch(const BatchArgs &b, int nwg, int nw, int zl, int vstage, int slot,
 	int fmt, bool fast, hipStream_t s);

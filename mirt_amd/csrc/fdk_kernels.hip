@@ -1118,12 +1118,13 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 	}
 }
 
-// 4 consumer + 4 producer waves: two workgroups share a CU; 8 + 8: one.
-// Either way 4 waves per SIMD, so at most 128 VGPRs (the second launch
-// bound is the minimum waves per SIMD).
+// The second launch bound MW is the minimum waves per SIMD: 4 (at most 128
+// VGPRs), or 8 (at most 64) for a second build of the one-chunk FP16/BF16
+// kernels, whose small LDS stages can then let four 8-wave workgroups share
+// a CU; which one runs is a plan's choice (its autotuner times both).
 // WIN: FP16/BF16 views of a helical plan (slice windows per view)
-template <int NW, int NP, int ZL, int FMT, bool FAST, int R, bool WIN>
-__global__ void __launch_bounds__((NW + NP) * 64, 4)
+template <int NW, int NP, int ZL, int FMT, bool FAST, int R, bool WIN, int MW>
+__global__ void __launch_bounds__((NW + NP) * 64, MW)
 back_kernel(const BackArgs a)
 {
 	const int nblocks = a.ntiles * a.ngroups;
@@ -1477,7 +1478,7 @@ view_quantize_kernel(const float *proj, long view_elems, int fmt,
 // ---------------------------------------------------------------------------
 // launchers (host side)
 
-template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1>
+template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1, int MW = 4>
 static hipError_t launch_back_t(const BackArgs &a, size_t lds_bytes, hipStream_t s)
 {
 	const int nblocks = a.ntiles * a.ngroups;
@@ -1486,12 +1487,12 @@ static hipError_t launch_back_t(const BackArgs &a, size_t lds_bytes, hipStream_t
 	// axial kernel's code stays as small as it is)
 	if constexpr (FMT != FMT_F32) {
 		if (a.izb) {
-			hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST, R, true>), dim3(nblocks),
+			hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST, R, true, MW>), dim3(nblocks),
 				dim3((NW + NP) * 64), lds_bytes, s, a);
 			return hipGetLastError();
 		}
 	}
-	hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST, R, false>), dim3(nblocks),
+	hipLaunchKernelGGL((back_kernel<NW, NP, ZL, FMT, FAST, R, false, MW>), dim3(nblocks),
 		dim3((NW + NP) * 64), lds_bytes, s, a);
 	return hipGetLastError();
 }
@@ -1510,13 +1511,22 @@ size_t back_lds_bytes(int nw, int vstage, int slot, int fmt)
 		constexpr int NP_ = (NW) >= 8 ? 8 : 4;                                                 \
 		if (fmt == FMT_F32 && !fast) return launch_back_t<NW, NP_, ZL, FMT_F32, false>(a, lds, s); \
 		if (fmt == FMT_F32 && fast) return launch_back_t<NW, NP_, ZL, FMT_F32, true>(a, lds, s);   \
-		if (fmt == FMT_F16) return launch_back_t<NW, NP_, ZL, FMT_F16, true>(a, lds, s);          \
-		if (fmt == FMT_BF16) return launch_back_t<NW, NP_, ZL, FMT_BF16, true>(a, lds, s);        \
+		if (fmt == FMT_F16) return minw == 8 ? launch_back_t<NW, NP_, ZL, FMT_F16, true, 1, 8>(a, lds, s) \
+			: launch_back_t<NW, NP_, ZL, FMT_F16, true>(a, lds, s);                                \
+		if (fmt == FMT_BF16) return minw == 8 ? launch_back_t<NW, NP_, ZL, FMT_BF16, true, 1, 8>(a, lds, s) \
+			: launch_back_t<NW, NP_, ZL, FMT_BF16, true>(a, lds, s);                               \
 	} while (0)
 
-hipError_t launch_back(const BackArgs &a, int nw, int zl, int zr, int fmt, bool fast,
-	hipStream_t s)
+bool back_minw_supported(int minw, int zr, int fmt)
 {
+	return minw == 4 || (minw == 8 && zr == 1 && fmt != FMT_F32);
+}
+
+hipError_t launch_back(const BackArgs &a, int nw, int zl, int zr, int fmt, bool fast,
+	int minw, hipStream_t s)
+{
+	if (!back_minw_supported(minw, zr, fmt))
+		return hipErrorInvalidValue;
 	const size_t lds = back_lds_bytes(nw * zr, a.vstage, a.slot, fmt);
 	if (zr > 1) {
 		// chunks interleaved over the waves

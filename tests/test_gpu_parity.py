@@ -229,6 +229,35 @@ def test_all_configs(device, waves, zper):
     assert O.rel_l2(img, want) <= 1e-5
 
 
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_occupancy_builds_same_bits(device, fmt):
+    """The 8-waves-per-SIMD kernel build of one-chunk FP16/BF16 shapes gives
+    the same bits as the 4-wave build (same instructions, other register
+    budget); multi-chunk and FP32 shapes keep the 4-wave build."""
+    import torch
+    from mirt_amd import FdkhipError, quantize_views
+    g, proj, beta, mask = make_case("wide_cone", seed=14, data="noise")
+    q, sc = quantize_views(torch.from_numpy(proj).to(device), fmt)
+    plan = _plan(g, beta, mask, fmt)
+    for waves, zper, vstage in ((4, 32, 2), (4, 16, 3), (8, 8, 0)):
+        plan.set_config(waves, zper, vstage)
+        plan.set_occupancy(4)
+        assert plan.occupancy == 4
+        img4 = plan(q, sc).cpu().numpy()
+        plan.set_occupancy(8)
+        assert plan.occupancy == 8
+        img8 = plan(q, sc).cpu().numpy()
+        print(f"{fmt} {waves}x{zper} vstage {vstage}: 4 vs 8 waves/SIMD equal", flush=True)
+        assert np.array_equal(img4, img8), (waves, zper, vstage)
+    plan.set_config(4, 48, 0)
+    assert plan.occupancy == 4          # no 8-wave build of multi-chunk shapes
+    f32 = _plan(g, beta, mask, "f32")
+    f32.set_occupancy(8)
+    assert f32.occupancy == 4           # nor of FP32 ones
+    with pytest.raises(FdkhipError):
+        plan.set_occupancy(6)
+
+
 @pytest.mark.parametrize("waves,zper", [(4, 16), (8, 16), (4, 8), (8, 8), (4, 32), (4, 48),
                                         (4, 64), (8, 32)])
 def test_all_configs_f16(device, waves, zper):
