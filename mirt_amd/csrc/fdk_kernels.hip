@@ -413,7 +413,10 @@ extern __shared__ float fdk_lds[];
 //               gathered from HBM: is
 //   T     [vstage][NW][64]      walk only: t_bin at each consumer wave's first
 //                               active slice (DIRECT evaluates it from U0)
-//   vinfo [vstage][4]           s_lo, t_lo, tw, (mode << 16) | staged columns
+//   vinfo [vstage][4]           walk: s_lo; DIRECT: mode | zlo << 2 | zhi << 12,
+//                               the union of the tile's slice ranges relative
+//                               to the z-group (at most 256 slices);
+//                               then t_lo, tw, (mode << 16) | staged columns
 template <int NW, bool TTAB>
 struct Lds {
 	int P, T, vinfo, total, zero;
@@ -576,9 +579,19 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 					act ? (meets ? Tk[k] : cr.first) : 0.f;
 			}
 		}
+		// DIRECT: the union of the active lanes' slice ranges, so that the
+		// consumers find the groups they work on with scalar tests (a lane
+		// is active only with a non-empty range; with none active the view's
+		// mode is MODE_NONE and the range is not read)
+		int zrange = 0;
+		if constexpr (DIRECT) {
+			const int zl = wave_min(act ? cr.zlo - zg0 : 0x7fffffff);
+			const int zh = wave_max(act ? cr.zhi - zg0 : (int) 0x80000000);
+			zrange = (mode == MODE_NONE) ? 0 : (zl << 2) | (zh << 12);
+		}
 		if (lane == 0) {
 			int *vi = reinterpret_cast<int *>(fdk_lds) + bo + L.vinfo + vv * 4;
-			vi[0] = f.s_lo;
+			vi[0] = DIRECT ? mode | zrange : f.s_lo;
 			vi[1] = f.t_lo;
 			vi[2] = tw;
 			vi[3] = (mode << 16) | (sw & 0xffff);
@@ -901,28 +914,25 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 	const GeomF &g = a.g;
 	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
 	const int2 *VI = reinterpret_cast<const int2 *>(fdk_lds + bo + L.vinfo + 2);
-	const float jc = (float) (wave * ZL);   // this wave's first slice in the z-group
-	float4 np0 = P4[0], np1 = P4[1];
-	int2 nvi = VI[0];
+	const int *ZR = reinterpret_cast<const int *>(fdk_lds + bo + L.vinfo);
+	const int zoff = wave * ZL;             // this wave's first slice in the z-group
+	const float jc = (float) zoff;
 	bool hbm_views = false, cut_views = false;
 	for (int vv = 0; vv < nv; ++vv) {
-		const float4 p0 = np0, p1 = np1;
-		const int2 vi = nvi;
-		if (vv + 1 < nv) {
-			np0 = P4[(vv + 1) * 128];
-			np1 = P4[(vv + 1) * 128 + 1];
-			nvi = VI[(vv + 1) * 2];
-		}
-		const int mode = __builtin_amdgcn_readfirstlane(vi.y) >> 16;
+		const int zr = __builtin_amdgcn_readfirstlane(ZR[vv * 4]);
+		const int mode = zr & 3;
 		hbm_views = hbm_views || mode == MODE_GLOBAL;
 		if (mode != MODE_LDS)
 			continue;
-		const int zlohi = __float_as_int(p1.y);
-		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
-		const int jlo = max(zlo - zc, 0);
-		const int jhi = min(zhi - zc, ZL);
-		if (!__any(jlo < jhi))
+		// the union of the tile's slice ranges in this wave's chunk (scalar):
+		// a group inside it where some lanes have no slices adds exactly 0
+		// for those lanes (their u is clamped onto zero rows, or the group
+		// is one the view's window cuts, added with a select below)
+		const int jlo = max(((zr >> 2) & 1023) - zoff, 0);
+		const int jhi = min(((zr >> 12) & 1023) - zoff, ZL);
+		if (jlo >= jhi)
 			continue;                  // no lane of this wave meets the view
+		const float4 p0 = P4[vv * 128], p1 = P4[vv * 128 + 1];
 		const int c1 = bo + __float_as_int(p0.x);
 		const float wl = p0.y, wr = p0.z;
 		const float ti = p1.x;
@@ -933,8 +943,7 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 		for (int g0 = 0; g0 < ZL; g0 += 8) {
 			// groups of 8 slices that no lane of the wave adds are skipped;
 			// groups the view's window cuts (a helix) are left to the pass below
-			if (g0 >= w.y || g0 + 8 <= w.x
-				|| !__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
+			if (g0 >= w.y || g0 + 8 <= w.x || g0 >= jhi || g0 + 8 <= jlo)
 				continue;
 			if (g0 < w.x || g0 + 8 > w.y) {
 				cut_views = true;
@@ -1030,6 +1039,7 @@ template <int NW, int NP, int ZL, int FMT, bool FAST, int R = 1, bool WIN = fals
 __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int group)
 {
 	static_assert(!WIN || FMT != FMT_F32, "the walk modes keep to the windows by their ranges");
+	static_assert(NW * R * ZL <= 1023, "DIRECT packs z-group-relative slice ranges in 10 bits");
 	constexpr int NC = NW * R;   // ZL-slice chunks of the z-group
 	const GeomF &g = a.g;
 	const int tid = threadIdx.x;
