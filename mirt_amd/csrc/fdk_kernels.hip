@@ -949,10 +949,18 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 				cut_views = true;
 				continue;
 			}
+			// axial: halves of 4 slices outside the union skipped as well
+			// (finer tests cost more than they save where windows cut groups)
+			constexpr int HS = WINDOW ? 8 : 4;
 #pragma unroll
-			for (int j = g0; j < g0 + 8; ++j) {
-				const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
-				acc[j] = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+			for (int h0 = g0; h0 < g0 + 8; h0 += HS) {
+				if (h0 + HS <= jlo || h0 >= jhi)
+					continue;
+#pragma unroll
+				for (int j = h0; j < h0 + HS; ++j) {
+					const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
+					acc[j] = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+				}
 			}
 		}
 	}
