@@ -413,10 +413,12 @@ extern __shared__ float fdk_lds[];
 //               gathered from HBM: is
 //   T     [vstage][NW][64]      walk only: t_bin at each consumer wave's first
 //                               active slice (DIRECT evaluates it from U0)
-//   vinfo [vstage][4]           walk: s_lo; DIRECT: mode | zlo << 2 | zhi << 12,
-//                               the union of the tile's slice ranges relative
-//                               to the z-group (at most 256 slices);
-//                               then t_lo, tw, (mode << 16) | staged columns
+//   vinfo [vstage][4]           mode | zlo << 2 | zhi << 12: the union of the
+//                               tile's slice ranges relative to the z-group
+//                               (at most 256 slices; 0 where not reduced),
+//                               ilo | ihi << 10: their intersection (walk
+//                               modes with several chunks per wave),
+//                               tw, (mode << 16) | staged columns
 template <int NW, bool TTAB>
 struct Lds {
 	int P, T, vinfo, total, zero;
@@ -518,8 +520,9 @@ __device__ __forceinline__ void stage_pairs(const uint16_t *view, const GeomF &g
 // wave `pw` of NP.  Walk modes (FP32) stage from the zero-padded copy with
 // asynchronous global-to-LDS loads, which the caller waits for (vmcnt)
 // before the stage barrier; DIRECT (FP16/BF16) stages column pairs from the
-// caller's views (stage_pairs).
-template <int NW, int NP, int ZL, int FMT, bool FAST>
+// caller's views (stage_pairs).  R: chunks per consumer wave (the walk
+// modes' consumers test slice groups against the tile's ranges with R > 1).
+template <int NW, int NP, int ZL, int FMT, bool FAST, int R>
 __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 	const Lds<NW, FMT == FMT_F32> &L,
 	int v0, int nv, int pw, int lane, bool colvalid, float xx, float yy, int zg0, int zg1)
@@ -579,20 +582,27 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 					act ? (meets ? Tk[k] : cr.first) : 0.f;
 			}
 		}
-		// DIRECT: the union of the active lanes' slice ranges, so that the
-		// consumers find the groups they work on with scalar tests (a lane
-		// is active only with a non-empty range; with none active the view's
-		// mode is MODE_NONE and the range is not read)
-		int zrange = 0;
-		if constexpr (DIRECT) {
+		// The union of the active lanes' slice ranges (DIRECT, and the walk
+		// modes with several chunks per wave, which also take the
+		// intersection of all 64 lanes' ranges), so that the consumers find
+		// the groups they work on with scalar tests.  A lane is active only
+		// with a non-empty range (an inactive one empties the intersection);
+		// with none active the view's mode is MODE_NONE and no range is read.
+		int zrange = 0, zboth = 0;
+		if constexpr (DIRECT || R > 1) {
 			const int zl = wave_min(act ? cr.zlo - zg0 : 0x7fffffff);
 			const int zh = wave_max(act ? cr.zhi - zg0 : (int) 0x80000000);
 			zrange = (mode == MODE_NONE) ? 0 : (zl << 2) | (zh << 12);
+			if constexpr (!DIRECT) {
+				const int il = min(wave_max(act ? cr.zlo - zg0 : 1023), 1023);
+				const int ih = max(wave_min(act ? cr.zhi - zg0 : 0), 0);
+				zboth = il | (ih << 10);
+			}
 		}
 		if (lane == 0) {
 			int *vi = reinterpret_cast<int *>(fdk_lds) + bo + L.vinfo + vv * 4;
-			vi[0] = DIRECT ? mode | zrange : f.s_lo;
-			vi[1] = f.t_lo;
+			vi[0] = mode | zrange;
+			vi[1] = zboth;
 			vi[2] = tw;
 			vi[3] = (mode << 16) | (sw & 0xffff);
 		}
@@ -756,16 +766,16 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 	const GeomF &g = a.g;
 	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
 	const float *Tw = fdk_lds + bo + L.T + wave * 64 + lane;
-	const int2 *VI = reinterpret_cast<const int2 *>(fdk_lds + bo + L.vinfo + 2);
+	const int4 *VI = reinterpret_cast<const int4 *>(fdk_lds + bo + L.vinfo);
 	float4 np0 = P4[0], np1 = P4[1];
 	float nts[R];
 #pragma unroll
 	for (int r = 0; r < R; ++r)
 		nts[r] = Tw[r * NW * 64];
-	int2 nvi = VI[0];
+	int4 nvi = VI[0];
 	for (int vv = 0; vv < nv; ++vv) {
 		const float4 p0 = np0, p1 = np1;
-		const int2 vi = nvi;
+		const int4 vi = nvi;
 		float ts[R];               // t_bin at the lane's first active slice of each chunk
 #pragma unroll
 		for (int r = 0; r < R; ++r)
@@ -776,21 +786,29 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 #pragma unroll
 			for (int r = 0; r < R; ++r)
 				nts[r] = Tw[((vv + 1) * NC + r * NW) * 64];
-			nvi = VI[(vv + 1) * 2];
+			nvi = VI[vv + 1];
 		}
-		const int mode = __builtin_amdgcn_readfirstlane(vi.y) >> 16;
+		const int zu = __builtin_amdgcn_readfirstlane(vi.x);
+		const int mode = zu & 3;
 		if (mode == MODE_NONE)
 			continue;
+		// R > 1: the tile's union and intersection of slice ranges (scalar)
+		const int zi = R > 1 ? __builtin_amdgcn_readfirstlane(vi.y) : 0;
 		const int zlohi = __float_as_int(p1.y);
 		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
 #pragma unroll
 		for (int r = 0; r < R; ++r) {
 			float (&acc)[ZL] = accs[r];
 			const float t0 = ts[r];
-			const int zc = zg0 + (wave + r * NW) * ZL;
+			const int zoff = (wave + r * NW) * ZL;
+			const int zc = zg0 + zoff;
 			const int jlo = max(zlo - zc, 0);
 			const int jhi = min(zhi - zc, ZL);
-			if (!__any(jlo < jhi))
+			const int ulo = max(((zu >> 2) & 1023) - zoff, 0);
+			const int uhi = min(((zu >> 12) & 1023) - zoff, ZL);
+			const int ilo = max((zi & 1023) - zoff, 0);
+			const int ihi = min(((zi >> 10) & 1023) - zoff, ZL);
+			if (R > 1 ? ulo >= uhi : !__any(jlo < jhi))
 				continue;                  // no lane of this wave meets the view here
 			const int cb = __float_as_int(p0.x);
 			const float wl = p0.y, wr = p0.z, w2 = p0.w;
@@ -798,9 +816,9 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 			const unsigned nj = (unsigned) max(jhi - jlo, 0);
 			if (mode == MODE_LDS) {
 				const int c1 = bo + cb;
-				const int tw = __builtin_amdgcn_readfirstlane(vi.x);
+				const int tw = __builtin_amdgcn_readfirstlane(vi.z);
 				const int c2 = c1 + tw;
-				if (__all(jlo == 0 && jhi == ZL)) {
+				if (R > 1 ? ilo == 0 && ihi == ZL : __all(jlo == 0 && jhi == ZL)) {
 					// every lane covers the whole chunk: straight-line slices
 					float t = t0;
 #pragma unroll
@@ -820,9 +838,10 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 					float t = t0;
 #pragma unroll
 					for (int g0 = 0; g0 < ZL; g0 += GS) {
-						if (!__any(jlo < g0 + GS && jhi > g0 && jlo < jhi))
+						if (R > 1 ? g0 >= uhi || g0 + GS <= ulo
+							: !__any(jlo < g0 + GS && jhi > g0 && jlo < jhi))
 							continue;
-						if (R > 1 && __all(jlo <= g0 && jhi >= g0 + GS)) {
+						if (R > 1 && ilo <= g0 && ihi >= g0 + GS) {
 #pragma unroll
 							for (int j = g0; j < g0 + GS; ++j) {
 								acc[j] = slice_update<FMT, FAST>(acc[j], t, c1, c2, wl, wr, w2);
@@ -1080,14 +1099,14 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 		if (FMT != FMT_F32 && pw == 0 && lane < 2)
 			fdk_lds[L.zero + lane] = 0.f;   // read by DIRECT lanes without work
 		if (nstage > 0)
-			produce<NC, NP, ZL, FMT, FAST>(a, 0, L, va, min(a.vstage, vb - va), pw, lane,
+			produce<NC, NP, ZL, FMT, FAST, R>(a, 0, L, va, min(a.vstage, vb - va), pw, lane,
 				colvalid, xx, yy, zg0, zg1);
 		__builtin_amdgcn_s_waitcnt(0);   // staged footprints have landed in LDS
 		__syncthreads();
 		for (int s = 0; s < nstage; ++s) {
 			if (s + 1 < nstage) {
 				const int v0 = va + (s + 1) * a.vstage;
-				produce<NC, NP, ZL, FMT, FAST>(a, ((s + 1) & 1) * L.total, L, v0,
+				produce<NC, NP, ZL, FMT, FAST, R>(a, ((s + 1) & 1) * L.total, L, v0,
 					min(a.vstage, vb - v0), pw, lane, colvalid, xx, yy, zg0, zg1);
 			}
 			__builtin_amdgcn_s_waitcnt(0);
