@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "fdk_kernels.h"
 
@@ -415,9 +416,9 @@ extern __shared__ float fdk_lds[];
 //                               active slice (DIRECT evaluates it from U0)
 //   vinfo [vstage][4]           mode | zlo << 2 | zhi << 12: the union of the
 //                               tile's slice ranges relative to the z-group
-//                               (at most 256 slices; 0 where not reduced),
-//                               ilo | ihi << 10: their intersection (walk
-//                               modes with several chunks per wave),
+//                               (at most 256 slices), and ilo | ihi << 10:
+//                               their intersection (walk modes with several
+//                               chunks per wave; with one: s_lo, t_lo),
 //                               tw, (mode << 16) | staged columns
 template <int NW, bool TTAB>
 struct Lds {
@@ -601,8 +602,8 @@ __device__ __forceinline__ void produce(const BackArgs &a, int bo,
 		}
 		if (lane == 0) {
 			int *vi = reinterpret_cast<int *>(fdk_lds) + bo + L.vinfo + vv * 4;
-			vi[0] = mode | zrange;
-			vi[1] = zboth;
+			vi[0] = (DIRECT || R > 1) ? mode | zrange : f.s_lo;
+			vi[1] = (DIRECT || R > 1) ? zboth : f.t_lo;
 			vi[2] = tw;
 			vi[3] = (mode << 16) | (sw & 0xffff);
 		}
@@ -766,16 +767,19 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 	const GeomF &g = a.g;
 	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
 	const float *Tw = fdk_lds + bo + L.T + wave * 64 + lane;
-	const int4 *VI = reinterpret_cast<const int4 *>(fdk_lds + bo + L.vinfo);
+	// R > 1: the view's four info words; R = 1: tw and mode (the last two)
+	using Info = std::conditional_t<(R > 1), int4, int2>;
+	constexpr int IW = R > 1 ? 1 : 2;   // Info entries per view
+	const Info *VI = reinterpret_cast<const Info *>(fdk_lds + bo + L.vinfo + (R > 1 ? 0 : 2));
 	float4 np0 = P4[0], np1 = P4[1];
 	float nts[R];
 #pragma unroll
 	for (int r = 0; r < R; ++r)
 		nts[r] = Tw[r * NW * 64];
-	int4 nvi = VI[0];
+	Info nvi = VI[0];
 	for (int vv = 0; vv < nv; ++vv) {
 		const float4 p0 = np0, p1 = np1;
-		const int4 vi = nvi;
+		const Info vi = nvi;
 		float ts[R];               // t_bin at the lane's first active slice of each chunk
 #pragma unroll
 		for (int r = 0; r < R; ++r)
@@ -786,14 +790,21 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 #pragma unroll
 			for (int r = 0; r < R; ++r)
 				nts[r] = Tw[((vv + 1) * NC + r * NW) * 64];
-			nvi = VI[vv + 1];
+			nvi = VI[(vv + 1) * IW];
 		}
-		const int zu = __builtin_amdgcn_readfirstlane(vi.x);
-		const int mode = zu & 3;
+		int zu = 0, zi = 0, mode, tw;
+		if constexpr (R > 1) {
+			// the tile's union and intersection of slice ranges (scalar)
+			zu = __builtin_amdgcn_readfirstlane(vi.x);
+			zi = __builtin_amdgcn_readfirstlane(vi.y);
+			mode = zu & 3;
+			tw = vi.z;
+		} else {
+			mode = __builtin_amdgcn_readfirstlane(vi.y) >> 16;
+			tw = vi.x;
+		}
 		if (mode == MODE_NONE)
 			continue;
-		// R > 1: the tile's union and intersection of slice ranges (scalar)
-		const int zi = R > 1 ? __builtin_amdgcn_readfirstlane(vi.y) : 0;
 		const int zlohi = __float_as_int(p1.y);
 		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
 #pragma unroll
@@ -816,8 +827,7 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 			const unsigned nj = (unsigned) max(jhi - jlo, 0);
 			if (mode == MODE_LDS) {
 				const int c1 = bo + cb;
-				const int tw = __builtin_amdgcn_readfirstlane(vi.z);
-				const int c2 = c1 + tw;
+				const int c2 = c1 + __builtin_amdgcn_readfirstlane(tw);
 				if (R > 1 ? ilo == 0 && ihi == ZL : __all(jlo == 0 && jhi == ZL)) {
 					// every lane covers the whole chunk: straight-line slices
 					float t = t0;
