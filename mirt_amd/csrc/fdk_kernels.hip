@@ -947,8 +947,11 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 	const int zoff = wave * ZL;             // this wave's first slice in the z-group
 	const float jc = (float) zoff;
 	bool hbm_views = false, cut_views = false;
+	int nzr = ZR[0];
 	for (int vv = 0; vv < nv; ++vv) {
-		const int zr = __builtin_amdgcn_readfirstlane(ZR[vv * 4]);
+		const int zr = __builtin_amdgcn_readfirstlane(nzr);
+		if (vv + 1 < nv)
+			nzr = ZR[(vv + 1) * 4];
 		const int mode = zr & 3;
 		hbm_views = hbm_views || mode == MODE_GLOBAL;
 		if (mode != MODE_LDS)
