@@ -806,6 +806,10 @@ def run_batch(args, dev, errors_without_baseline=False):
         tuned = batch.autotune(projs, scales, outs, reps=2)
         log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned:.3f} ms, "
             f"config {batch.config}")
+    if args.batch_form == "per_scan" and not batch.per_scan:
+        # each scan's plan takes the launch shape its autotune remembered
+        batch.set_per_scan()
+        log("one launch per scan (--batch-form per_scan)")
     launch = dict(batch.config)
     if batch.per_scan:
         launch = {"mode": "one launch per scan", "workgroups": launch["workgroups"],
@@ -899,6 +903,10 @@ def main():
     ap.add_argument("--nscan", type=int, default=32, help="uneven_scans: number of scans")
     ap.add_argument("--slabs", type=int, default=0, metavar="G",
                     help="also time the G z-slabs a G-GPU split would give each GPU")
+    ap.add_argument("--batch-form", choices=["auto", "per_scan"], default="auto",
+                    help="uneven_scans: the form the batch autotuner picks, or one launch "
+                         "per scan whatever it picks (profiling runs with counters, which "
+                         "serialise the two streams)")
     ap.add_argument("--errors-scans", type=int, default=4,
                     help="uneven_scans: scans checked against the FP64 oracle")
     ap.add_argument("--error-cols", type=int, default=256,
