@@ -973,8 +973,8 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 		const int2 w = view_window<ZL, WINDOW>(a, v0 + vv, zc);
 #pragma unroll
 		for (int g0 = 0; g0 < ZL; g0 += 8) {
-			// groups of 8 slices that no lane of the wave adds are skipped;
-			// groups the view's window cuts (a helix) are left to the pass below
+			// groups of 8 slices outside the union are skipped; groups the
+			// view's window cuts (a helix) are left to the pass below
 			if (g0 >= w.y || g0 + 8 <= w.x || g0 >= jhi || g0 + 8 <= jlo)
 				continue;
 			if (g0 < w.x || g0 + 8 > w.y) {
