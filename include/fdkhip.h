@@ -313,13 +313,15 @@ int fdkhip_plan_destroy(fdkhip_plan *plan);
 int fdkhip_plan_count_updates(fdkhip_plan *plan, int64_t *n_updates);
 
 /* Time every launch shape the build supports with each views-per-stage depth
- * that fits LDS (`reps` launches of the backprojection kernel each, over the
- * first 128 views, on `stream`), keep the fastest for this plan and remember
- * it for later plans of the same geometry, view count, format and math mode
- * in this process.  image_dev is overwritten (its contents are undefined
- * afterwards).  *best_ms (may be NULL) receives the best average time of a
- * 128-view launch.  Synchronous.  No counterpart in the reference, whose only
- * knob is the caller's nthread. */
+ * that fits LDS and each kernel build (`reps` launches of the backprojection
+ * kernel each, over the first 128 views, on `stream`), time the four fastest
+ * again over all views (plans of more than 128 views), keep the fastest for
+ * this plan and remember it for later plans of the same geometry, view
+ * count, format and math mode in this process.  image_dev is overwritten
+ * (its contents are undefined afterwards).  *best_ms (may be NULL) receives
+ * the chosen launch's average time over the views of the last round.
+ * Synchronous.  No counterpart in the reference, whose only knob is the
+ * caller's nthread. */
 int fdkhip_plan_autotune(fdkhip_plan *plan, const void *proj_dev,
 	const float *scale_dev, float *image_dev, void *stream, int32_t reps,
 	float *best_ms);

@@ -510,10 +510,12 @@ class FdkPlan:
         return out
 
     def autotune(self, proj, scale=None, out=None, *, reps=3, stream=None):
-        """Time the supported launch shapes and stage depths on these buffers
-        (the backprojection kernel over the first 128 views) and keep the
-        fastest (fdkhip_plan_autotune); returns the best average milliseconds
-        of such a launch.  ``out`` (or a scratch image) is overwritten."""
+        """Time the supported launch shapes, stage depths and kernel builds on
+        these buffers (the backprojection kernel over the first 128 views,
+        then the four fastest over all views) and keep the fastest
+        (fdkhip_plan_autotune); returns its average milliseconds over the
+        views of the last round.  ``out`` (or a scratch image) is
+        overwritten."""
         import torch
         self._check_proj(proj, scale)
         g = self.geom

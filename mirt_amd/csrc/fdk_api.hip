@@ -1926,6 +1926,7 @@ const int kPlanShapes[][2] = {{8, 16}, {8, 8}, {4, 16}, {4, 32}, {4, 8}, {4, 48}
 	{8, 32}};
 const int kStages[] = {0, 2, 3, 4, 6, 8, 12, 16};
 const int kTuneViews = 128;   // views per timed launch while tuning
+const int kTuneFinalists = 4; // fastest candidates timed again over all views
 
 struct Tuned {
 	int nw, zl, vstage;   // zl: slices per consumer wave (the configuration's zper)
@@ -2017,6 +2018,7 @@ extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const floa
 	rc = plan_launch(p, proj, scale, image, s);
 	if (rc)
 		return rc;
+	std::vector<std::pair<float, Tuned>> cand;
 	for (const auto &sh : kPlanShapes) {
 		int last_vstage = -1;
 		for (int vs : kStages) {
@@ -2043,10 +2045,40 @@ extern "C" int fdkhip_plan_autotune(fdkhip_plan *p, const void *proj, const floa
 				}, s, reps, &t);
 				if (rc)
 					return rc;
+				cand.emplace_back(t, Tuned{sh[0], sh[1], p->vstage, mw});
 				if (t < best_t) {
 					best_t = t;
 					best = Tuned{sh[0], sh[1], p->vstage, mw};
 				}
+			}
+		}
+	}
+	if (p->na > kTuneViews && cand.size() > 1) {
+		// the fastest few again over all views: a prefix can rank shapes
+		// whose start-up and per-view costs differ in another order
+		std::sort(cand.begin(), cand.end(),
+			[](const std::pair<float, Tuned> &x, const std::pair<float, Tuned> &y) {
+				return x.first < y.first;
+			});
+		cand.resize(std::min(cand.size(), (size_t) kTuneFinalists));
+		best_t = 3.4e38f;
+		for (const auto &c : cand) {
+			rc = configure(p, c.second.nw, c.second.zl, c.second.vstage, 0);
+			if (rc)
+				return rc;
+			p->minw = c.second.minw;
+			float t = 0.f;
+			rc = time_launches([&](hipStream_t st) {
+				const BackArgs a = make_args(p, proj, scale, image);
+				HIP_TRY(launch_back(a, p->nw, p->zl, p->zr, p->fmt,
+					p->math == FDKHIP_MATH_FAST, plan_minw(p), st));
+				return FDKHIP_OK;
+			}, s, reps, &t);
+			if (rc)
+				return rc;
+			if (t < best_t) {
+				best_t = t;
+				best = c.second;
 			}
 		}
 	}

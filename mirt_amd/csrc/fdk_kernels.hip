@@ -988,10 +988,32 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 			for (int h0 = g0; h0 < g0 + 8; h0 += HS) {
 				if (h0 + HS <= jlo || h0 >= jhi)
 					continue;
+				if constexpr (!WINDOW) {
+					// the half's four LDS reads first, then its arithmetic
+					// (a helical plan's 8-slice groups would need the
+					// registers of eight reads in flight)
+					uint32_t w0[HS], w1[HS];
+					float wu[HS];
 #pragma unroll
-				for (int j = h0; j < h0 + HS; ++j) {
-					const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
-					acc[j] = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+					for (int k = 0; k < HS; ++k) {
+						const float u = __builtin_amdgcn_fmed3f(fmaf((float) (h0 + k), ti, t0),
+							0.f, uh);
+						const int iu = (int) u;
+						wu[k] = __builtin_amdgcn_fractf(u);
+						w0[k] = __float_as_uint(fdk_lds[c1 + iu]);
+						w1[k] = __float_as_uint(fdk_lds[c1 + iu + 1]);
+					}
+#pragma unroll
+					for (int k = 0; k < HS; ++k)
+						acc[h0 + k] = bilerp_d(acc[h0 + k], pair_lo<FMT>(w0[k]),
+							pair_hi<FMT>(w0[k]), pair_lo<FMT>(w1[k]), pair_hi<FMT>(w1[k]),
+							wl, wr, wu[k]);
+				} else {
+#pragma unroll
+					for (int j = h0; j < h0 + HS; ++j) {
+						const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
+						acc[j] = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+					}
 				}
 			}
 		}
