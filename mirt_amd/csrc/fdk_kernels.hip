@@ -926,26 +926,36 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 // place in registers).  The slices outside the window
 // are not on the clamp's zero rows when the window, not the cone, ends a
 // lane's range, so they must not be added.
-template <int ZL, bool WINDOW>
-__device__ __forceinline__ int2 view_window(const BackArgs &a, int v, int zc)
+// The view's slice window [izb0, izb1) (a helix; wave-uniform), else all
+// slices.
+template <bool WINDOW>
+__device__ __forceinline__ int2 view_window(const BackArgs &a, int v)
 {
 	if (!WINDOW)
-		return make_int2(0, ZL);
+		return make_int2((int) 0x80000000, 0x7fffffff);
 	const int2 w = a.izb[v];
-	return make_int2(__builtin_amdgcn_readfirstlane(max(w.x - zc, 0)),
-		__builtin_amdgcn_readfirstlane(min(w.y - zc, ZL)));
+	return make_int2(__builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y));
 }
 
-template <int NW, int ZL, int FMT, bool WINDOW>
-__device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const Lds<NW, false> &L,
-	int v0, int nv, int wave, int lane, int zc, float (&acc)[ZL])
+// The R chunks of consumer wave `wave` (of NCW) are chunks wave + r NCW of the
+// z-group, ZL slices each, with accumulators accs[r]; one pass over the
+// stage's views adds all of them (each view's record and window read once).
+template <int NCW, int R, int ZL, int FMT, bool WINDOW>
+__device__ __forceinline__ void consume_direct(const BackArgs &a, int bo,
+	const Lds<NCW * R, false> &L, int v0, int nv, int wave, int lane, int zg0,
+	float (&accs)[R][ZL])
 {
 	const GeomF &g = a.g;
 	const float4 *P4 = reinterpret_cast<const float4 *>(fdk_lds + bo + L.P) + lane * 2;
 	const int2 *VI = reinterpret_cast<const int2 *>(fdk_lds + bo + L.vinfo + 2);
 	const int *ZR = reinterpret_cast<const int *>(fdk_lds + bo + L.vinfo);
-	const int zoff = wave * ZL;             // this wave's first slice in the z-group
-	const float jc = (float) zoff;
+	// relative to the z-group: chunk r starts at slice zoff(r)
+	auto zoff = [&](int r) { return (wave + r * NCW) * ZL; };
+	// window of chunk r relative to its first slice, clamped to [0, ZL]
+	auto window = [&](int2 wa, int r) {
+		const int zc = zg0 + zoff(r);
+		return WINDOW ? make_int2(max(wa.x - zc, 0), min(wa.y - zc, ZL)) : make_int2(0, ZL);
+	};
 	bool hbm_views = false, cut_views = false;
 	int nzr = ZR[0];
 	for (int vv = 0; vv < nv; ++vv) {
@@ -956,63 +966,76 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 		hbm_views = hbm_views || mode == MODE_GLOBAL;
 		if (mode != MODE_LDS)
 			continue;
-		// the union of the tile's slice ranges in this wave's chunk (scalar):
-		// a group inside it where some lanes have no slices adds exactly 0
-		// for those lanes (their u is clamped onto zero rows, or the group
-		// is one the view's window cuts, added with a select below)
-		const int jlo = max(((zr >> 2) & 1023) - zoff, 0);
-		const int jhi = min(((zr >> 12) & 1023) - zoff, ZL);
-		if (jlo >= jhi)
+		// the union of the tile's slice ranges (scalar): a group inside it
+		// where some lanes have no slices adds exactly 0 for those lanes
+		// (their u is clamped onto zero rows, or the group is one the view's
+		// window cuts, added with a select below)
+		const int ulo = (zr >> 2) & 1023, uhi = (zr >> 12) & 1023;
+		bool meets = false;
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			meets = meets || (ulo < zoff(r) + ZL && uhi > zoff(r));
+		if (!meets)
 			continue;                  // no lane of this wave meets the view
 		const float4 p0 = P4[vv * 128], p1 = P4[vv * 128 + 1];
 		const int c1 = bo + __float_as_int(p0.x);
 		const float wl = p0.y, wr = p0.z;
 		const float ti = p1.x;
-		const float t0 = fmaf(jc, ti, p0.w);   // u at this wave's first slice
 		const float uh = p1.z;
-		const int2 w = view_window<ZL, WINDOW>(a, v0 + vv, zc);
+		const int2 wa = view_window<WINDOW>(a, v0 + vv);
 #pragma unroll
-		for (int g0 = 0; g0 < ZL; g0 += 8) {
-			// groups of 8 slices outside the union are skipped; groups the
-			// view's window cuts (a helix) are left to the pass below
-			if (g0 >= w.y || g0 + 8 <= w.x || g0 >= jhi || g0 + 8 <= jlo)
+		for (int r = 0; r < R; ++r) {
+			float (&acc)[ZL] = accs[r];
+			const int jlo = max(ulo - zoff(r), 0);
+			const int jhi = min(uhi - zoff(r), ZL);
+			if (jlo >= jhi)
 				continue;
-			if (g0 < w.x || g0 + 8 > w.y) {
-				cut_views = true;
-				continue;
-			}
-			// axial: halves of 4 slices outside the union skipped as well
-			// (finer tests cost more than they save where windows cut groups)
-			constexpr int HS = WINDOW ? 8 : 4;
+			const float t0 = fmaf((float) zoff(r), ti, p0.w);   // u at the chunk's first slice
+			const int2 w = window(wa, r);
 #pragma unroll
-			for (int h0 = g0; h0 < g0 + 8; h0 += HS) {
-				if (h0 + HS <= jlo || h0 >= jhi)
+			for (int g0 = 0; g0 < ZL; g0 += 8) {
+				// groups of 8 slices outside the union are skipped; groups the
+				// view's window cuts (a helix) are left to the pass below
+				if (g0 >= w.y || g0 + 8 <= w.x || g0 >= jhi || g0 + 8 <= jlo)
 					continue;
-				if constexpr (!WINDOW) {
-					// the half's four LDS reads first, then its arithmetic
-					// (a helical plan's 8-slice groups would need the
-					// registers of eight reads in flight)
-					uint32_t w0[HS], w1[HS];
-					float wu[HS];
+				if (g0 < w.x || g0 + 8 > w.y) {
+					cut_views = true;
+					continue;
+				}
+				// axial: halves of 4 slices outside the union skipped as well
+				// (finer tests cost more than they save where windows cut groups)
+				constexpr int HS = WINDOW ? 8 : 4;
 #pragma unroll
-					for (int k = 0; k < HS; ++k) {
-						const float u = __builtin_amdgcn_fmed3f(fmaf((float) (h0 + k), ti, t0),
-							0.f, uh);
-						const int iu = (int) u;
-						wu[k] = __builtin_amdgcn_fractf(u);
-						w0[k] = __float_as_uint(fdk_lds[c1 + iu]);
-						w1[k] = __float_as_uint(fdk_lds[c1 + iu + 1]);
-					}
+				for (int h0 = g0; h0 < g0 + 8; h0 += HS) {
+					if (h0 + HS <= jlo || h0 >= jhi)
+						continue;
+					if constexpr (!WINDOW) {
+						// the half's four LDS reads first, then its arithmetic
+						// (a helical plan's 8-slice groups would need the
+						// registers of eight reads in flight)
+						uint32_t w0[HS], w1[HS];
+						float wu[HS];
 #pragma unroll
-					for (int k = 0; k < HS; ++k)
-						acc[h0 + k] = bilerp_d(acc[h0 + k], pair_lo<FMT>(w0[k]),
-							pair_hi<FMT>(w0[k]), pair_lo<FMT>(w1[k]), pair_hi<FMT>(w1[k]),
-							wl, wr, wu[k]);
-				} else {
+						for (int k = 0; k < HS; ++k) {
+							const float u = __builtin_amdgcn_fmed3f(
+								fmaf((float) (h0 + k), ti, t0), 0.f, uh);
+							const int iu = (int) u;
+							wu[k] = __builtin_amdgcn_fractf(u);
+							w0[k] = __float_as_uint(fdk_lds[c1 + iu]);
+							w1[k] = __float_as_uint(fdk_lds[c1 + iu + 1]);
+						}
 #pragma unroll
-					for (int j = h0; j < h0 + HS; ++j) {
-						const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
-						acc[j] = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+						for (int k = 0; k < HS; ++k)
+							acc[h0 + k] = bilerp_d(acc[h0 + k], pair_lo<FMT>(w0[k]),
+								pair_hi<FMT>(w0[k]), pair_lo<FMT>(w1[k]),
+								pair_hi<FMT>(w1[k]), wl, wr, wu[k]);
+					} else {
+#pragma unroll
+						for (int j = h0; j < h0 + HS; ++j) {
+							const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0),
+								0.f, uh);
+							acc[j] = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+						}
 					}
 				}
 			}
@@ -1022,32 +1045,44 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 		// groups of 8 slices that a view's window cuts: the slices outside the
 		// window keep their sums (a select on a wave-uniform condition)
 		for (int vv = 0; vv < nv; ++vv) {
-			const int2 vi = VI[vv * 2];
-			if ((__builtin_amdgcn_readfirstlane(vi.y) >> 16) != MODE_LDS)
+			const int zr = __builtin_amdgcn_readfirstlane(ZR[vv * 4]);
+			if ((zr & 3) != MODE_LDS)
 				continue;
-			const int2 w = view_window<ZL, WINDOW>(a, v0 + vv, zc);
-			if (w.x <= 0 && w.y >= ZL)
+			const int2 wa = view_window<WINDOW>(a, v0 + vv);
+			bool cut = false;
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				const int2 w = window(wa, r);
+				cut = cut || !(w.x <= 0 && w.y >= ZL);
+			}
+			if (!cut)
 				continue;
+			const int ulo = (zr >> 2) & 1023, uhi = (zr >> 12) & 1023;
 			const float4 p0 = P4[vv * 128], p1 = P4[vv * 128 + 1];
-			const int zlohi = __float_as_int(p1.y);
-			const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
-			const int jlo = max(zlo - zc, 0);
-			const int jhi = min(zhi - zc, ZL);
 			const int c1 = bo + __float_as_int(p0.x);
 			const float wl = p0.y, wr = p0.z;
 			const float ti = p1.x;
-			const float t0 = fmaf(jc, ti, p0.w);
 			const float uh = p1.z;
 #pragma unroll
-			for (int g0 = 0; g0 < ZL; g0 += 8) {
-				if (g0 >= w.y || g0 + 8 <= w.x || (w.x <= g0 && g0 + 8 <= w.y)
-					|| !__any(jlo < g0 + 8 && jhi > g0 && jlo < jhi))
+			for (int r = 0; r < R; ++r) {
+				float (&acc)[ZL] = accs[r];
+				const int jlo = max(ulo - zoff(r), 0);
+				const int jhi = min(uhi - zoff(r), ZL);
+				const int2 w = window(wa, r);
+				if (jlo >= jhi || (w.x <= 0 && w.y >= ZL))
 					continue;
+				const float t0 = fmaf((float) zoff(r), ti, p0.w);
 #pragma unroll
-				for (int j = g0; j < g0 + 8; ++j) {
-					const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
-					const float up = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
-					acc[j] = (j >= w.x && j < w.y) ? up : acc[j];
+				for (int g0 = 0; g0 < ZL; g0 += 8) {
+					if (g0 >= w.y || g0 + 8 <= w.x || (w.x <= g0 && g0 + 8 <= w.y)
+						|| g0 >= jhi || g0 + 8 <= jlo)
+						continue;
+#pragma unroll
+					for (int j = g0; j < g0 + 8; ++j) {
+						const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
+						const float up = slice_update_d<FMT>(acc[j], u, c1, wl, wr);
+						acc[j] = (j >= w.x && j < w.y) ? up : acc[j];
+					}
 				}
 			}
 		}
@@ -1064,31 +1099,37 @@ __device__ __forceinline__ void consume_direct(const BackArgs &a, int bo, const 
 		const float4 p0 = P4[vv * 128], p1 = P4[vv * 128 + 1];
 		const int zlohi = __float_as_int(p1.y);
 		const int zlo = zlohi & 0xffff, zhi = (int) ((unsigned) zlohi >> 16);
-		const int jlo = max(zlo - zc, 0);
-		const int jhi = min(zhi - zc, ZL);
-		if (!__any(jlo < jhi))
-			continue;
-		const int is = (jlo < jhi) ? __float_as_int(p0.x) : -3;   // the record holds is
 		const float wl = p0.y, wr = p0.z;
 		const float ti = p1.x;
-		const float t0 = fmaf(jc, ti, p0.w);
 		const float uh = p1.z;
 		const uint16_t *pv = static_cast<const uint16_t *>(a.proj)
 			+ (size_t) (v0 + vv) * g.ns * g.nt;
-		const int2 w = view_window<ZL, WINDOW>(a, v0 + vv, zc);
+		const int2 wa = view_window<WINDOW>(a, v0 + vv);
 #pragma unroll
-		for (int j = 0; j < ZL; ++j) {
-			const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
-			const int iu = (int) u;
-			const float wu = __builtin_amdgcn_fractf(u);
-			const float up = bilerp_d(acc[j],
-				sample16<FMT>(padded16(pv, g, is + 1, iu)),
-				sample16<FMT>(padded16(pv, g, is + 2, iu)),
-				sample16<FMT>(padded16(pv, g, is + 1, iu + 1)),
-				sample16<FMT>(padded16(pv, g, is + 2, iu + 1)), wl, wr, wu);
-			acc[j] = (j >= w.x && j < w.y) ? up : acc[j];
-			// one slice's four gathers at a time (register budget)
-			__builtin_amdgcn_sched_barrier(0);
+		for (int r = 0; r < R; ++r) {
+			float (&acc)[ZL] = accs[r];
+			const int zc = zg0 + zoff(r);
+			const int jlo = max(zlo - zc, 0);
+			const int jhi = min(zhi - zc, ZL);
+			if (!__any(jlo < jhi))
+				continue;
+			const int is = (jlo < jhi) ? __float_as_int(p0.x) : -3;   // the record holds is
+			const float t0 = fmaf((float) zoff(r), ti, p0.w);
+			const int2 w = window(wa, r);
+#pragma unroll
+			for (int j = 0; j < ZL; ++j) {
+				const float u = __builtin_amdgcn_fmed3f(fmaf((float) j, ti, t0), 0.f, uh);
+				const int iu = (int) u;
+				const float wu = __builtin_amdgcn_fractf(u);
+				const float up = bilerp_d(acc[j],
+					sample16<FMT>(padded16(pv, g, is + 1, iu)),
+					sample16<FMT>(padded16(pv, g, is + 2, iu)),
+					sample16<FMT>(padded16(pv, g, is + 1, iu + 1)),
+					sample16<FMT>(padded16(pv, g, is + 2, iu + 1)), wl, wr, wu);
+				acc[j] = (j >= w.x && j < w.y) ? up : acc[j];
+				// one slice's four gathers at a time (register budget)
+				__builtin_amdgcn_sched_barrier(0);
+			}
 		}
 	}
 }
@@ -1163,13 +1204,10 @@ __device__ __forceinline__ void tile_pipeline(const BackArgs &a, int tile, int g
 				consume_walk<NW, R, ZL, FMT, FAST>(a, (s & 1) * L.total, L, v0,
 					min(a.vstage, vb - v0), wave, lane, zg0, accs);
 			else {
-				// chunk wave + r NW of the z-group, one pass over the stage's
-				// views each (u is in closed form from the group's U0)
-#pragma unroll
-				for (int r = 0; r < R; ++r)
-					consume_direct<NC, ZL, FMT, WIN>(a, (s & 1) * L.total, L, v0,
-						min(a.vstage, vb - v0), wave + r * NW, lane,
-						zg0 + (wave + r * NW) * ZL, accs[r]);
+				// chunks wave + r NW of the z-group in one pass over the
+				// stage's views (u is in closed form from the group's U0)
+				consume_direct<NW, R, ZL, FMT, WIN>(a, (s & 1) * L.total, L, v0,
+					min(a.vstage, vb - v0), wave, lane, zg0, accs);
 			}
 			__syncthreads();
 		}
