@@ -911,6 +911,17 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 	}
 }
 
+// The view's slice window [izb0, izb1) (a helix; wave-uniform), else all
+// slices.
+template <bool WINDOW>
+__device__ __forceinline__ int2 view_window(const BackArgs &a, int v)
+{
+	if (!WINDOW)
+		return make_int2((int) 0x80000000, 0x7fffffff);
+	const int2 w = a.izb[v];
+	return make_int2(__builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y));
+}
+
 // Consumer side of DIRECT (FP16/BF16 storage): the views staged in LDS
 // first, in view order, then -- after the stage's LDS views, which only
 // changes the FP32 summation order -- any view whose footprint exceeded the
@@ -926,17 +937,6 @@ __device__ __forceinline__ void consume_walk(const BackArgs &a, int bo,
 // place in registers).  The slices outside the window
 // are not on the clamp's zero rows when the window, not the cone, ends a
 // lane's range, so they must not be added.
-// The view's slice window [izb0, izb1) (a helix; wave-uniform), else all
-// slices.
-template <bool WINDOW>
-__device__ __forceinline__ int2 view_window(const BackArgs &a, int v)
-{
-	if (!WINDOW)
-		return make_int2((int) 0x80000000, 0x7fffffff);
-	const int2 w = a.izb[v];
-	return make_int2(__builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y));
-}
-
 // The R chunks of consumer wave `wave` (of NCW) are chunks wave + r NCW of the
 // z-group, ZL slices each, with accumulators accs[r]; one pass over the
 // stage's views adds all of them (each view's record and window read once).
